@@ -1,0 +1,426 @@
+// extern "C" entry points of include/exhip.h.  Each call validates its
+// arguments, dispatches on precision and converts exceptions into status
+// codes; no entry point exits the process.
+#include <cstring>
+
+#include "common.hpp"
+#include "host_tables.hpp"
+#include "kernels.hpp"
+
+namespace exh {
+const char* last_error();
+}
+
+using namespace exh;
+
+#define EXH_API_BEGIN try {
+#define EXH_API_END                                   \
+  }                                                   \
+  catch (const exh::Error& e) {                       \
+    return e.code;                                    \
+  }                                                   \
+  catch (const std::exception& e) {                   \
+    exh::set_error("internal error: %s", e.what());   \
+    return EXH_ERR_INTERNAL;                          \
+  }                                                   \
+  return EXH_OK;
+
+static void check_ctx(exh_ctx* c) {
+  EXH_REQUIRE(c != nullptr, EXH_ERR_ARG, "null context");
+  EXH_CHECK_HIP(hipSetDevice(c->device));
+}
+
+static void check_box(const exh_box* b) {
+  EXH_REQUIRE(b != nullptr, EXH_ERR_ARG, "null box");
+  EXH_REQUIRE(b->precision == 4 || b->precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
+  EXH_REQUIRE(b->nd[0] >= 2 && b->nd[1] >= 2 && b->nd[2] >= 2, EXH_ERR_ARG, "grid too small");
+  EXH_REQUIRE(b->nd[0] % 2 == 0 && b->nd[1] % 2 == 0 && b->nd[2] % 2 == 0, EXH_ERR_ARG,
+              "grid sizes must be even");
+}
+
+static void check_src(const exh_field_src* s) {
+  EXH_REQUIRE(s != nullptr && s->data != nullptr, EXH_ERR_ARG, "null field source");
+  EXH_REQUIRE(s->kind == EXH_SRC_MODES || s->kind == EXH_SRC_DELTA || s->kind == EXH_SRC_DELTAK,
+              EXH_ERR_ARG, "unknown field source kind %d", s->kind);
+  if (s->kind == EXH_SRC_MODES)
+    EXH_REQUIRE(s->k && s->P && s->Nk >= 3, EXH_ERR_ARG, "modes need a P(k) table (Nk >= 3)");
+}
+
+static void check_halo_opts(const exh_halo_opts* o) {
+  EXH_REQUIRE(o != nullptr, EXH_ERR_ARG, "null halo options");
+  EXH_REQUIRE(o->k && o->P && o->Nk >= 3, EXH_ERR_ARG, "halo finder needs P(k) (Nk >= 3)");
+}
+
+extern "C" {
+
+const char* exh_version(void) { return "exshalos-amd 0.1 (gfx950)"; }
+
+const char* exh_last_error(void) { return exh::last_error(); }
+
+int exh_ctx_create(int device, exh_ctx** out) {
+  EXH_API_BEGIN
+  EXH_REQUIRE(out != nullptr, EXH_ERR_ARG, "null output");
+  int n = 0;
+  EXH_CHECK_HIP(hipGetDeviceCount(&n));
+  EXH_REQUIRE(device >= 0 && device < n, EXH_ERR_ARG, "device %d not present (%d devices)", device, n);
+  EXH_CHECK_HIP(hipSetDevice(device));
+  exh_ctx* c = new exh_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    EXH_CHECK_HIP(e);
+  }
+  c->own_stream = true;
+  *out = c;
+  EXH_API_END
+}
+
+int exh_ctx_destroy(exh_ctx* c) {
+  EXH_API_BEGIN
+  if (!c) return EXH_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  fft_release(c);
+  for (int s = 0; s < BUF_NSLOTS; s++)
+    if (c->buf[s]) (void)hipFree(c->buf[s]);
+  for (auto& t : c->events) {
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  EXH_API_END
+}
+
+int exh_ctx_set_stream(exh_ctx* c, void* stream) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  if (stream) {
+    if (c->own_stream) EXH_CHECK_HIP(hipStreamDestroy(c->stream));
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+  } else if (!c->own_stream) {
+    EXH_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+  }
+  EXH_API_END
+}
+
+int exh_ctx_synchronize(exh_ctx* c) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
+int exh_ctx_set_timing(exh_ctx* c, int enable) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  for (auto& t : c->events) {
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  c->events.clear();
+  c->timing = enable != 0;
+  EXH_API_END
+}
+
+int exh_ctx_timings(exh_ctx* c, char* names, int names_len, double* ms, int max) {
+  int n = 0;
+  try {
+    check_ctx(c);
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    std::string all;
+    for (auto& t : c->events) {
+      if (n >= max) break;
+      float v = 0;
+      EXH_CHECK_HIP(hipEventElapsedTime(&v, t.start, t.stop));
+      if (ms) ms[n] = v;
+      all += t.name;
+      all += '\n';
+      n++;
+    }
+    if (names && names_len > 0) {
+      std::strncpy(names, all.c_str(), names_len - 1);
+      names[names_len - 1] = 0;
+    }
+  } catch (const exh::Error& e) {
+    return e.code;
+  }
+  return n;
+}
+
+int exh_ctx_halo_stats(exh_ctx* c, long long* out4) {
+  EXH_API_BEGIN
+  EXH_REQUIRE(c && out4, EXH_ERR_ARG, "null argument");
+  out4[0] = c->stat_peaks;
+  out4[1] = c->stat_candidates;
+  out4[2] = c->stat_rounds;
+  out4[3] = c->stat_sort_levels;
+  EXH_API_END
+}
+
+int exh_box_init(exh_box* box, int ndx, int ndy, int ndz, double Lc_or_Mc, double Om0,
+                 double redshift, double dc, int precision) {
+  EXH_API_BEGIN
+  EXH_REQUIRE(box != nullptr, EXH_ERR_ARG, "null box");
+  EXH_REQUIRE(precision == 4 || precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
+  EXH_REQUIRE(ndx > 0 && ndy > 0 && ndz > 0, EXH_ERR_ARG, "grid sizes must be positive");
+  int rc = box_init(box, ndx, ndy, ndz, Lc_or_Mc, Om0, redshift, dc, precision);
+  EXH_REQUIRE(rc == 0, EXH_ERR_ARG,
+              "Lc/Mc = %g: a cell larger than 100 Mpc/h or lighter than 1e5 Msun/h is not "
+              "supported",
+              Lc_or_Mc);
+  EXH_API_END
+}
+
+int exh_density_grid(exh_ctx* c, const exh_box* b, const exh_field_src* modes, double k_smooth,
+                     void* delta, void* deltak) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  check_box(b);
+  check_src(modes);
+  EXH_REQUIRE(modes->kind == EXH_SRC_MODES, EXH_ERR_ARG, "density grid needs white-noise modes");
+  EXH_REQUIRE(delta != nullptr, EXH_ERR_ARG, "null delta");
+  if (b->precision == 4)
+    launch_field<float>(c, b, modes, k_smooth, delta, deltak);
+  else
+    launch_field<double>(c, b, modes, k_smooth, delta, deltak);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
+int exh_halo_tables(const exh_box* b, const exh_halo_opts* o, void* sig_grid, void* barrier,
+                    int nB, double* Mknots, double* Int, int* Ncells) {
+  EXH_API_BEGIN
+  check_box(b);
+  check_halo_opts(o);
+  EXH_REQUIRE(sig_grid && barrier && Mknots && Int && nB >= 1, EXH_ERR_ARG, "null table");
+  int nc = halo_tables(b, o, sig_grid, barrier, nB, Mknots, Int);
+  EXH_REQUIRE(nc >= 0, EXH_ERR_RANGE, "sigma(M) evaluated outside its knots");
+  if (Ncells) *Ncells = nc;
+  EXH_API_END
+}
+
+int exh_find_halos(exh_ctx* c, const exh_box* b, const void* delta, const exh_halo_opts* o,
+                   long long* nh) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  check_box(b);
+  check_halo_opts(o);
+  EXH_REQUIRE(delta != nullptr, EXH_ERR_ARG, "null delta");
+  long long n = b->precision == 4 ? run_find_halos<float>(c, b, (const float*)delta, o)
+                                  : run_find_halos<double>(c, b, (const double*)delta, o);
+  if (b->precision == 4)
+    run_halo_posh<float>(c, b, nullptr, nullptr, 1);
+  else
+    run_halo_posh<double>(c, b, nullptr, nullptr, 1);
+  c->halo.have_posh = true;
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  if (nh) *nh = n;
+  EXH_API_END
+}
+
+static void copy_out(exh_ctx* c, void* dst, const void* src, size_t bytes, int to_host) {
+  if (!dst || bytes == 0) return;
+  EXH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes,
+                               to_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                               c->stream));
+}
+
+int exh_halos_get(exh_ctx* c, int* x, int* count, int* r2, void* Mh, int to_host) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  size_t n = (size_t)c->nh;
+  size_t es = c->last_prec == 4 ? 4 : 8;
+  copy_out(c, x, c->halo.hx, n * 12, to_host);
+  copy_out(c, count, c->halo.hcount, n * 4, to_host);
+  copy_out(c, r2, c->halo.hr2, n * 4, to_host);
+  copy_out(c, Mh, c->halo.Mh, n * es, to_host);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
+int exh_halos_posh(exh_ctx* c, void* posh, void* velh, int to_host) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  size_t n = (size_t)c->nh;
+  size_t es = c->last_prec == 4 ? 4 : 8;
+  EXH_REQUIRE(c->halo.have_posh || n == 0, EXH_ERR_ARG, "no halo positions computed");
+  EXH_REQUIRE(!velh || c->halo.have_velh || n == 0, EXH_ERR_ARG, "no halo velocities computed");
+  copy_out(c, posh, c->halo.posh, n * 3 * es, to_host);
+  copy_out(c, velh, c->halo.velh, n * 3 * es, to_host);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
+int exh_halos_flag(exh_ctx* c, long long* flag, int to_host) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_REQUIRE(flag != nullptr && c->halo.flag != nullptr, EXH_ERR_ARG, "no flag grid");
+  exh_box b{};
+  b.ng = c->halo.ng;
+  long long* dst = flag;
+  if (to_host) dst = (long long*)ctx_buf(c, BUF_R1, (size_t)b.ng * 8);
+  run_halo_flag(c, &b, dst);
+  if (to_host)
+    EXH_CHECK_HIP(hipMemcpyAsync(flag, dst, (size_t)b.ng * 8, hipMemcpyDeviceToHost, c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
+int exh_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth, int do_2lpt,
+            int out_pos, void* S, void* V) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  check_box(b);
+  check_src(s);
+  EXH_REQUIRE(S != nullptr, EXH_ERR_ARG, "null displacement output");
+  EXH_REQUIRE(b->nd[0] == b->nd[1] && b->nd[1] == b->nd[2], EXH_ERR_UNSUPPORTED,
+              "LPT needs a cubic grid (the reference's y stencil indexes rows with nd[0], "
+              "lpt.c:66)");
+  if (b->precision == 4)
+    launch_lpt<float>(c, b, s, k_smooth, do_2lpt, out_pos, (float*)S, (float*)V);
+  else
+    launch_lpt<double>(c, b, s, k_smooth, do_2lpt, out_pos, (double*)S, (double*)V);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
+}  // extern "C"
+
+template <typename T>
+static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* modes,
+                          const exh_halo_opts* o, double k_smooth, int do_2lpt, int out_vel,
+                          void* delta_out, void* S_out, void* V_out) {
+  const size_t ng = (size_t)b->ng;
+  T* delta = delta_out ? (T*)delta_out : (T*)ctx_buf(c, BUF_R1, ng * sizeof(T));
+  launch_field<T>(c, b, modes, 1e5, delta, nullptr);
+  long long nh = run_find_halos<T>(c, b, delta, o);
+  T* S = S_out ? (T*)S_out : (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
+  T* V = out_vel ? (V_out ? (T*)V_out : (T*)ctx_buf(c, BUF_V, ng * 3 * sizeof(T))) : nullptr;
+  // the field's BUF_R1 copy may be reused by the 2LPT source from here on
+  launch_lpt<T>(c, b, modes, k_smooth, do_2lpt, 0, S, V);
+  run_halo_posh<T>(c, b, S, V, 0);
+  c->halo.have_posh = true;
+  c->halo.have_velh = V != nullptr;
+  if (S_out) launch_positions<T>(c, b, S);
+  return nh;
+}
+
+extern "C" int exh_halos_box_from_pk(exh_ctx* c, const exh_box* b, const exh_field_src* modes,
+                          const exh_halo_opts* o, double k_smooth, int do_2lpt, int out_vel,
+                          void* delta, void* S, void* V, long long* nh) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  check_box(b);
+  check_src(modes);
+  check_halo_opts(o);
+  EXH_REQUIRE(modes->kind == EXH_SRC_MODES, EXH_ERR_ARG, "pipeline needs white-noise modes");
+  EXH_REQUIRE(b->nd[0] == b->nd[1] && b->nd[1] == b->nd[2], EXH_ERR_UNSUPPORTED,
+              "the pipeline needs a cubic grid");
+  long long n = b->precision == 4
+                    ? pipeline<float>(c, b, modes, o, k_smooth, do_2lpt, out_vel, delta, S, V)
+                    : pipeline<double>(c, b, modes, o, k_smooth, do_2lpt, out_vel, delta, S, V);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  if (nh) *nh = n;
+  EXH_API_END
+}
+
+template <typename T>
+static long long pipeline_grid(exh_ctx* c, const exh_box* b, const T* delta,
+                               const exh_halo_opts* o, double k_smooth, int do_2lpt, int out_vel,
+                               const T* S_in, const T* V_in, T* S_out, T* V_out) {
+  const size_t ng = (size_t)b->ng;
+  long long nh = run_find_halos<T>(c, b, delta, o);
+  if (S_in) {
+    run_halo_posh<T>(c, b, S_in, V_in, 0);
+    c->halo.have_velh = V_in != nullptr;
+  } else {
+    exh_field_src s{};
+    s.kind = EXH_SRC_DELTA;
+    s.data = delta;
+    T* S = S_out ? S_out : (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
+    T* V = out_vel ? (V_out ? V_out : (T*)ctx_buf(c, BUF_V, ng * 3 * sizeof(T))) : nullptr;
+    launch_lpt<T>(c, b, &s, k_smooth, do_2lpt, 0, S, V);
+    run_halo_posh<T>(c, b, S, V, 0);
+    c->halo.have_velh = V != nullptr;
+    if (S_out) launch_positions<T>(c, b, S_out);
+  }
+  c->halo.have_posh = true;
+  return nh;
+}
+
+extern "C" {
+
+int exh_halos_box_from_grid(exh_ctx* c, const exh_box* b, const void* delta,
+                            const exh_halo_opts* o, double k_smooth, int do_2lpt, int out_vel,
+                            const void* S_in, const void* V_in, void* S_out, void* V_out,
+                            long long* nh) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  check_box(b);
+  check_halo_opts(o);
+  EXH_REQUIRE(delta != nullptr, EXH_ERR_ARG, "null delta");
+  EXH_REQUIRE(S_in || (b->nd[0] == b->nd[1] && b->nd[1] == b->nd[2]), EXH_ERR_UNSUPPORTED,
+              "LPT needs a cubic grid");
+  long long n =
+      b->precision == 4
+          ? pipeline_grid<float>(c, b, (const float*)delta, o, k_smooth, do_2lpt, out_vel,
+                                 (const float*)S_in, (const float*)V_in, (float*)S_out,
+                                 (float*)V_out)
+          : pipeline_grid<double>(c, b, (const double*)delta, o, k_smooth, do_2lpt, out_vel,
+                                  (const double*)S_in, (const double*)V_in, (double*)S_out,
+                                  (double*)V_out);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  if (nh) *nh = n;
+  EXH_API_END
+}
+
+int exh_grid_compute(exh_ctx* c, const void* pos, const void* vel, const void* mass,
+                     const int* type, long long np, int ntype, int nd, double L, double Om0,
+                     double z, int direction, int window, int interlacing, int folds,
+                     int precision, void* grid) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_REQUIRE(precision == 4 || precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
+  EXH_REQUIRE(grid != nullptr && (pos != nullptr || np == 0), EXH_ERR_ARG, "null array");
+  EXH_REQUIRE(np >= 0 && nd > 0 && L > 0, EXH_ERR_ARG, "bad sizes");
+  EXH_REQUIRE(ntype == 1 || type != nullptr, EXH_ERR_ARG, "ntype > 1 needs a type array");
+  EXH_REQUIRE(direction >= -1 && direction <= 2, EXH_ERR_ARG, "direction must be -1, 0, 1 or 2");
+  if (precision == 4)
+    run_grid<float>(c, (const float*)pos, (const float*)vel, (const float*)mass, type, np, ntype,
+                    nd, L, Om0, z, direction, window, interlacing, folds, (float*)grid);
+  else
+    run_grid<double>(c, (const double*)pos, (const double*)vel, (const double*)mass, type, np,
+                     ntype, nd, L, Om0, z, direction, window, interlacing, folds, (double*)grid);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
+int exh_power_compute(exh_ctx* c, const void* grid, int ntype, int nd, double L, int window,
+                      int interlacing, int Nk, double k_min, double k_max, int l_max,
+                      int direction, int precision, int kbin_precision, double* k_out,
+                      double* P_out, long long* count_out) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_REQUIRE(precision == 4 || precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
+  EXH_REQUIRE(kbin_precision == 4 || kbin_precision == 8, EXH_ERR_ARG,
+              "kbin_precision must be 4 or 8");
+  EXH_REQUIRE(grid != nullptr, EXH_ERR_ARG, "null grid");
+  EXH_REQUIRE(k_max > k_min, EXH_ERR_ARG, "k_max must exceed k_min");
+  EXH_REQUIRE(direction >= -1 && direction <= 2, EXH_ERR_ARG, "direction must be -1, 0, 1 or 2");
+  if (precision == 4)
+    run_power<float>(c, (const float*)grid, ntype, nd, L, window, interlacing, Nk, k_min, k_max,
+                     l_max, direction, kbin_precision, k_out, P_out, count_out);
+  else
+    run_power<double>(c, (const double*)grid, ntype, nd, L, window, interlacing, Nk, k_min, k_max,
+                      l_max, direction, kbin_precision, k_out, P_out, count_out);
+  EXH_API_END
+}
+
+}  // extern "C"

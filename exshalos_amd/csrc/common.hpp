@@ -1,0 +1,198 @@
+// Shared definitions for the gfx950 HIP library behind the C-ABI in
+// include/exhip.h: error reporting, the per-context state (stream, rocFFT
+// plans, reusable device buffers) and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "exhip.h"
+
+namespace exh {
+
+// ---------------------------------------------------------------------------
+// errors: every entry point returns an int status and keeps a thread-local
+// message readable through exh_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+
+struct Error {
+  int code;
+};
+
+#define EXH_CHECK_HIP(expr)                                                       \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      ::exh::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,         \
+                       hipGetErrorString(_e));                                    \
+      throw ::exh::Error{EXH_ERR_HIP};                                            \
+    }                                                                             \
+  } while (0)
+
+#define EXH_REQUIRE(cond, code, ...)     \
+  do {                                   \
+    if (!(cond)) {                       \
+      ::exh::set_error(__VA_ARGS__);     \
+      throw ::exh::Error{code};          \
+    }                                    \
+  } while (0)
+
+#define EXH_LAUNCH_CHECK() EXH_CHECK_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// reusable device buffers owned by a context
+// ---------------------------------------------------------------------------
+enum Slot {
+  BUF_CK0,      // half-complex grid (field modes, potentials)
+  BUF_CK1,      // second half-complex grid
+  BUF_R0,       // real grid (potential)
+  BUF_R1,       // real grid (2LPT source / second potential)
+  BUF_S,        // displacement field [ng][3]
+  BUF_V,        // velocity field [ng][3]
+  BUF_FLAG,     // int32 halo flags
+  BUF_CLAIM,    // uint64 per-cell claims used by the halo rounds
+  BUF_PEAK_IDX, // peak cell indices
+  BUF_PEAK_KEY, // peak densities
+  BUF_SORT_K,   // sort ping-pong keys
+  BUF_SORT_V,   // sort ping-pong values
+  BUF_SORT_AUX, // sort scratch
+  BUF_CAND,     // halo candidate state
+  BUF_HALO,     // halo outputs
+  BUF_MISC,     // small scratch (counters, tables)
+  BUF_SCAN,     // hipcub temporary storage
+  BUF_GRIDK,    // transformed grids for P(k)
+  BUF_PART,     // P(k) block partials
+  BUF_TAB,      // P(k) spline knots
+  BUF_SHELL,    // shell offset table of the halo finder
+  BUF_NSLOTS
+};
+
+struct FftKey {
+  int n0, n1, n2, prec, inverse, batch;
+  bool operator<(const FftKey& o) const {
+    return std::tie(n0, n1, n2, prec, inverse, batch) <
+           std::tie(o.n0, o.n1, o.n2, o.prec, o.inverse, o.batch);
+  }
+};
+
+struct FftPlan {
+  void* plan = nullptr;   // rocfft_plan
+  void* info = nullptr;   // rocfft_execution_info
+  void* work = nullptr;   // work buffer
+  size_t work_bytes = 0;
+};
+
+struct Timing {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+// Lattice offsets grouped by r^2: shell s occupies offs[start[s] ..
+// start[s+1]) in the reference's cube order.
+struct ShellTab {
+  const int* start = nullptr;
+  const unsigned* offs = nullptr;
+  int stab = 0;
+};
+
+// Device arrays of the last halo run (views into context buffers).
+struct HaloState {
+  int* hx = nullptr;
+  int* hcount = nullptr;
+  int* hr2 = nullptr;
+  void* Mh = nullptr;
+  void* posh = nullptr;
+  void* velh = nullptr;
+  long long* hidx = nullptr;
+  int* flag = nullptr;
+  ShellTab sh;
+  int sh_tmax = -1;
+  long long ng = 0;
+  bool have_posh = false, have_velh = false;
+};
+
+}  // namespace exh
+
+struct exh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* buf[exh::BUF_NSLOTS] = {};
+  size_t cap[exh::BUF_NSLOTS] = {};
+  std::map<exh::FftKey, exh::FftPlan> plans;
+  // per-kernel HIP-event timing (enabled by exh_ctx_set_timing)
+  bool timing = false;
+  std::vector<exh::Timing> events;
+  // results of the last halo run kept on the device
+  exh::HaloState halo;
+  long long nh = 0;
+  int last_prec = 4;
+  // pinned host staging for small results
+  void* pinned = nullptr;
+  size_t pinned_cap = 0;
+  // halo finder statistics of the last run
+  long long stat_peaks = 0, stat_candidates = 0, stat_rounds = 0, stat_sort_levels = 0;
+};
+
+namespace exh {
+
+void* ctx_buf(exh_ctx* c, Slot s, size_t bytes);
+void* ctx_pinned(exh_ctx* c, size_t bytes);
+
+// rocFFT wrappers (fft.cpp).  Layouts are C order [n0][n1][n2] real and
+// [n0][n1][n2/2+1] complex; `batch` grids are contiguous.
+void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void* in, void* out);
+void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, void* out);
+void fft_release(exh_ctx* c);
+
+// event timing around a launch sequence
+struct TimedScope {
+  exh_ctx* c;
+  int idx = -1;
+  TimedScope(exh_ctx* ctx, const char* name);
+  ~TimedScope();
+};
+
+inline unsigned grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 0x7fffffffLL) g = 0x7fffffffLL;
+  return (unsigned)g;
+}
+
+// integer floor(sqrt(s)) for s >= 0, exact (the reference evaluates
+// floor(sqrt((double)s)) with a correctly rounded sqrt).
+__host__ __device__ inline int isqrt_floor(int s) {
+  int t = (int)sqrt((double)s);
+  while ((t + 1) * (t + 1) <= s) t++;
+  while (t * t > s) t--;
+  return t;
+}
+
+// periodic single wrap (exshalos_h.c:22-30)
+__host__ __device__ inline int wrap1(int i, int j, int n) {
+  int r = i + j;
+  if (r >= n) r -= n;
+  if (r < 0) r += n;
+  return r;
+}
+
+template <typename T>
+struct cplx;
+template <>
+struct cplx<float> {
+  using type = float2;
+};
+template <>
+struct cplx<double> {
+  using type = double2;
+};
+
+}  // namespace exh

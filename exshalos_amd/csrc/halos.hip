@@ -1,0 +1,698 @@
+// Excursion-set halo finder on the GPU (find_halos.c:96-476).
+//
+// 1. peaks: strict 6-neighbour periodic maxima, compacted in lexicographic
+//    cell order (find_halos.c:96-156);
+// 2. Lomuto quicksort emulation (sort.hip), exact tie order;
+// 3. greedy spherical growth in deterministic rounds.  The reference visits
+//    peaks from the densest down and each growth sees the flags of all
+//    denser halos (find_halos.c:226-379).  Every round, each pending
+//    candidate grows speculatively against the flags committed so far
+//    (thread per candidate, the reference's exact double-precision sum
+//    order) and claims the cells it could write (r^2 < G, G = first failing
+//    shell; more flags can only stop the growth earlier, so the final halo
+//    lies inside that ball).  A candidate whose read ball (r^2 <= G) holds
+//    no claim of a denser pending candidate has seen its final flags: it
+//    commits.  The densest pending candidate always commits, so the rounds
+//    terminate; the result equals the sequential loop.
+// 4. masses: taus uniforms by GF(2) jump-ahead, Tinker-integral splines
+//    (find_halos.c:442-476);
+// 5. halo positions from the displacement field (lpt.c:70-74, 245-255).
+#include "common.hpp"
+#include "kernels.hpp"
+#include "util.hpp"
+
+namespace exh {
+
+static constexpr int FLAG_FREE = -1;
+static constexpr int FLAG_REJECTED = -2;
+static constexpr int S_TAB_MAX = 1024;  // largest r^2 a halo may reach
+
+__device__ inline size_t cell_off(int ci, int cj, int ck, unsigned p, int n0, int n1, int n2) {
+  int di = (int)(p & 255u) - 128, dj = (int)((p >> 8) & 255u) - 128,
+      dk = (int)((p >> 16) & 255u) - 128;
+  return ((size_t)wrap1(ci, di, n0) * n1 + wrap1(cj, dj, n1)) * n2 + wrap1(ck, dk, n2);
+}
+
+// ---------------------------------------------------------------------------
+// peaks
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline bool is_peak(const T* d, size_t c, int n0, int n1, int n2) {
+  int k = (int)(c % n2);
+  size_t r = c / n2;
+  int j = (int)(r % n1);
+  int i = (int)(r / n1);
+  T v = d[c];
+  size_t sx = (size_t)n1 * n2;
+  size_t bi = c - (size_t)i * sx, bj = c - (size_t)j * n2, bk = c - k;
+  return v > d[bi + (size_t)wrap1(i, 1, n0) * sx] && v > d[bi + (size_t)wrap1(i, -1, n0) * sx] &&
+         v > d[bj + (size_t)wrap1(j, 1, n1) * n2] && v > d[bj + (size_t)wrap1(j, -1, n1) * n2] &&
+         v > d[bk + wrap1(k, 1, n2)] && v > d[bk + wrap1(k, -1, n2)];
+}
+
+static constexpr int PK_ITEMS = 8;
+static constexpr int PK_TILE = 256 * PK_ITEMS;
+
+template <typename T>
+__global__ void k_peak_count(const T* d, int n0, int n1, int n2, long long* bcnt) {
+  __shared__ long long sh[4];
+  size_t ng = (size_t)n0 * n1 * n2;
+  size_t base = (size_t)blockIdx.x * PK_TILE + (size_t)threadIdx.x * PK_ITEMS;
+  long long cnt = 0;
+  for (int q = 0; q < PK_ITEMS; q++)
+    if (base + q < ng && is_peak(d, base + q, n0, n1, n2)) cnt++;
+  long long tot;
+  block_exclusive_256<long long>(cnt, sh, &tot);
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ void k_peak_write(const T* d, int n0, int n1, int n2, const long long* boff,
+                             T* key, int* cell) {
+  __shared__ long long sh[4];
+  size_t ng = (size_t)n0 * n1 * n2;
+  size_t base = (size_t)blockIdx.x * PK_TILE + (size_t)threadIdx.x * PK_ITEMS;
+  bool pk[PK_ITEMS];
+  long long cnt = 0;
+  for (int q = 0; q < PK_ITEMS; q++) {
+    pk[q] = base + q < ng && is_peak(d, base + q, n0, n1, n2);
+    cnt += pk[q];
+  }
+  long long tot;
+  long long o = block_exclusive_256<long long>(cnt, sh, &tot) + boff[blockIdx.x];
+  for (int q = 0; q < PK_ITEMS; q++)
+    if (pk[q]) {
+      key[o] = d[base + q];
+      cell[o] = (int)(base + q);
+      o++;
+    }
+}
+
+// first index with key >= Bmin in the ascending array (one thread)
+template <typename T>
+__global__ void k_lower_bound(const T* key, long long n, T Bmin, long long* out) {
+  long long lo = 0, hi = n;
+  while (lo < hi) {
+    long long mid = (lo + hi) >> 1;
+    if (key[mid] < Bmin)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  *out = lo;
+}
+
+__global__ void k_fill_i32(int* p, long long n, int v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, long long n, unsigned long long v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_iota(int* p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = (int)i;
+}
+
+// ---------------------------------------------------------------------------
+// growth rounds
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GrowArgs {
+  int n0, n1, n2;
+  const T* delta;
+  int* flag;
+  unsigned long long* claim;
+  ShellTab sh;
+  const T* Bt;
+  int nB;
+  T Bmin;
+  T negMtot;
+  const T* pkey;   // sorted ascending peak densities
+  const int* pcell;
+  long long np;    // candidate rank r <-> sorted index np - 1 - r
+  int Nmin;
+  // per-rank results
+  int* G;          // first failing shell (read radius^2); -1 = skipped
+  int* cnt;        // final count
+  int* r2;         // final grows_tmp
+  unsigned char* kept;
+  int* err;
+};
+
+template <typename T>
+__device__ inline void decode_cell(int cell, int n1, int n2, int& i, int& j, int& k) {
+  k = cell % n2;
+  int r = cell / n2;
+  j = r % n1;
+  i = r / n1;
+}
+
+template <typename T>
+__global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long long tag_hi) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npend) return;
+  int rank = pend[t];
+  long long l = a.np - 1 - rank;
+  int cell = a.pcell[l];
+  if (a.flag[cell] != FLAG_FREE) {  // already inside a denser halo
+    a.G[rank] = -1;
+    a.kept[rank] = 0;
+    return;
+  }
+  int ci, cj, ck;
+  decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
+  const double Bmin = (double)a.Bmin;
+  double den = (double)a.pkey[l], den_tmp = den;
+  int count = 0, count_tmp = 1, grows = 0, grows_tmp = 0;
+  while (den_tmp >= Bmin) {
+    if (count < count_tmp) grows = grows_tmp;
+    grows_tmp++;
+    if (grows_tmp >= a.sh.stab) {
+      atomicOr(a.err, 1);
+      a.G[rank] = -1;
+      return;
+    }
+    den = den_tmp;
+    count = count_tmp;
+    den_tmp = den * ((double)count);
+    for (int o = a.sh.start[grows_tmp]; o < a.sh.start[grows_tmp + 1]; o++) {
+      size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
+      if (a.flag[id] != FLAG_FREE)
+        den_tmp += (double)a.negMtot;
+      else
+        den_tmp += (double)a.delta[id];
+      count_tmp++;
+    }
+    den_tmp = den_tmp / ((double)count_tmp);
+  }
+  const int G = grows_tmp;
+  for (;;) {
+    if (count >= a.nB) {
+      atomicOr(a.err, 2);
+      a.G[rank] = -1;
+      return;
+    }
+    if (!(den < (double)a.Bt[count] && count > 0)) break;
+    den_tmp = den;
+    count_tmp = count;
+    den = den * ((double)count);
+    for (int o = a.sh.start[grows]; o < a.sh.start[grows + 1]; o++) {
+      den -= (double)a.delta[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)];
+      count--;
+    }
+    if (count > 0) den = den / ((double)count);
+    if (count < count_tmp) grows_tmp = grows;
+    grows--;
+  }
+  a.G[rank] = G;
+  a.cnt[rank] = count;
+  a.r2[rank] = grows_tmp;
+  // claim every cell the final halo could occupy (r^2 < G)
+  const unsigned long long tag = tag_hi | (unsigned long long)(0xffffffffu - (unsigned)rank);
+  for (int o = 0; o < a.sh.start[G]; o++) {
+    size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
+    atomicMax(&a.claim[id], tag);
+  }
+}
+
+template <typename T>
+__global__ void k_commit(GrowArgs<T> a, const int* pend, int npend, unsigned long long tag_hi,
+                         int* next, int* nnext, int tmax) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npend) return;
+  int rank = pend[t];
+  int G = a.G[rank];
+  if (G < 0) return;  // skipped or failed
+  long long l = a.np - 1 - rank;
+  int cell = a.pcell[l];
+  int ci, cj, ck;
+  decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
+  // safe iff no denser pending candidate claims a cell of the read ball
+  const unsigned lim = 0xffffffffu - (unsigned)rank;
+  for (int o = 0; o < a.sh.start[G + 1]; o++) {
+    size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
+    unsigned long long v = a.claim[id];
+    if ((v & 0xffffffff00000000ull) == tag_hi && (unsigned)(v & 0xffffffffu) > lim) {
+      int p = atomicAdd(nnext, 1);
+      next[p] = rank;
+      return;
+    }
+  }
+  int count = a.cnt[rank];
+  if (count == 0) {
+    a.kept[rank] = 0;
+    return;
+  }
+  int R2 = a.r2[rank];
+  if (isqrt_floor(R2 - 1) > tmax) {  // flag sphere would wrap (reference writes past its cube)
+    atomicOr(a.err, 4);
+    a.kept[rank] = 0;
+    return;
+  }
+  bool keep = count >= a.Nmin;
+  int v = keep ? (rank << 1) : FLAG_REJECTED;
+  for (int o = 0; o < a.sh.start[R2]; o++)
+    a.flag[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)] = v;
+  if (keep) a.flag[cell] = (rank << 1) | 1;
+  a.kept[rank] = keep ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// catalogue, masses, flags, positions
+// ---------------------------------------------------------------------------
+__global__ void k_kept_to_int(const unsigned char* kept, long long n, long long* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = kept[i];
+}
+
+template <typename T>
+__global__ void k_gather_halos(const unsigned char* kept, const long long* hidx, long long nc,
+                               const int* pcell, long long np, const int* cnt, const int* r2,
+                               int n1, int n2, int* hx, int* hcount, int* hr2) {
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nc;
+       r += (long long)gridDim.x * blockDim.x) {
+    if (!kept[r]) continue;
+    long long h = hidx[r];
+    int cell = pcell[np - 1 - r];
+    int i, j, k;
+    decode_cell<T>(cell, n1, n2, i, j, k);
+    hx[3 * h] = i;
+    hx[3 * h + 1] = j;
+    hx[3 * h + 2] = k;
+    hcount[h] = cnt[r];
+    hr2[h] = r2[r];
+  }
+}
+
+// taus uniforms: thread b produces draws [b*64, b*64+64)
+__global__ void k_taus(const unsigned* mats, unsigned s1, unsigned s2, unsigned s3, long long n,
+                       double* u) {
+  long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long first = b * 64;
+  if (first >= n) return;
+  unsigned st[3] = {s1, s2, s3};
+  for (int comp = 0; comp < 3; comp++) {
+    const unsigned* M = mats + comp * 40 * 32;
+    unsigned v = st[comp];
+    for (int bit = 0; bit < 40; bit++) {
+      if (!((first >> bit) & 1)) continue;
+      unsigned r = 0;
+      for (int col = 0; col < 32; col++)
+        if ((v >> col) & 1u) r ^= M[bit * 32 + col];
+      v = r;
+    }
+    st[comp] = v;
+  }
+  for (int q = 0; q < 64 && first + q < n; q++) {
+    unsigned b1 = ((st[0] << 13) ^ st[0]) >> 19;
+    st[0] = ((st[0] & 4294967294u) << 12) ^ b1;
+    unsigned b2 = ((st[1] << 2) ^ st[1]) >> 25;
+    st[1] = ((st[1] & 4294967288u) << 4) ^ b2;
+    unsigned b3 = ((st[2] << 3) ^ st[2]) >> 11;
+    st[2] = ((st[2] & 4294967280u) << 17) ^ b3;
+    u[first + q] = (double)(st[0] ^ st[1] ^ st[2]) / 4294967296.0;
+  }
+}
+
+__device__ inline double spline_eval_d(const double* x, const double* y, const double* m, int n,
+                                       double xv, int* bad) {
+  if (xv < x[0] || xv > x[n - 1]) {
+    *bad = 1;
+    xv = xv < x[0] ? x[0] : x[n - 1];
+  }
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (x[mid] > xv)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  double h = x[hi] - x[lo];
+  double t = xv - x[lo];
+  double b = (y[hi] - y[lo]) / h - h * (2.0 * m[lo] + m[hi]) / 6.0;
+  double c = m[lo] / 2.0;
+  double d = (m[hi] - m[lo]) / (6.0 * h);
+  return y[lo] + t * (b + t * (c + t * d));
+}
+
+// Mh = InvI(I(M_count) + (I(M_next) - I(M_count)) u)   (find_halos.c:455-476)
+template <typename T>
+__global__ void k_masses(const int* hcount, long long nh, const int* sphere, int nsph,
+                         int Ncells, T Mcell, const double* Mk, const double* Int,
+                         const double* mI, const double* mInv, int Nr, const double* u, T* Mh,
+                         int* err) {
+  for (long long h = blockIdx.x * (long long)blockDim.x + threadIdx.x; h < nh;
+       h += (long long)gridDim.x * blockDim.x) {
+    int cnt = hcount[h];
+    // first index with sphere[i] == cnt among the first Ncells entries
+    int lo = 0, hi = (Ncells < nsph ? Ncells : nsph);
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (sphere[mid] < cnt)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    int bad = 0;
+    if (lo >= (Ncells < nsph ? Ncells : nsph) || sphere[lo] != cnt || lo + 1 >= nsph) {
+      atomicOr(err, 8);
+      Mh[h] = (T)0;
+      continue;
+    }
+    T m0 = (T)cnt * Mcell, m1 = (T)sphere[lo + 1] * Mcell;
+    double I0 = spline_eval_d(Mk, Int, mI, Nr, (double)m0, &bad);
+    T v = (T)(I0 + (spline_eval_d(Mk, Int, mI, Nr, (double)m1, &bad) - I0) * u[h]);
+    Mh[h] = (T)spline_eval_d(Int, Mk, mInv, Nr, (double)v, &bad);
+    if (bad) atomicOr(err, 16);
+  }
+}
+
+__global__ void k_flag_out(const int* flag, long long ng, const long long* hidx, long long* out) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ng;
+       c += (long long)gridDim.x * blockDim.x) {
+    int v = flag[c];
+    long long o;
+    if (v == FLAG_FREE)
+      o = ng;
+    else if (v == FLAG_REJECTED)
+      o = ng + 1;
+    else {
+      long long h = hidx[v >> 1];
+      o = (v & 1) ? -h : h;
+    }
+    out[c] = o;
+  }
+}
+
+// halo displacement sums over the member sphere, in double
+template <typename T>
+__global__ void k_halo_posh(const int* hx, const int* hcount, const int* hr2, long long nh,
+                            ShellTab sh, int n0, int n1, int n2, const T* S, const T* V, T Lc,
+                            T L0, T L1, T L2, int lagrangian, T* posh, T* velh) {
+  for (long long h = blockIdx.x * (long long)blockDim.x + threadIdx.x; h < nh;
+       h += (long long)gridDim.x * blockDim.x) {
+    int ci = hx[3 * h], cj = hx[3 * h + 1], ck = hx[3 * h + 2];
+    double s[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    if (!lagrangian) {
+      int end = sh.start[hr2[h]];
+      for (int o = 0; o < end; o++) {
+        size_t id = cell_off(ci, cj, ck, sh.offs[o], n0, n1, n2);
+        for (int a = 0; a < 3; a++) {
+          s[a] += (double)S[3 * id + a];
+          if (V) v[a] += (double)V[3 * id + a];
+        }
+      }
+    }
+    int q[3] = {ci, cj, ck};
+    T L[3] = {L0, L1, L2};
+    for (int a = 0; a < 3; a++) {
+      T base = (T)((double)((T)q[a] * Lc) + (double)Lc / 2.0);
+      T r = base + (T)(s[a] / hcount[h]);
+      if (r >= L[a]) r -= L[a];
+      if (r < (T)0) r += L[a];
+      posh[3 * h + a] = r;
+      if (velh) velh[3 * h + a] = (T)(v[a] / hcount[h]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+// Upload the shell table for (stab, tmax) unless the context already holds
+// it; returns the number of offsets.
+static long long ensure_shells(exh_ctx* c, int stab, int tmax) {
+  HaloState& S = c->halo;
+  std::vector<int> start;
+  std::vector<unsigned> packed;
+  shell_offsets(stab, tmax, start, packed);
+  if (S.sh.stab == stab && S.sh_tmax == tmax) return (long long)packed.size();
+  size_t bytes = (start.size() + packed.size()) * 4;
+  int* d = (int*)ctx_buf(c, BUF_SHELL, bytes);
+  EXH_CHECK_HIP(hipMemcpyAsync(d, start.data(), start.size() * 4, hipMemcpyHostToDevice, c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(d + start.size(), packed.data(), packed.size() * 4,
+                               hipMemcpyHostToDevice, c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  S.sh.start = d;
+  S.sh.offs = (const unsigned*)(d + start.size());
+  S.sh.stab = stab;
+  S.sh_tmax = tmax;
+  return (long long)packed.size();
+}
+
+template <typename T>
+long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o) {
+  const int n0 = b->nd[0], n1 = b->nd[1], n2 = b->nd[2];
+  const long long ng = b->ng;
+  EXH_REQUIRE(ng < (1LL << 30), EXH_ERR_ARG, "grid too large for int32 cell indices");
+  HaloState& S = c->halo;
+  S.ng = ng;
+  S.have_posh = S.have_velh = false;
+  const int tmax = b->nmin / 2;
+  const int stab = std::min(S_TAB_MAX, 3 * tmax * tmax + 2);
+  const long long noffs = ensure_shells(c, stab, tmax);
+
+  // tables (host): barrier per count, mass-function knots
+  const int nBt = (int)noffs + 2;
+  std::vector<T> sig(std::max(1, (int)std::floor(1e16 / (T)b->Mcell)) + 1), Bt(nBt);
+  std::vector<double> Mk(o->Nk), Int(o->Nk);
+  int Ncells = halo_tables(b, o, sig.data(), Bt.data(), nBt, Mk.data(), Int.data());
+  EXH_REQUIRE(Ncells >= 1, Ncells < 0 ? EXH_ERR_RANGE : EXH_ERR_ARG,
+              "sigma(M) table: cell mass %g gives %d cells below M_max", (double)b->Mcell, Ncells);
+  std::vector<double> mI(o->Nk), mInv(o->Nk);
+  spline_second_derivs(Mk.data(), Int.data(), o->Nk, mI.data());
+  spline_second_derivs(Int.data(), Mk.data(), o->Nk, mInv.data());
+  std::vector<int> sph = cells_in_spheres(10000);
+  std::vector<unsigned> mats(3 * 40 * 32);
+  taus_jump_tables(mats.data());
+
+  // misc device tables in BUF_MISC
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t r = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  size_t o_bt = take(nBt * sizeof(T)), o_mk = take(o->Nk * 8), o_int = take(o->Nk * 8),
+         o_mi = take(o->Nk * 8), o_minv = take(o->Nk * 8), o_sph = take(sph.size() * 4),
+         o_mats = take(mats.size() * 4), o_cnt = take(256);
+  char* misc = (char*)ctx_buf(c, BUF_MISC, off);
+  auto up = [&](size_t o2, const void* src, size_t bytes) {
+    EXH_CHECK_HIP(hipMemcpyAsync(misc + o2, src, bytes, hipMemcpyHostToDevice, c->stream));
+  };
+  up(o_bt, Bt.data(), nBt * sizeof(T));
+  up(o_mk, Mk.data(), o->Nk * 8);
+  up(o_int, Int.data(), o->Nk * 8);
+  up(o_mi, mI.data(), o->Nk * 8);
+  up(o_minv, mInv.data(), o->Nk * 8);
+  up(o_sph, sph.data(), sph.size() * 4);
+  up(o_mats, mats.data(), mats.size() * 4);
+  long long* dcnt = (long long*)(misc + o_cnt);  // [0..7] scratch counters
+  int* derr = (int*)(misc + o_cnt + 128);
+  long long* hbuf = (long long*)ctx_pinned(c, 256);
+
+  // 1. peaks
+  long long nblk = (ng + PK_TILE - 1) / PK_TILE;
+  long long* bc = (long long*)ctx_buf(c, BUF_HALO, (size_t)(2 * nblk + 2) * 8);
+  {
+    TimedScope ts(c, "peaks");
+    hipLaunchKernelGGL(k_peak_count<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
+                       n1, n2, bc);
+    EXH_LAUNCH_CHECK();
+    exclusive_scan<long long>(c, bc, bc + nblk + 1, nblk);
+  }
+  EXH_CHECK_HIP(hipMemcpyAsync(hbuf, bc + nblk + 1 + nblk, 8, hipMemcpyDeviceToHost, c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  const long long np = hbuf[0];
+  c->stat_peaks = np;
+  T* pkey = (T*)ctx_buf(c, BUF_PEAK_KEY, (size_t)std::max(np, 1LL) * sizeof(T));
+  int* pcell = (int*)ctx_buf(c, BUF_PEAK_IDX, (size_t)std::max(np, 1LL) * sizeof(int));
+  {
+    TimedScope ts(c, "peaks_write");
+    hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
+                       n1, n2, bc + nblk + 1, pkey, pcell);
+    EXH_LAUNCH_CHECK();
+  }
+  // 2. sort
+  lomuto_sort<T>(c, pkey, pcell, np);
+
+  // 3. candidates
+  const T Bmin = Bt[Ncells - 1];
+  hipLaunchKernelGGL(k_lower_bound<T>, dim3(1), dim3(1), 0, c->stream, pkey, np, Bmin, dcnt);
+  EXH_LAUNCH_CHECK();
+  EXH_CHECK_HIP(hipMemcpyAsync(hbuf, dcnt, 8, hipMemcpyDeviceToHost, c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  long long lb = std::max(hbuf[0], 1LL);
+  long long nc = np > lb ? np - lb : 0;
+  c->stat_candidates = nc;
+
+  // candidate state in BUF_CAND: G, cnt, r2 (int), kept (u8), pend x2, hidx
+  size_t coff = 0;
+  auto ctake = [&](size_t bytes) {
+    size_t r = coff;
+    coff += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  size_t ncs = (size_t)std::max(nc, 1LL);
+  size_t o_G = ctake(ncs * 4), o_c = ctake(ncs * 4), o_r = ctake(ncs * 4), o_k = ctake(ncs),
+         o_p0 = ctake(ncs * 4), o_p1 = ctake(ncs * 4), o_kl = ctake((ncs + 1) * 8),
+         o_hid = ctake((ncs + 1) * 8);
+  char* cand = (char*)ctx_buf(c, BUF_CAND, coff);
+  int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
+  unsigned long long* claim = (unsigned long long*)ctx_buf(c, BUF_CLAIM, (size_t)ng * 8);
+  S.flag = flag;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ng, 256 * 8)), dim3(256), 0, c->stream, flag, ng,
+                     FLAG_FREE);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ng, 256 * 8)), dim3(256), 0, c->stream, claim, ng,
+                     0ull);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, derr, 1, 0);
+  EXH_LAUNCH_CHECK();
+
+  GrowArgs<T> ga{};
+  ga.n0 = n0;
+  ga.n1 = n1;
+  ga.n2 = n2;
+  ga.delta = delta;
+  ga.flag = flag;
+  ga.claim = claim;
+  ga.sh = S.sh;
+  ga.Bt = (const T*)(misc + o_bt);
+  ga.nB = nBt;
+  ga.Bmin = Bmin;
+  ga.negMtot = -(T)b->Mtot;
+  ga.pkey = pkey;
+  ga.pcell = pcell;
+  ga.np = np;
+  ga.Nmin = o->Nmin;
+  ga.G = (int*)(cand + o_G);
+  ga.cnt = (int*)(cand + o_c);
+  ga.r2 = (int*)(cand + o_r);
+  ga.kept = (unsigned char*)(cand + o_k);
+  ga.err = derr;
+  int* pend = (int*)(cand + o_p0);
+  int* pnext = (int*)(cand + o_p1);
+  int* dnext = (int*)(dcnt + 1);
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(nc, 256 * 8)), dim3(256), 0, c->stream, pend, nc);
+  EXH_LAUNCH_CHECK();
+  long long npend = nc;
+  int round = 0;
+  {
+    TimedScope ts(c, "halo_rounds");
+    while (npend > 0) {
+      round++;
+      unsigned long long tag_hi = (unsigned long long)round << 32;
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, dnext, 1, 0);
+      hipLaunchKernelGGL(k_grow<T>, dim3(grid_for(npend, 256)), dim3(256), 0, c->stream, ga, pend,
+                         (int)npend, tag_hi);
+      EXH_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_commit<T>, dim3(grid_for(npend, 256)), dim3(256), 0, c->stream, ga,
+                         pend, (int)npend, tag_hi, pnext, dnext, tmax);
+      EXH_LAUNCH_CHECK();
+      EXH_CHECK_HIP(hipMemcpyAsync(hbuf, dnext, 4, hipMemcpyDeviceToHost, c->stream));
+      EXH_CHECK_HIP(hipMemcpyAsync((int*)hbuf + 1, derr, 4, hipMemcpyDeviceToHost, c->stream));
+      EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+      int e = ((int*)hbuf)[1];
+      EXH_REQUIRE(e == 0, EXH_ERR_RANGE,
+                  "halo growth left the supported range (code %d): radius^2 beyond %d, count "
+                  "beyond the barrier table, or a sphere wider than half the box",
+                  e, stab - 1);
+      npend = ((int*)hbuf)[0];
+      std::swap(pend, pnext);
+      EXH_REQUIRE(round < 1000000, EXH_ERR_INTERNAL, "halo rounds did not converge");
+    }
+  }
+  c->stat_rounds = round;
+
+  // 4. catalogue
+  long long* kl = (long long*)(cand + o_kl);
+  long long* hidx = (long long*)(cand + o_hid);
+  hipLaunchKernelGGL(k_kept_to_int, dim3(grid_for(nc, 256 * 8)), dim3(256), 0, c->stream,
+                     ga.kept, nc, kl);
+  EXH_LAUNCH_CHECK();
+  exclusive_scan<long long>(c, kl, hidx, nc);
+  EXH_CHECK_HIP(hipMemcpyAsync(hbuf, hidx + nc, 8, hipMemcpyDeviceToHost, c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  const long long nh = hbuf[0];
+  S.hidx = hidx;
+  size_t hs = (size_t)std::max(nh, 1LL);
+  size_t hoff = 0;
+  auto htake = [&](size_t bytes) {
+    size_t r = hoff;
+    hoff += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  size_t o_hx = htake(hs * 12), o_hc = htake(hs * 4), o_hr = htake(hs * 4),
+         o_mh = htake(hs * sizeof(T)), o_u = htake(hs * 8), o_ph = htake(hs * 3 * sizeof(T)),
+         o_vh = htake(hs * 3 * sizeof(T));
+  char* hal = (char*)ctx_buf(c, BUF_HALO, hoff);
+  S.hx = (int*)(hal + o_hx);
+  S.hcount = (int*)(hal + o_hc);
+  S.hr2 = (int*)(hal + o_hr);
+  S.Mh = hal + o_mh;
+  S.posh = hal + o_ph;
+  S.velh = hal + o_vh;
+  hipLaunchKernelGGL(k_gather_halos<T>, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream,
+                     ga.kept, hidx, nc, pcell, np, ga.cnt, ga.r2, n1, n2, S.hx, S.hcount, S.hr2);
+  EXH_LAUNCH_CHECK();
+  // 5. masses
+  if (nh > 0) {
+    TimedScope ts(c, "halo_masses");
+    uint32_t st[3];
+    taus_seed_state(o->seed, st);
+    double* u = (double*)(hal + o_u);
+    hipLaunchKernelGGL(k_taus, dim3(grid_for((nh + 63) / 64, 256)), dim3(256), 0, c->stream,
+                       (const unsigned*)(misc + o_mats), st[0], st[1], st[2], nh, u);
+    EXH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_masses<T>, dim3(grid_for(nh, 256)), dim3(256), 0, c->stream, S.hcount, nh,
+                       (const int*)(misc + o_sph), (int)sph.size(), Ncells, (T)b->Mcell,
+                       (const double*)(misc + o_mk), (const double*)(misc + o_int),
+                       (const double*)(misc + o_mi), (const double*)(misc + o_minv), o->Nk, u,
+                       (T*)S.Mh, derr);
+    EXH_LAUNCH_CHECK();
+    EXH_CHECK_HIP(hipMemcpyAsync(hbuf, derr, 4, hipMemcpyDeviceToHost, c->stream));
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    int e = ((int*)hbuf)[0];
+    EXH_REQUIRE(e == 0, EXH_ERR_RANGE,
+                "halo mass draw left the tabulated mass function (code %d): a halo's next "
+                "sphere exceeds M_max = 1e16 Msun/h",
+                e);
+  }
+  c->nh = nh;
+  c->last_prec = b->precision;
+  return nh;
+}
+
+template <typename T>
+void run_halo_posh(exh_ctx* c, const exh_box* b, const T* S, const T* V, int lagrangian) {
+  HaloState& st = c->halo;
+  if (c->nh <= 0) return;
+  TimedScope ts(c, "halo_posh");
+  hipLaunchKernelGGL(k_halo_posh<T>, dim3(grid_for(c->nh, 256)), dim3(256), 0, c->stream, st.hx,
+                     st.hcount, st.hr2, c->nh, st.sh, b->nd[0], b->nd[1], b->nd[2], S, V,
+                     (T)b->Lc, (T)b->L[0], (T)b->L[1], (T)b->L[2], lagrangian, (T*)st.posh,
+                     V ? (T*)st.velh : nullptr);
+  EXH_LAUNCH_CHECK();
+}
+
+void run_halo_flag(exh_ctx* c, const exh_box* b, long long* flag_dev) {
+  HaloState& st = c->halo;
+  hipLaunchKernelGGL(k_flag_out, dim3(grid_for(b->ng, 256 * 8)), dim3(256), 0, c->stream, st.flag,
+                     b->ng, st.hidx, flag_dev);
+  EXH_LAUNCH_CHECK();
+}
+
+template long long run_find_halos<float>(exh_ctx*, const exh_box*, const float*, const exh_halo_opts*);
+template long long run_find_halos<double>(exh_ctx*, const exh_box*, const double*, const exh_halo_opts*);
+template void run_halo_posh<float>(exh_ctx*, const exh_box*, const float*, const float*, int);
+template void run_halo_posh<double>(exh_ctx*, const exh_box*, const double*, const double*, int);
+
+}  // namespace exh

@@ -1,0 +1,31 @@
+// Host-side parameter derivation and tables; see host_tables.cpp.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+
+namespace exh {
+
+int box_init(exh_box* b, int ndx, int ndy, int ndz, double Lc_or_Mc, double Om0, double z,
+             double dc, int precision);
+
+void spline_second_derivs(const double* x, const double* y, int n, double* m);
+double spline_eval_host(const double* x, const double* y, const double* m, int n, double xv,
+                        int* err);
+
+// Returns Ncells (>= 0), or -1 when sigma(M) was evaluated outside its knots.
+int halo_tables(const exh_box* b, const exh_halo_opts* o, void* sig_grid, void* Bt, int nB,
+                double* Mknots, double* Int);
+
+template <typename R>
+R barrier_value(R S, int do_eb, R dc, R a, R beta, R alpha);
+
+std::vector<int> cells_in_spheres(int smax);
+void shell_offsets(int smax, int tmax, std::vector<int>& start, std::vector<unsigned>& packed);
+
+void taus_jump_tables(uint32_t* mats);
+void taus_seed_state(unsigned long s, uint32_t st[3]);
+
+}  // namespace exh

@@ -1,0 +1,43 @@
+// Launchers shared between the translation units of the library.
+#pragma once
+
+#include "common.hpp"
+#include "host_tables.hpp"
+
+namespace exh {
+
+// field_lpt.hip
+template <typename T>
+void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
+                  void* delta, void* deltak_raw);
+template <typename T>
+void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
+                int do_2lpt, int out_pos, T* S, T* V);
+template <typename T>
+void launch_positions(exh_ctx* c, const exh_box* b, T* S);
+
+// halos.hip
+struct HaloResult {
+  long long nh;
+};
+template <typename T>
+long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o);
+template <typename T>
+void run_halo_posh(exh_ctx* c, const exh_box* b, const T* S, const T* V, int lagrangian);
+void run_halo_flag(exh_ctx* c, const exh_box* b, long long* flag_dev);
+
+// sort.hip: Lomuto quicksort emulation (ascending, "<=" left, pivot last)
+template <typename K>
+void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n);
+
+// spectrum.hip
+template <typename T>
+void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
+              long long np, int ntype, int nd, double L, double Om0, double z, int direction,
+              int window, int interlacing, int folds, T* grid);
+template <typename T>
+void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window,
+               int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
+               int kbin_precision, double* k_out, double* P_out, long long* count_out);
+
+}  // namespace exh

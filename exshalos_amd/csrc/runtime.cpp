@@ -1,0 +1,146 @@
+// Context runtime: error messages, device buffers, HIP-event timing and the
+// rocFFT plan cache.
+#include <rocfft/rocfft.h>
+
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace exh {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char* last_error() { return g_err; }
+
+void* ctx_buf(exh_ctx* c, Slot s, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (c->cap[s] < bytes) {
+    if (c->buf[s]) {
+      EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+      EXH_CHECK_HIP(hipFree(c->buf[s]));
+      c->buf[s] = nullptr;
+      c->cap[s] = 0;
+    }
+    size_t want = (bytes + 4095) & ~(size_t)4095;
+    EXH_CHECK_HIP(hipMalloc(&c->buf[s], want));
+    c->cap[s] = want;
+  }
+  return c->buf[s];
+}
+
+void* ctx_pinned(exh_ctx* c, size_t bytes) {
+  if (c->pinned_cap < bytes) {
+    if (c->pinned) EXH_CHECK_HIP(hipHostFree(c->pinned));
+    c->pinned = nullptr;
+    size_t want = (bytes + 4095) & ~(size_t)4095;
+    EXH_CHECK_HIP(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+    c->pinned_cap = want;
+  }
+  return c->pinned;
+}
+
+TimedScope::TimedScope(exh_ctx* ctx, const char* name) : c(ctx) {
+  if (!c->timing) return;
+  Timing t;
+  t.name = name;
+  EXH_CHECK_HIP(hipEventCreate(&t.start));
+  EXH_CHECK_HIP(hipEventCreate(&t.stop));
+  EXH_CHECK_HIP(hipEventRecord(t.start, c->stream));
+  c->events.push_back(t);
+  idx = (int)c->events.size() - 1;
+}
+
+TimedScope::~TimedScope() {
+  if (idx >= 0) (void)hipEventRecord(c->events[idx].stop, c->stream);
+}
+
+// ---------------------------------------------------------------------------
+// rocFFT
+// ---------------------------------------------------------------------------
+static std::once_flag g_fft_once;
+
+#define EXH_CHECK_FFT(expr)                                                        \
+  do {                                                                             \
+    rocfft_status _s = (expr);                                                     \
+    if (_s != rocfft_status_success) {                                             \
+      ::exh::set_error("%s:%d: %s failed (rocfft status %d)", __FILE__, __LINE__, \
+                       #expr, (int)_s);                                            \
+      throw ::exh::Error{EXH_ERR_FFT};                                             \
+    }                                                                              \
+  } while (0)
+
+static FftPlan& get_plan(exh_ctx* c, int n0, int n1, int n2, int prec, int inverse, int batch) {
+  std::call_once(g_fft_once, [] { rocfft_setup(); });
+  FftKey key{n0, n1, n2, prec, inverse, batch};
+  auto it = c->plans.find(key);
+  if (it != c->plans.end()) return it->second;
+  FftPlan p;
+  // rocFFT lengths are fastest-varying first
+  size_t len[3] = {(size_t)n2, (size_t)n1, (size_t)n0};
+  rocfft_plan_description desc = nullptr;
+  EXH_CHECK_FFT(rocfft_plan_description_create(&desc));
+  if (inverse)
+    EXH_CHECK_FFT(rocfft_plan_description_set_data_layout(
+        desc, rocfft_array_type_hermitian_interleaved, rocfft_array_type_real, nullptr,
+        nullptr, 0, nullptr, 0, 0, nullptr, 0));
+  else
+    EXH_CHECK_FFT(rocfft_plan_description_set_data_layout(
+        desc, rocfft_array_type_real, rocfft_array_type_hermitian_interleaved, nullptr,
+        nullptr, 0, nullptr, 0, 0, nullptr, 0));
+  rocfft_plan plan = nullptr;
+  EXH_CHECK_FFT(rocfft_plan_create(
+      &plan, rocfft_placement_notinplace,
+      inverse ? rocfft_transform_type_real_inverse : rocfft_transform_type_real_forward,
+      prec == 4 ? rocfft_precision_single : rocfft_precision_double, 3, len, (size_t)batch,
+      desc));
+  rocfft_plan_description_destroy(desc);
+  size_t ws = 0;
+  EXH_CHECK_FFT(rocfft_plan_get_work_buffer_size(plan, &ws));
+  rocfft_execution_info info = nullptr;
+  EXH_CHECK_FFT(rocfft_execution_info_create(&info));
+  if (ws) {
+    EXH_CHECK_HIP(hipMalloc(&p.work, ws));
+    EXH_CHECK_FFT(rocfft_execution_info_set_work_buffer(info, p.work, ws));
+  }
+  p.plan = plan;
+  p.info = info;
+  p.work_bytes = ws;
+  return c->plans.emplace(key, p).first->second;
+}
+
+static void run_plan(exh_ctx* c, FftPlan& p, const void* in, void* out) {
+  EXH_CHECK_FFT(rocfft_execution_info_set_stream((rocfft_execution_info)p.info, c->stream));
+  void* ib[1] = {const_cast<void*>(in)};
+  void* ob[1] = {out};
+  EXH_CHECK_FFT(rocfft_execute((rocfft_plan)p.plan, ib, ob, (rocfft_execution_info)p.info));
+}
+
+void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void* in, void* out) {
+  TimedScope ts(c, "fft_r2c");
+  run_plan(c, get_plan(c, n0, n1, n2, prec, 0, batch), in, out);
+}
+
+void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, void* out) {
+  TimedScope ts(c, "fft_c2r");
+  run_plan(c, get_plan(c, n0, n1, n2, prec, 1, batch), in, out);
+}
+
+void fft_release(exh_ctx* c) {
+  for (auto& kv : c->plans) {
+    if (kv.second.plan) rocfft_plan_destroy((rocfft_plan)kv.second.plan);
+    if (kv.second.info) rocfft_execution_info_destroy((rocfft_execution_info)kv.second.info);
+    if (kv.second.work) (void)hipFree(kv.second.work);
+  }
+  c->plans.clear();
+}
+
+}  // namespace exh

@@ -1,0 +1,516 @@
+// Mass assignment and power-spectrum estimator (gfx950).
+//
+//  k_deposit     NGP / CIC / TSC scatter of particles (and of the copy
+//                shifted by -Lb/2 for interlacing) with device-scope atomics
+//                (gridmodule.c:9-53, 146-270); float atomics in FP32, double
+//                atomics in FP64
+//  k_normalise   rho / rho_bar - 1 for unweighted grids (gridmodule.c:197-202)
+//  k_power_bins  one pass over the half-complex modes of all tracer grids:
+//                interlacing combine, window deconvolution and Normk
+//                (spectrum_h.c:112-157), the reference's mode mask and bin
+//                index (powermodule.c:47-55), and the pair products times
+//                P_2l(mu) (powermodule.c:66-72).  Each wavefront takes one
+//                (i, j) row at a time; along a row the bin index never
+//                decreases, so bins are summed by a segmented scan across
+//                the 64 lanes and added by the segment's last lane into a
+//                wave-private LDS accumulator.  Waves and blocks are then
+//                reduced in a fixed order: the double sums are bit-for-bit
+//                reproducible.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "util.hpp"
+
+namespace exh {
+
+// ---------------------------------------------------------------------------
+// deposit
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T wrap_while(T x, T L) {
+  while (x >= L) x -= L;
+  while (x < (T)0) x += L;
+  return x;
+}
+
+template <typename T>
+__device__ inline void atomic_add_t(T* p, T v) {
+  atomicAdd(p, v);
+}
+
+template <typename T>
+__device__ inline void dep_ngp(T* g, const T* x, int nd, T Ld, T m) {
+  int xt[3];
+  for (int a = 0; a < 3; a++) {
+    xt[a] = (int)floor(x[a] / Ld);
+    if (xt[a] == nd) xt[a] -= 1;
+  }
+  atomic_add_t(g + ((size_t)xt[0] * nd + xt[1]) * nd + xt[2], m);
+}
+
+template <typename T>
+__device__ inline void dep_cic(T* g, const T* x, int nd, T Ld, T m) {
+  int xt[3], o[3];
+  T dx[3], t[3];
+  for (int a = 0; a < 3; a++) {
+    T q = x[a] / Ld;
+    xt[a] = (int)floor(q);
+    if (xt[a] == nd) xt[a] -= 1;
+    dx[a] = (T)((double)q - ((double)xt[a] + 0.5));
+    if (dx[a] >= (T)0) {
+      o[a] = xt[a] + 1 == nd ? 0 : xt[a] + 1;
+    } else {
+      o[a] = xt[a] == 0 ? nd - 1 : xt[a] - 1;
+      dx[a] = -dx[a];
+    }
+    t[a] = (T)(1.0 - (double)dx[a]);
+  }
+  const size_t n2 = (size_t)nd * nd;
+  const size_t X0 = xt[0] * n2, X1 = o[0] * n2, Y0 = (size_t)xt[1] * nd, Y1 = (size_t)o[1] * nd;
+  const size_t Z0 = xt[2], Z1 = o[2];
+  atomic_add_t(g + X0 + Y0 + Z0, m * t[0] * t[1] * t[2]);
+  atomic_add_t(g + X1 + Y0 + Z0, m * dx[0] * t[1] * t[2]);
+  atomic_add_t(g + X0 + Y1 + Z0, m * t[0] * dx[1] * t[2]);
+  atomic_add_t(g + X0 + Y0 + Z1, m * t[0] * t[1] * dx[2]);
+  atomic_add_t(g + X1 + Y1 + Z0, m * dx[0] * dx[1] * t[2]);
+  atomic_add_t(g + X1 + Y0 + Z1, m * dx[0] * t[1] * dx[2]);
+  atomic_add_t(g + X0 + Y1 + Z1, m * t[0] * dx[1] * dx[2]);
+  atomic_add_t(g + X1 + Y1 + Z1, m * dx[0] * dx[1] * dx[2]);
+}
+
+template <typename T>
+__device__ inline void dep_tsc(T* g, const T* x, int nd, T Ld, T m) {
+  int c[3];
+  T w[3][3];
+  for (int a = 0; a < 3; a++) {
+    double u = (double)x[a] / (double)Ld - 0.5;
+    double cf = floor(u + 0.5);
+    double d = u - cf;
+    c[a] = (int)cf;
+    w[a][0] = (T)(0.5 * (0.5 - d) * (0.5 - d));
+    w[a][1] = (T)(0.75 - d * d);
+    w[a][2] = (T)(0.5 * (0.5 + d) * (0.5 + d));
+  }
+  for (int a = 0; a < 3; a++) {
+    int ix = c[0] + a - 1;
+    ix = ix < 0 ? ix + nd : (ix >= nd ? ix - nd : ix);
+    for (int b = 0; b < 3; b++) {
+      int iy = c[1] + b - 1;
+      iy = iy < 0 ? iy + nd : (iy >= nd ? iy - nd : iy);
+      T wab = m * w[0][a] * w[1][b];
+      for (int e = 0; e < 3; e++) {
+        int iz = c[2] + e - 1;
+        iz = iz < 0 ? iz + nd : (iz >= nd ? iz - nd : iz);
+        atomic_add_t(g + ((size_t)ix * nd + iy) * nd + iz, wab * w[2][e]);
+      }
+    }
+  }
+}
+
+template <typename T, int WIN>
+__global__ void k_deposit(const T* pos, const T* vel, const T* mass, const int* type,
+                          long long np, int nd, T L, T Lb, int direction, T rsd, int nsub,
+                          T* grid, unsigned long long* counts) {
+  const size_t ng = (size_t)nd * nd * nd;
+  const T Ld = L / (T)nd;
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < np;
+       j += (long long)gridDim.x * blockDim.x) {
+    T x[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    if (direction >= 0) x[direction] = wrap_while<T>(x[direction] + vel[3 * j + direction] * rsd, L);
+    int t = type ? type[j] : 0;
+    T m = mass ? mass[j] : (T)1;
+    T p[3];
+    for (int a = 0; a < 3; a++) p[a] = wrap_while<T>(x[a], L);
+    T* g = grid + (size_t)t * nsub * ng;
+    if (WIN == 1) dep_ngp(g, p, nd, Ld, m);
+    if (WIN == 2) dep_cic(g, p, nd, Ld, m);
+    if (WIN == 5) dep_tsc(g, p, nd, Ld, m);
+    if (nsub == 2) {
+      T sh = (T)(-(double)Lb / 2.0);
+      for (int a = 0; a < 3; a++) p[a] = wrap_while<T>(x[a] + sh, L);
+      if (WIN == 1) dep_ngp(g + ng, p, nd, Ld, m);
+      if (WIN == 2) dep_cic(g + ng, p, nd, Ld, m);
+      if (WIN == 5) dep_tsc(g + ng, p, nd, Ld, m);
+    }
+    if (!mass && counts) atomicAdd(&counts[t], 1ull);
+  }
+}
+
+template <typename T>
+__global__ void k_zero(T* g, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    g[i] = (T)0;
+}
+
+template <typename T>
+__global__ void k_normalise(T* grid, size_t ng, int ngrids, int nsub,
+                            const unsigned long long* counts) {
+  size_t total = ng * ngrids;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    int gi = (int)(i / ng);
+    double M = (double)counts[gi / nsub];
+    grid[i] = (T)((double)grid[i] / (M / (double)ng) - 1.0);
+  }
+}
+
+template <typename T>
+void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
+              long long np, int ntype, int nd, double L, double Om0, double z, int direction,
+              int window, int interlacing, int folds, T* grid) {
+  EXH_REQUIRE(window == EXH_WINDOW_NGP || window == EXH_WINDOW_CIC || window == EXH_WINDOW_TSC,
+              EXH_ERR_UNSUPPORTED, "window %d: only NGP (1), CIC (2) and TSC (5) are implemented",
+              window);
+  EXH_REQUIRE(nd > 0 && ntype >= 1 && folds >= 1, EXH_ERR_ARG, "bad grid arguments");
+  EXH_REQUIRE(direction < 0 || vel != nullptr, EXH_ERR_ARG, "redshift space needs velocities");
+  const int nsub = interlacing ? 2 : 1;
+  const size_t ng = (size_t)nd * nd * nd;
+  T Lf = (T)((T)L / (T)folds);
+  T Lb = Lf / (T)nd;
+  // reference: vel * (1+z)^3 / H(Om0, z) (gridmodule.c:167-169)
+  T Hz = (T)(100.0 * std::sqrt((T)Om0 * std::pow(1.0 + (T)z, 3) + (1.0 - (T)Om0)));
+  T rsd = (T)(std::pow(1.0 + (T)z, 3.0) / (double)Hz);
+  unsigned long long* counts = (unsigned long long*)ctx_buf(c, BUF_MISC, 64 * 8);
+  EXH_REQUIRE(ntype <= 64, EXH_ERR_ARG, "at most 64 tracer types");
+  TimedScope ts(c, "deposit");
+  hipLaunchKernelGGL(k_zero<T>, dim3(grid_for(ng * ntype * nsub, 256 * 8)), dim3(256), 0,
+                     c->stream, grid, ng * ntype * nsub);
+  hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(1), dim3(64), 0, c->stream, counts,
+                     (size_t)64);
+  EXH_LAUNCH_CHECK();
+  unsigned gsz = grid_for(np, 256);
+  if (gsz > 65536 * 4) gsz = 65536 * 4;
+#define EXH_DEP(W)                                                                         \
+  hipLaunchKernelGGL((k_deposit<T, W>), dim3(gsz), dim3(256), 0, c->stream, pos, vel, mass, \
+                     type, np, nd, Lf, Lb, direction, rsd, nsub, grid, counts)
+  if (window == 1) EXH_DEP(1);
+  if (window == 2) EXH_DEP(2);
+  if (window == 5) EXH_DEP(5);
+#undef EXH_DEP
+  EXH_LAUNCH_CHECK();
+  if (!mass) {
+    hipLaunchKernelGGL(k_normalise<T>, dim3(grid_for(ng * ntype * nsub, 256 * 8)), dim3(256), 0,
+                       c->stream, grid, ng, ntype * nsub, nsub, counts);
+    EXH_LAUNCH_CHECK();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// power spectrum
+// ---------------------------------------------------------------------------
+struct PowerArgs {
+  int nd, nz, ntype, nsub, Nk, ls, NPs, NV, direction, kbin;
+  const double2* ex;  // per-axis phase e^{-i k Lb/2}: x/y [nd], z [nz]
+  const double2* ez;
+  const double* wx;   // per-axis window^-1 factors
+  const double* wz;
+  double Normk;
+  // bin index arithmetic
+  float knf, kminf, dkf;
+  double kn, kmin, dk;
+};
+
+__device__ inline double legendre_d(int l, double x) {
+  if (l == 0) return 1.0;
+  double p0 = 1.0, p1 = x;
+  for (int n = 1; n < l; n++) {
+    double p2 = ((2 * n + 1) * x * p1 - n * p0) / (n + 1);
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+__device__ inline double shfl_up_d(double v, int o) {
+  return __shfl_up(v, o, 64);
+}
+
+static constexpr int PW_MAXV = 64;  // max values per mode (pairs x multipoles + 1)
+
+template <typename T>
+__global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, double* part,
+                             unsigned* pcnt) {
+  extern __shared__ double lds[];
+  using C2 = typename cplx<T>::type;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int NV = a.NV, Nk = a.Nk;
+  double* acc = lds + (size_t)wave * Nk * NV;
+  unsigned* cacc = (unsigned*)(lds + (size_t)nw * Nk * NV) + (size_t)wave * Nk;
+  for (int q = lane; q < Nk * NV; q += 64) acc[q] = 0.0;
+  for (int q = lane; q < Nk; q += 64) cacc[q] = 0;
+  __syncthreads();
+  const int nd = a.nd, nz = a.nz;
+  const size_t nh = (size_t)nd * nd * nz;
+  const long long nrows = (long long)nd * nd;
+  const long long gw = (long long)blockIdx.x * nw + wave, tw = (long long)gridDim.x * nw;
+  for (long long row = gw; row < nrows; row += tw) {
+    const int i = (int)(row / nd), j = (int)(row % nd);
+    const int fi = (2 * i < nd) ? i : i - nd, fj = (2 * j < nd) ? j : j - nd;
+    for (int k0 = 0; k0 < nz; k0 += 64) {
+      const int k = k0 + lane;
+      int b = -1;
+      double kmod = 0, mu = 0;
+      if (k < nz) {
+        const int fk = (2 * k < nd) ? k : k - nd;
+        bool masked = (k == 0 || k == nd / 2) &&
+                      ((i > nd / 2 && j > nd / 2) || (i == 0 || i == nd / 2) ||
+                       (j == 0 || j == nd / 2) || (i > nd / 2 && j < nd / 2));
+        if (!masked) {
+          if (a.kbin == 4) {
+            float kx = (float)fi * a.knf, ky = (float)fj * a.knf, kz = (float)fk * a.knf;
+            float km = sqrtf(kx * kx + ky * ky + kz * kz);
+            float tmp = (km - a.kminf) / a.dkf;
+            b = (int)floorf(tmp);
+            kmod = km;
+            if (a.direction == 0) mu = (double)(kx / km);
+            if (a.direction == 1) mu = (double)(ky / km);
+            if (a.direction == 2) mu = (double)(kz / km);
+          } else {
+            double kx = fi * a.kn, ky = fj * a.kn, kz = fk * a.kn;
+            double km = sqrt(kx * kx + ky * ky + kz * kz);
+            b = (int)floor((km - a.kmin) / a.dk);
+            kmod = km;
+            if (a.direction == 0) mu = kx / km;
+            if (a.direction == 1) mu = ky / km;
+            if (a.direction == 2) mu = kz / km;
+          }
+          if (b < 0 || b >= Nk) b = -1;
+        }
+      }
+      const bool valid = b >= 0;
+      // segment bookkeeping
+      const int bprev = __shfl_up(b, 1, 64);
+      const bool head = valid && (lane == 0 || bprev != b);
+      const unsigned long long heads = __ballot(head);
+      const unsigned long long upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+      const int sstart = upto ? 63 - __clzll((long long)upto) : 0;
+      const int bnext = __shfl_down(b, 1, 64);
+      const bool tail = valid && (lane == 63 || bnext != b);
+      // combined modes of every tracer
+      double dr[8], di[8];
+      const int nt = a.ntype;
+      if (valid) {
+        const size_t id = ((size_t)i * nd + j) * nz + k;
+        const double2 e1 = a.ex[i], e2 = a.ex[j], e3 = a.ez[k];
+        const double wk = a.wx[i] * a.wx[j] * a.wz[k] * a.Normk;
+        // e^{-i theta} = e1 e2 e3
+        double pr = e1.x * e2.x - e1.y * e2.y, pi = e1.x * e2.y + e1.y * e2.x;
+        double er = pr * e3.x - pi * e3.y, ei = pr * e3.y + pi * e3.x;
+        for (int t = 0; t < nt; t++) {
+          const C2 f1 = gk[(size_t)(t * a.nsub) * nh + id];
+          double re = (double)f1.x, im = (double)f1.y;
+          if (a.nsub == 2) {
+            const C2 f2 = gk[(size_t)(t * a.nsub + 1) * nh + id];
+            re = (re + ((double)f2.x * er - (double)f2.y * ei)) / 2.0;
+            im = (im + ((double)f2.x * ei + (double)f2.y * er)) / 2.0;
+          }
+          dr[t] = re * wk;
+          di[t] = im * wk;
+        }
+      }
+      // values: pair products x P_2l(mu), then |k|
+      for (int v = 0; v < NV; v++) {
+        double x = 0.0;
+        if (valid) {
+          if (v == NV - 1) {
+            x = kmod;
+          } else {
+            int pair = v / a.ls, l2 = 2 * (v % a.ls);
+            // pair index -> (l, m <= l) in the reference's order
+            int l = 0;
+            while ((l + 1) * (l + 2) / 2 <= pair) l++;
+            int m = pair - l * (l + 1) / 2;
+            x = (dr[l] * dr[m] + di[l] * di[m]) * legendre_d(l2, mu);
+          }
+        }
+        for (int o = 1; o < 64; o <<= 1) {
+          double y = shfl_up_d(x, o);
+          if (lane - o >= sstart) x += y;
+        }
+        if (tail) acc[(size_t)b * NV + v] += x;
+      }
+      if (tail) cacc[b] += (unsigned)(lane - sstart + 1);
+    }
+  }
+  __syncthreads();
+  // fixed-order reduction over waves, one partial per block
+  double* out = part + (size_t)blockIdx.x * Nk * NV;
+  unsigned* cout = pcnt + (size_t)blockIdx.x * Nk;
+  for (int q = threadIdx.x; q < Nk * NV; q += blockDim.x) {
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += lds[(size_t)w * Nk * NV + q];
+    out[q] = s;
+  }
+  for (int q = threadIdx.x; q < Nk; q += blockDim.x) {
+    unsigned s = 0;
+    unsigned* cbase = (unsigned*)(lds + (size_t)nw * Nk * NV);
+    for (int w = 0; w < nw; w++) s += cbase[(size_t)w * Nk + q];
+    cout[q] = s;
+  }
+}
+
+__global__ void k_power_reduce(const double* part, const unsigned* pcnt, int nblocks, int Nk,
+                               int NV, double* sum, unsigned long long* cnt) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < Nk * NV) {
+    double s = 0.0;
+    for (int bl = 0; bl < nblocks; bl++) s += part[(size_t)bl * Nk * NV + q];
+    sum[q] = s;
+  }
+  if (q < Nk) {
+    unsigned long long s = 0;
+    for (int bl = 0; bl < nblocks; bl++) s += pcnt[(size_t)bl * Nk + q];
+    cnt[q] = s;
+  }
+}
+
+template <typename T>
+static double sinc_t(T x) {
+  T r;
+  if (x != (T)0)
+    r = (T)(std::sin((double)x) / (double)x);
+  else
+    r = (T)1;
+  return r;
+}
+
+template <typename T>
+void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window,
+               int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
+               int kbin_precision, double* k_out, double* P_out, long long* count_out) {
+  using C2 = typename cplx<T>::type;
+  EXH_REQUIRE(nd >= 2 && nd % 2 == 0, EXH_ERR_ARG, "nd must be even");
+  EXH_REQUIRE(ntype >= 1 && ntype <= 8, EXH_ERR_UNSUPPORTED, "1 to 8 tracer types supported");
+  EXH_REQUIRE(Nk >= 1 && l_max >= 0, EXH_ERR_ARG, "bad binning");
+  EXH_REQUIRE(window == 0 || window == 1 || window == 2 || window == 5, EXH_ERR_UNSUPPORTED,
+              "window %d cannot be deconvolved (Sphere/Exp windows are not implemented)", window);
+  const int nsub = interlacing ? 2 : 1;
+  const int nz = nd / 2 + 1;
+  const size_t nh = (size_t)nd * nd * nz;
+  const int NPs = ntype * (ntype + 1) / 2, ls = l_max / 2 + 1;
+  const int NV = NPs * ls + 1;
+  EXH_REQUIRE(NV <= PW_MAXV, EXH_ERR_UNSUPPORTED, "too many spectra x multipoles");
+  C2* gk = (C2*)ctx_buf(c, BUF_GRIDK, nh * ntype * nsub * sizeof(C2));
+  fft_r2c(c, nd, nd, nd, sizeof(T) == 4 ? 4 : 8, ntype * nsub, grid, gk);
+
+  // per-axis tables with the reference's fft_real rounding of kn and Lb
+  T Lr = (T)L;
+  T kn = (T)(2 * M_PI / (double)Lr);
+  T Lb = Lr / (T)nd;
+  T Normk = (T)std::pow((double)(Lr / (T)(nd * nd)), 3.0 / 2.0);
+  int pw = window == 0 ? 0 : (window == 1 ? 1 : (window == 2 ? 2 : 3));
+  std::vector<double2> ex(nd), ez(nz);
+  std::vector<double> wx(nd), wz(nz);
+  for (int i = 0; i < nd; i++) {
+    T kx = (2 * i < nd) ? (T)i * kn : (T)(i - nd) * kn;
+    double th = (double)kx * (double)Lb / 2.0;
+    ex[i] = make_double2(std::cos(th), -std::sin(th));
+    wx[i] = 1.0 / std::pow(sinc_t<T>((T)((double)kx * (double)Lb / 2.0)), pw);
+  }
+  for (int k = 0; k < nz; k++) {
+    T kz = (2 * k < nd) ? (T)k * kn : (T)(k - nd) * kn;
+    double th = (double)kz * (double)Lb / 2.0;
+    ez[k] = make_double2(std::cos(th), -std::sin(th));
+    wz[k] = 1.0 / std::pow(sinc_t<T>((T)((double)kz * (double)Lb / 2.0)), pw);
+  }
+  size_t tb = (2 * (size_t)nd + 2 * nz) * sizeof(double2);
+  char* tab = (char*)ctx_buf(c, BUF_TAB, tb);
+  double2* dex = (double2*)tab;
+  double2* dez = dex + nd;
+  double* dwx = (double*)(dez + nz);
+  double* dwz = dwx + nd;
+  EXH_CHECK_HIP(hipMemcpyAsync(dex, ex.data(), nd * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(dez, ez.data(), nz * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(dwx, wx.data(), nd * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(dwz, wz.data(), nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+
+  PowerArgs a{};
+  a.nd = nd;
+  a.nz = nz;
+  a.ntype = ntype;
+  a.nsub = nsub;
+  a.Nk = Nk;
+  a.ls = ls;
+  a.NPs = NPs;
+  a.NV = NV;
+  a.direction = direction;
+  a.kbin = kbin_precision;
+  a.ex = dex;
+  a.ez = dez;
+  a.wx = dwx;
+  a.wz = dwz;
+  a.Normk = (double)Normk;
+  // bin edges: fft_real arithmetic for kbin 4 (spectrum_h.c:39-48, powermodule.c:13)
+  a.knf = (float)(2 * M_PI / (double)(float)L);
+  a.kminf = (float)k_min;
+  a.dkf = ((float)k_max - (float)k_min) / (float)Nk;
+  a.kn = 2 * M_PI / L;
+  a.kmin = k_min;
+  a.dk = (k_max - k_min) / Nk;
+  if (sizeof(T) == 4) {
+    // the FP32 build rounds kn to float before use
+    a.kn = (double)(float)a.kn;
+  }
+
+  // LDS: per wave Nk*NV doubles + Nk counters
+  size_t per_wave = (size_t)Nk * NV * 8 + (size_t)Nk * 4;
+  int waves = 4;
+  while (waves > 1 && per_wave * waves > 60 * 1024) waves--;
+  EXH_REQUIRE(per_wave <= 60 * 1024, EXH_ERR_UNSUPPORTED,
+              "Nk x spectra x multipoles too large for the LDS accumulator (%zu bytes)", per_wave);
+  long long nrows = (long long)nd * nd;
+  int nblocks = (int)std::min<long long>((nrows + waves - 1) / waves, 2048);
+  size_t poff = 0;
+  auto ptake = [&](size_t bytes) {
+    size_t r = poff;
+    poff += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  size_t o_part = ptake((size_t)nblocks * Nk * NV * 8), o_pcnt = ptake((size_t)nblocks * Nk * 4),
+         o_sum = ptake((size_t)Nk * NV * 8), o_cnt = ptake((size_t)Nk * 8);
+  char* pbuf = (char*)ctx_buf(c, BUF_PART, poff);
+  double* part = (double*)(pbuf + o_part);
+  unsigned* pcnt = (unsigned*)(pbuf + o_pcnt);
+  double* sum = (double*)(pbuf + o_sum);
+  unsigned long long* cnt = (unsigned long long*)(pbuf + o_cnt);
+  {
+    TimedScope ts(c, "power_bins");
+    hipLaunchKernelGGL(k_power_bins<T>, dim3(nblocks), dim3(64 * waves), per_wave * waves,
+                       c->stream, a, gk, part, pcnt);
+    EXH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_power_reduce, dim3(grid_for((long long)Nk * NV, 256)), dim3(256), 0,
+                       c->stream, part, pcnt, nblocks, Nk, NV, sum, cnt);
+    EXH_LAUNCH_CHECK();
+  }
+  size_t hbytes = (size_t)Nk * NV * 8 + (size_t)Nk * 8;
+  char* h = (char*)ctx_pinned(c, hbytes);
+  EXH_CHECK_HIP(hipMemcpyAsync(h, sum, (size_t)Nk * NV * 8, hipMemcpyDeviceToHost, c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(h + (size_t)Nk * NV * 8, cnt, (size_t)Nk * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  const double* hs = (const double*)h;
+  const unsigned long long* hc = (const unsigned long long*)(h + (size_t)Nk * NV * 8);
+  for (int b = 0; b < Nk; b++) {
+    long long n = (long long)hc[b];
+    if (count_out) count_out[b] = n;
+    if (k_out) k_out[b] = n > 0 ? hs[(size_t)b * NV + NV - 1] / n : 0.0;
+    for (int p = 0; p < NPs; p++)
+      for (int l = 0; l < ls; l++) {
+        double s = hs[(size_t)b * NV + p * ls + l];
+        if (P_out) P_out[((size_t)p * ls + l) * Nk + b] = n > 0 ? (4.0 * l + 1.0) * s / n : 0.0;
+      }
+  }
+}
+
+template void run_grid<float>(exh_ctx*, const float*, const float*, const float*, const int*,
+                              long long, int, int, double, double, double, int, int, int, int,
+                              float*);
+template void run_grid<double>(exh_ctx*, const double*, const double*, const double*, const int*,
+                               long long, int, int, double, double, double, int, int, int, int,
+                               double*);
+template void run_power<float>(exh_ctx*, const float*, int, int, double, int, int, int, double,
+                               double, int, int, int, double*, double*, long long*);
+template void run_power<double>(exh_ctx*, const double*, int, int, double, int, int, int, double,
+                                double, int, int, int, double*, double*, long long*);
+
+}  // namespace exh

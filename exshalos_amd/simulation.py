@@ -1,0 +1,149 @@
+"""Density grids and power spectra: the hot-path part of
+pyexshalos.simulation (simulation.py:9-311), same signatures and defaults.
+
+Extra keyword-only arguments: ``device_out`` (torch device tensors) and,
+for Compute_Power_Spectrum, ``kbin_precision`` (4: float bin index as the
+reference's FP32 build; 8: double).  Window "TSC" (5) is an addition.
+"""
+
+from __future__ import annotations
+
+from typing import Dict, Optional, Union
+
+import numpy as np
+
+_WINDOWS = {"NO": 0, "NGP": 1, "CIC": 2, "SPHERICAL": 3, "EXPONENTIAL": 4, "TSC": 5}
+
+
+def _window_id(window, allow_none):
+    if isinstance(window, str):
+        w = window.upper()
+        if w not in _WINDOWS:
+            raise ValueError(f"unknown window {window!r}")
+        wid = _WINDOWS[w]
+    else:
+        wid = int(window)
+    if wid == 0 and not allow_none:
+        return None
+    return wid
+
+
+def _direction_id(direction):
+    if direction is None or direction == -1:
+        return -1
+    if direction in ("x", "X", 0):
+        return 0
+    if direction in ("y", "Y", 1):
+        return 1
+    if direction in ("z", "Z", 2):
+        return 2
+    raise ValueError("Direction must be None, x, y or z!")
+
+
+def Compute_Density_Grid(
+    pos: np.ndarray,
+    vel: Optional[np.ndarray] = None,
+    mass: Optional[np.ndarray] = None,
+    types: Optional[np.ndarray] = None,
+    nd: int = 256,
+    L: float = 1000.0,
+    Om0: float = 0.31,
+    z: float = 0.0,
+    direction: Optional[int] = None,
+    window: Union[str, int] = "CIC",
+    R: float = 4.0,
+    R_times: float = 5.0,
+    interlacing: bool = False,
+    folds: int = 1,
+    verbose: bool = False,
+    nthreads: int = 1,
+    *,
+    device_out: bool = False,
+) -> np.ndarray:
+    """Density grid of a set of tracers (reference simulation.py:9-166)."""
+    from .lib.spectrum import check_precision, grid_compute
+
+    precision = check_precision()
+    Rt = np.float32 if precision == 4 else np.float64
+    nmass = 0 if mass is None else 1
+    if types is not None:
+        types = np.asarray(types)
+        types = np.fabs(types)
+        types = types - np.min(types)
+        types = types.astype("int32")
+        ntypes = len(np.unique(types))
+    else:
+        ntypes = 1
+    direction = _direction_id(direction)
+    wid = _window_id(window, allow_none=False)
+    if wid is None:
+        print("You need to choose some density assigment method to construct the density grid!")
+        return None
+    grid = grid_compute(pos, vel, mass, np.int32(nmass), types, np.int32(ntypes), np.int32(nd),
+                        Rt(L), Rt(Om0), Rt(z), np.int32(direction), np.int32(wid), Rt(R),
+                        Rt(R_times), np.int32(interlacing), np.int32(folds), np.int32(verbose),
+                        np.int32(nthreads), device_out=device_out)
+    if not interlacing and ntypes == 1:
+        grid = grid.reshape([nd, nd, nd])
+    elif not interlacing:
+        grid = grid.reshape([ntypes, nd, nd, nd])
+    elif ntypes == 1:
+        grid = grid.reshape([2, nd, nd, nd])
+    return grid
+
+
+def Compute_Power_Spectrum(
+    grid: np.ndarray,
+    L: float = 1000.0,
+    window: Union[str, int] = "CIC",
+    R: float = 4.0,
+    Nk: int = 25,
+    k_min: Optional[float] = None,
+    k_max: Optional[float] = None,
+    l_max: int = 0,
+    direction: Optional[Union[str, int]] = None,
+    folds: int = 1,
+    verbose: bool = False,
+    nthreads: int = 1,
+    ntypes: int = 1,
+    *,
+    kbin_precision: Optional[int] = None,
+    device_out: bool = False,
+) -> Dict[str, np.ndarray]:
+    """Power spectra of density grids (reference simulation.py:170-311)."""
+    shape = tuple(grid.shape)
+    if len(shape) == 3:
+        interlacing = 0
+        ntypes = 1
+    elif len(shape) == 5:
+        interlacing = 1
+        ntypes = shape[0]
+    elif len(shape) == 4 and ntypes == 1:
+        interlacing = 1
+    elif len(shape) == 4 and ntypes > 1:
+        interlacing = 0
+    else:
+        raise ValueError("grid must have 3, 4 or 5 dimensions")
+    nd = shape[-1]
+    L = L / folds
+    if k_min is None:
+        k_min = 2.0 * np.pi / L
+    if k_max is None:
+        k_max = np.pi / L * nd
+    from .lib.spectrum import check_precision, power_compute
+
+    precision = check_precision()
+    Rt = np.float32 if precision == 4 else np.float64
+    wid = _window_id(window, allow_none=True)
+    direction = _direction_id(direction)
+    x = power_compute(grid, np.int32(ntypes), np.int32(nd), Rt(L), np.int32(wid), Rt(R),
+                      np.int32(interlacing), np.int32(Nk), Rt(k_min), Rt(k_max), np.int32(l_max),
+                      np.int32(direction), np.int32(verbose), np.int32(nthreads),
+                      kbin_precision=kbin_precision, device_out=device_out)
+    if ntypes == 1 and l_max == 0:
+        x["Pk"] = x["Pk"].reshape([Nk])
+    elif ntypes == 1:
+        x["Pk"] = x["Pk"].reshape([int(l_max / 2) + 1, Nk])
+    elif l_max == 0:
+        x["Pk"] = x["Pk"].reshape([int(ntypes * (ntypes + 1) / 2), Nk])
+    return x

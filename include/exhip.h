@@ -1,0 +1,216 @@
+/*
+ * exhip.h -- C-ABI of the MI355X (gfx950) library for the ExSHalos
+ * field -> 1LPT/2LPT -> excursion-set halos -> mass assignment -> P(k) path.
+ *
+ * The reference exposes this path through two CPython extension modules,
+ * pyexshalos.lib.exshalos and pyexshalos.lib.spectrum (pyexshalos/lib/
+ * __init__.py:1-4).  Each entry point below replaces one of their C
+ * functions; the citation names the function it stands in for.  Bindings
+ * (the Python shim in exshalos_amd/lib, or any ctypes/cffi/C caller) pass
+ * plain pointers and sizes: device pointers for grids and particle arrays,
+ * host pointers for the small tables and for outputs marked "host".
+ *
+ * Conventions
+ *  - Every function returns EXH_OK (0) or a negative status; the message of
+ *    the last failure on the calling thread is returned by exh_last_error().
+ *    Nothing here exits the process (the reference calls exit(0) or lets GSL
+ *    abort, e.g. exshalos_h.h:40-44, exshalos_h.c:88-91).
+ *  - `precision` is 4 (fields, FFTs and displacements in float, as the
+ *    reference's default build, include/exshalos/exshalos_h.h:17) or 8
+ *    (double).  Device arrays typed "T" are float or double accordingly.
+ *  - Grids are C order [nd0][nd1][nd2]; half-complex grids are
+ *    [nd0][nd1][nd2/2+1] complex, interleaved (re, im).
+ *  - A context owns one HIP stream (or borrows the caller's), the rocFFT
+ *    plans and reusable device buffers; contexts are independent, there is
+ *    no process-global state (the reference keeps its parameters in
+ *    globals, exshalos_h.c:127-132).  Calls on one context are serialised on
+ *    its stream; calls on different contexts may run concurrently.
+ */
+#ifndef EXHIP_H
+#define EXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXH_OK 0
+#define EXH_ERR_ARG (-1)         /* invalid argument or shape */
+#define EXH_ERR_HIP (-2)         /* HIP runtime failure */
+#define EXH_ERR_FFT (-3)         /* rocFFT failure */
+#define EXH_ERR_RANGE (-4)       /* value outside a tabulated range (GSL aborts there) */
+#define EXH_ERR_UNSUPPORTED (-5) /* option the library does not implement */
+#define EXH_ERR_INTERNAL (-6)
+
+#define EXH_WINDOW_NONE 0
+#define EXH_WINDOW_NGP 1
+#define EXH_WINDOW_CIC 2
+#define EXH_WINDOW_TSC 5 /* not in the reference; deconvolved by sinc^6 */
+
+/* Field sources accepted by the LPT and pipeline entry points. */
+#define EXH_SRC_MODES 0  /* unit white-noise modes + P(k) (Compute_Den)  */
+#define EXH_SRC_DELTA 1  /* real-space density grid (Compute_Denk)       */
+#define EXH_SRC_DELTAK 2 /* half-complex density grid, unnormalised modes */
+
+typedef struct exh_ctx exh_ctx;
+
+/* Box and cosmology, derived as set_cosmology + set_box do
+ * (exshalos_h.c:43-106).  With precision 4 every value is rounded to float
+ * where the reference stores an fft_real. */
+typedef struct {
+  int nd[3];
+  int nz2;
+  int nmin;
+  int precision;
+  long long ng;
+  double Om0, redshift, dc, rhoc, Hz, Omz, rhomz, Dv;
+  double Lc, Mcell, L[3], Mtot, kl[3], Normx, Normk;
+} exh_box;
+
+/* A density field given in one of three forms (EXH_SRC_*). */
+typedef struct {
+  int kind;
+  const void* data;  /* device: noise [nd0][nd1][nz2][2] T, delta [ng] T, or deltak */
+  const double* k;   /* host: P(k) knots (EXH_SRC_MODES) */
+  const double* P;   /* host */
+  int Nk;
+  int fixed;         /* fixed amplitudes (phase from the noise pair) */
+  double phase;      /* constant phase rotation */
+} exh_field_src;
+
+/* Halo-finder options (find_halos / halos_box_from_* arguments). */
+typedef struct {
+  int Nmin;            /* minimum number of cells of a kept halo */
+  double a, beta, alpha; /* ellipsoidal barrier; (1, 0, 0) = constant barrier */
+  unsigned long seed;  /* seed of the mass draw (taus) */
+  const double* k;     /* host: P(k) used for sigma(M) (find_halos.c:4-44) */
+  const double* P;
+  int Nk;
+} exh_halo_opts;
+
+/* ---------------------------------------------------------------- basics */
+const char* exh_version(void);
+/* Message of the last failed call on this thread. */
+const char* exh_last_error(void);
+
+/* Create a context on HIP device `device` with its own non-blocking
+ * stream.  No reference counterpart (the reference has no device state). */
+int exh_ctx_create(int device, exh_ctx** out);
+int exh_ctx_destroy(exh_ctx* ctx);
+/* Run subsequent work on `stream` (a hipStream_t, e.g. torch's current
+ * stream); NULL restores the context's own stream. */
+int exh_ctx_set_stream(exh_ctx* ctx, void* stream);
+int exh_ctx_synchronize(exh_ctx* ctx);
+/* Per-kernel HIP-event timing: when enabled, each library stage records a
+ * start/stop event pair on the context stream; exh_ctx_timings() returns
+ * up to `max` (name, milliseconds) pairs of the calls since the last reset,
+ * names separated by '\n' in `names`. */
+int exh_ctx_set_timing(exh_ctx* ctx, int enable);
+int exh_ctx_timings(exh_ctx* ctx, char* names, int names_len, double* ms, int max);
+/* Halo-finder statistics of the last run: peaks, candidates, growth rounds,
+ * sort levels. */
+int exh_ctx_halo_stats(exh_ctx* ctx, long long* out4);
+
+/* set_cosmology + set_box (exshalos_h.c:43-106).  Lc_or_Mc < 100 is a cell
+ * size in Mpc/h, > 1e5 a cell mass in Msun/h, anything else EXH_ERR_ARG. */
+int exh_box_init(exh_box* box, int ndx, int ndy, int ndz, double Lc_or_Mc, double Om0,
+                 double redshift, double dc, int precision);
+
+/* ----------------------------------------------------------------- field */
+/* Gaussian density grid from unit white-noise modes: replaces
+ * density_grid_compute (exshalos.c:46-121) / Compute_Den
+ * (density_grid.c:71-217).  delta: device [ng] T, written as c2r(deltak)
+ * * Normx.  deltak (optional, device [nd0][nd1][nz2][2] T) receives the
+ * unnormalised modes. */
+int exh_density_grid(exh_ctx* ctx, const exh_box* box, const exh_field_src* modes,
+                     double k_smooth, void* delta, void* deltak);
+
+/* ----------------------------------------------------------------- halos */
+/* Halo tables of find_halos.c:4-93,200-212 computed on the host:
+ * sig_grid[Ncells] (T), barrier[nB] (T; entries >= Ncells repeat the last),
+ * Mknots[Nk], Int[Nk] (double).  Returns Ncells through *Ncells. */
+int exh_halo_tables(const exh_box* box, const exh_halo_opts* opts, void* sig_grid,
+                    void* barrier, int nB, double* Mknots, double* Int, int* Ncells);
+
+/* Excursion-set halos of a density grid: replaces find_halos
+ * (exshalos.c:509-610) / Find_Halos (find_halos.c:479-563): peaks,
+ * Lomuto quicksort with the reference's tie order, spherical growth against
+ * the barrier, masses.  delta: device [ng] T.  The catalogue stays on the
+ * device; *nh receives its size. */
+int exh_find_halos(exh_ctx* ctx, const exh_box* box, const void* delta,
+                   const exh_halo_opts* opts, long long* nh);
+
+/* Copy the last catalogue out (each pointer may be NULL):
+ * x [nh][3] int32 centre cells, count [nh] int32, r2 [nh] int32 (cells
+ * with r^2 < r2 belong to the halo), Mh [nh] T.  to_host: destinations are
+ * host memory (1) or device memory (0). */
+int exh_halos_get(exh_ctx* ctx, int* x, int* count, int* r2, void* Mh, int to_host);
+/* Halo positions (and velocities) of the last pipeline run
+ * (Compute_Posh, lpt.c:245-255): posh/velh [nh][3] T. */
+int exh_halos_posh(exh_ctx* ctx, void* posh, void* velh, int to_host);
+/* Flag grid of the last halo run in the reference's encoding
+ * (find_halos.c:349-376): ng free, ng+1 rejected, h member of halo h, -h
+ * its centre.  flag: [ng] int64. */
+int exh_halos_flag(exh_ctx* ctx, long long* flag, int to_host);
+
+/* ------------------------------------------------------------------- LPT */
+/* Displacements of the ng particles: replaces lpt_compute
+ * (exshalos.c:613-707) / Compute_1LPT + Compute_2LPT + Compute_Pos
+ * (lpt.c:4-242).  S: device [ng][3] T (displacements, or positions when
+ * out_pos); V: device [ng][3] T or NULL. */
+int exh_lpt(exh_ctx* ctx, const exh_box* box, const exh_field_src* src, double k_smooth,
+            int do_2lpt, int out_pos, void* S, void* V);
+
+/* --------------------------------------------------------------- pipeline */
+/* Halos and particles from a linear P(k): replaces halos_box_from_pk
+ * (exshalos.c:710-940) / Generate_Halos_Box_from_Pk (box.c:4-68).
+ * modes: EXH_SRC_MODES source.  delta (optional) receives the density grid;
+ * S (optional) the particle positions [ng][3] (OUT_LPT); V (optional) the
+ * particle velocities.  The catalogue, positions and masses stay on the
+ * device (exh_halos_get / exh_halos_posh / exh_halos_flag). */
+int exh_halos_box_from_pk(exh_ctx* ctx, const exh_box* box, const exh_field_src* modes,
+                          const exh_halo_opts* opts, double k_smooth, int do_2lpt,
+                          int out_vel, void* delta, void* S, void* V, long long* nh);
+
+/* Halos from a given density grid: replaces halos_box_from_grid
+ * (exshalos.c:943-1103) / Generate_Halos_Box_from_Grid (box.c:71-150).
+ * With S_in (device [ng][3] T) the given displacements are attributed to
+ * the halos (IN_disp); otherwise LPT is computed from the grid and written
+ * to S_out / V_out when those are not NULL. */
+int exh_halos_box_from_grid(exh_ctx* ctx, const exh_box* box, const void* delta,
+                            const exh_halo_opts* opts, double k_smooth, int do_2lpt,
+                            int out_vel, const void* S_in, const void* V_in, void* S_out,
+                            void* V_out, long long* nh);
+
+/* -------------------------------------------------------------- spectrum */
+/* Mass assignment: replaces grid_compute (spectrum.c:46-108) / Tracer_Grid
+ * (gridmodule.c:146-375).  pos/vel: device [np][3] T; mass: device [np] T
+ * or NULL; type: device [np] int32 or NULL (ntype == 1).  window: NGP, CIC
+ * or TSC.  grid: device [ntype][1+interlacing][nd^3] T, overwritten.
+ * Unweighted grids are normalised to rho/rho_bar - 1, weighted grids are
+ * left as mass sums (gridmodule.c:311). */
+int exh_grid_compute(exh_ctx* ctx, const void* pos, const void* vel, const void* mass,
+                     const int* type, long long np, int ntype, int nd, double L,
+                     double Om0, double z, int direction, int window, int interlacing,
+                     int folds, int precision, void* grid);
+
+/* Power spectra of all tracer pairs: replaces power_compute
+ * (spectrum.c:111-197) / Compute_gridk + Power_Spectrum
+ * (spectrum_h.c:83-167, powermodule.c:4-108).  grid: device
+ * [ntype][1+interlacing][nd^3] T (not modified).  Outputs (host):
+ * k_out[Nk], P_out[NPs][l_max/2+1][Nk], count_out[Nk] with NPs =
+ * ntype(ntype+1)/2.  kbin_precision selects the arithmetic of the bin
+ * index: 4 follows the reference's float evaluation, 8 double.  Bin sums
+ * are accumulated in double. */
+int exh_power_compute(exh_ctx* ctx, const void* grid, int ntype, int nd, double L,
+                      int window, int interlacing, int Nk, double k_min, double k_max,
+                      int l_max, int direction, int precision, int kbin_precision,
+                      double* k_out, double* P_out, long long* count_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXHIP_H */

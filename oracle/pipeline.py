@@ -1,0 +1,375 @@
+"""CPU oracle driver (TEST INFRASTRUCTURE ONLY).
+
+Runs the reference's algorithms for the field -> LPT -> halos -> grid ->
+P(k) path on the CPU, stage by stage, through the C restatement in
+``oracle/exh_oracle.c`` (built into ``oracle/liboracle.so`` by
+``oracle/Makefile``), with the 3D FFTs done by ``numpy.fft`` in float64.
+Only the tests, the smoke check and the benchmark's CPU-baseline leg import
+this module; the product never does.
+
+``precision=4`` follows the reference's single-precision build (float
+fields with its double/long-double islands, file:line citations in the C
+source); ``precision=8`` is the FP64 oracle.
+
+Entry points mirror the reference's C functions:
+
+* ``field_from_modes``   -- Compute_Den (density_grid.c:71-217)
+* ``halo_tables``        -- Compute_Sig / Compute_nh / Barrier (find_halos.c:4-93, 200-212)
+* ``find_halos``         -- Find_Halos (find_halos.c:479-563)
+* ``lpt``                -- Compute_1LPT / Compute_2LPT / Compute_Pos (lpt.c)
+* ``halos_box_from_pk``  -- Generate_Halos_Box_from_Pk (box.c:4-68)
+* ``grid_compute``       -- Tracer_Grid (gridmodule.c:146-375)
+* ``power_compute``      -- Compute_gridk + Power_Spectrum (spectrum_h.c:83-167,
+                            powermodule.c:4-108)
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+M_MAX = 1e16
+
+
+def build() -> str:
+    """Compile the oracle library with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _real(precision: int):
+    return np.float32 if precision == 4 else np.float64
+
+
+def _sfx(precision: int) -> str:
+    return "_f32" if precision == 4 else "_f64"
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _params_struct(precision: int):
+    R = C.c_float if precision == 4 else C.c_double
+
+    class Params(C.Structure):
+        _fields_ = [
+            ("Om0", R), ("redshift", R), ("dc", R), ("rhoc", R), ("Hz", R),
+            ("Omz", R), ("rhomz", R), ("Dv", R),
+            ("Lc", R), ("Mcell", R), ("L", R * 3), ("Mtot", R), ("kl", R * 3),
+            ("Normx", R), ("Normk", R),
+            ("nd", C.c_int * 3), ("nz2", C.c_int), ("nmin", C.c_int),
+            ("ng", C.c_longlong),
+        ]
+
+    return Params
+
+
+_PARAMS = {4: _params_struct(4), 8: _params_struct(8)}
+
+
+def params(nd, Lc, Om0=0.31, z=0.0, dc=-1.0, precision=8):
+    """Derived cosmology/box parameters (exshalos_h.c:43-106)."""
+    nd3 = (nd, nd, nd) if np.isscalar(nd) else tuple(nd)
+    p = _PARAMS[precision]()
+    R = _real(precision)
+    fn = getattr(lib(), "ora_set_params" + _sfx(precision))
+    RC = C.c_float if precision == 4 else C.c_double
+    rc = fn(C.byref(p), C.c_int(nd3[0]), C.c_int(nd3[1]), C.c_int(nd3[2]),
+            RC(R(Lc)), RC(R(Om0)), RC(R(z)), RC(R(dc)))
+    if rc != 0:
+        raise ValueError("Lc must be a cell size < 100 Mpc/h or a cell mass > 1e5 Msun/h")
+    return p
+
+
+def _rc(precision):
+    return C.c_float if precision == 4 else C.c_double
+
+
+# --------------------------------------------------------------------------
+# field
+# --------------------------------------------------------------------------
+def field_modes(p, k, P, noise, precision=8, k_smooth=1e5, fixed=False, phase=0.0):
+    R = _real(precision)
+    nd = tuple(p.nd)
+    k64 = np.ascontiguousarray(np.asarray(k, dtype=R).astype(np.float64))
+    P64 = np.ascontiguousarray(np.asarray(P, dtype=R).astype(np.float64))
+    noise = np.ascontiguousarray(noise, dtype=R)
+    assert noise.shape == (nd[0], nd[1], nd[2] // 2 + 1, 2)
+    deltak = np.zeros_like(noise)
+    rc = getattr(lib(), "ora_field_modes" + _sfx(precision))(
+        C.byref(p), _ptr(k64), _ptr(P64), C.c_int(len(k64)), _ptr(noise),
+        _rc(precision)(k_smooth), C.c_int(int(fixed)), _rc(precision)(phase), _ptr(deltak))
+    if rc < 0:
+        raise RuntimeError("oracle field_modes failed")
+    if rc == 1:
+        raise ValueError("|k| of the grid lies outside the tabulated P(k)")
+    return deltak
+
+
+def c2r(ck: np.ndarray, nd) -> np.ndarray:
+    """Unnormalised complex-to-real 3D transform (FFTW c2r convention)."""
+    z = ck[..., 0].astype(np.float64) + 1j * ck[..., 1].astype(np.float64)
+    n = int(np.prod(nd))
+    return np.fft.irfftn(z, s=tuple(nd), axes=(0, 1, 2)) * n
+
+
+def r2c(x: np.ndarray) -> np.ndarray:
+    """Unnormalised real-to-complex 3D transform, returned as [..., 2]."""
+    z = np.fft.rfftn(x.astype(np.float64), axes=(0, 1, 2))
+    return np.stack([z.real, z.imag], axis=-1)
+
+
+def field_from_modes(k, P, nd, Lc, noise, Om0=0.31, z=0.0, precision=8, k_smooth=1e5,
+                     fixed=False, phase=0.0):
+    """Compute_Den: returns (delta [nd]^3, deltak [nd][nd][nd/2+1][2], params)."""
+    R = _real(precision)
+    p = params(nd, Lc, Om0, z, -1.0, precision)
+    dk = field_modes(p, k, P, noise, precision, k_smooth, fixed, phase)
+    delta = (c2r(dk, tuple(p.nd)).astype(R) * R(p.Normx)).astype(R)
+    return delta, dk, p
+
+
+# --------------------------------------------------------------------------
+# halo finder
+# --------------------------------------------------------------------------
+def cells_in_spheres(smax=10000):
+    cap = 20000
+    out = np.zeros(cap, dtype=np.int32)
+    m = lib().ora_cells_in_spheres(_ptr(out), C.c_int(cap), C.c_int(smax))
+    return out[:m].copy()
+
+
+def halo_tables(p, k, P, a=1.0, beta=0.0, alpha=0.0, precision=8, nB=None):
+    """sigma^2 per cell count, barrier table, and the mass-function knots."""
+    R = _real(precision)
+    k = np.ascontiguousarray(k, dtype=R)
+    P = np.ascontiguousarray(P, dtype=R)
+    a, beta, alpha = R(a), R(beta), R(alpha)
+    do_eb = 0 if (a == 1.0 and beta == 0.0 and alpha == 0.0) else 1
+    Ncells = int(np.floor(M_MAX / float(p.Mcell)))
+    if nB is None:
+        nB = max(Ncells, 1) + 8192
+    sig = np.zeros(max(Ncells, 1), dtype=R)
+    Bt = np.zeros(nB, dtype=R)
+    Mk = np.zeros(len(k), dtype=np.float64)
+    Int = np.zeros(len(k), dtype=np.float64)
+    RC = _rc(precision)
+    rc = getattr(lib(), "ora_halo_tables" + _sfx(precision))(
+        C.byref(p), _ptr(k), _ptr(P), C.c_int(len(k)), C.c_int(do_eb), RC(a), RC(beta),
+        RC(alpha), _ptr(sig), _ptr(Bt), C.c_int(nB), _ptr(Mk), _ptr(Int))
+    if rc < 0:
+        raise ValueError("sigma(M) spline evaluated outside its knots")
+    return dict(Ncells=rc, Sig_grid=sig, Bt=Bt, Mknots=Mk, Int=Int, do_eb=do_eb)
+
+
+def find_peaks(delta, precision=8):
+    R = _real(precision)
+    delta = np.ascontiguousarray(delta, dtype=R)
+    nd = (C.c_int * 3)(*delta.shape)
+    fn = getattr(lib(), "ora_find_peaks" + _sfx(precision))
+    fn.restype = C.c_longlong
+    n = fn(nd, _ptr(delta), None, None, C.c_longlong(0))
+    px = np.zeros((max(n, 1), 3), dtype=np.int32)
+    pd = np.zeros(max(n, 1), dtype=R)
+    fn(nd, _ptr(delta), _ptr(px), _ptr(pd), C.c_longlong(n))
+    return px[:n], pd[:n]
+
+
+def lomuto_sort(keys, precision=8):
+    """Return (sorted keys, perm) with perm[new] = old, as quickSort_peaks."""
+    R = _real(precision)
+    key = np.ascontiguousarray(keys, dtype=R).copy()
+    perm = np.arange(len(key), dtype=np.int32)
+    getattr(lib(), "ora_lomuto_sort" + _sfx(precision))(_ptr(key), _ptr(perm),
+                                                          C.c_longlong(len(key)))
+    return key, perm
+
+
+def taus_uniforms(seed, n):
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    lib().ora_taus_uniforms(C.c_ulong(seed), _ptr(out), C.c_longlong(n))
+    return out[:n]
+
+
+def find_halos(delta, k, P, Lc, Om0=0.31, z=0.0, dc=-1.0, Nmin=1, a=1.0, beta=0.0,
+               alpha=0.0, seed=12345, precision=8, p=None):
+    """Find_Halos: peaks, Lomuto sort, growth, masses.
+
+    Returns a dict with the halo centres ``x`` [nh,3] (cell indices),
+    ``count``, ``r2`` (cells with r^2 < r2 belong to the halo), ``Mh``, the
+    int64 ``flag`` grid, and the sorted peak list (``peak_x``, ``peak_den``).
+    """
+    R = _real(precision)
+    delta = np.ascontiguousarray(delta, dtype=R)
+    nd = delta.shape
+    if p is None:
+        p = params(nd, Lc, Om0, z, dc, precision)
+    tabs = halo_tables(p, k, P, a, beta, alpha, precision)
+    px, pd = find_peaks(delta, precision)
+    skey, perm = lomuto_sort(pd, precision)
+    spx = np.ascontiguousarray(px[perm])
+    ng = int(np.prod(nd))
+    flag = np.full(ng, ng, dtype=np.int64)
+    cap = max(len(pd), 1)
+    hx = np.zeros((cap, 3), dtype=np.int32)
+    hc = np.zeros(cap, dtype=np.int32)
+    hr = np.zeros(cap, dtype=np.int32)
+    fn = getattr(lib(), "ora_grow_halos" + _sfx(precision))
+    fn.restype = C.c_longlong
+    nh = fn(C.byref(p), C.c_longlong(len(pd)), _ptr(spx), _ptr(skey), _ptr(delta),
+            _ptr(tabs["Bt"]), C.c_int(len(tabs["Bt"])), C.c_int(tabs["Ncells"]),
+            C.c_int(Nmin), _ptr(flag), _ptr(hx), _ptr(hc), _ptr(hr))
+    if nh < 0:
+        raise RuntimeError(f"oracle grow_halos failed ({nh})")
+    hx, hc, hr = hx[:nh].copy(), hc[:nh].copy(), hr[:nh].copy()
+    sph = cells_in_spheres()
+    u = taus_uniforms(seed, nh)
+    Mh = np.zeros(max(nh, 1), dtype=R)
+    fn2 = getattr(lib(), "ora_halo_masses" + _sfx(precision))
+    err = fn2(C.byref(p), _ptr(tabs["Mknots"]), _ptr(tabs["Int"]), C.c_int(len(k)),
+              _ptr(sph), C.c_int(len(sph)), C.c_int(tabs["Ncells"]), _ptr(hc),
+              C.c_longlong(nh), _ptr(u), _ptr(Mh))
+    return dict(x=hx, count=hc, r2=hr, Mh=Mh[:nh], flag=flag.reshape(nd), peak_x=spx,
+                peak_den=skey, mass_err=err, tables=tabs, params=p)
+
+
+# --------------------------------------------------------------------------
+# LPT
+# --------------------------------------------------------------------------
+def lpt(deltak, p, k_smooth=1e4, do_2lpt=False, precision=8):
+    """Compute_1LPT (+ Compute_2LPT): returns (S1, S2 or None) as [ng,3]."""
+    R = _real(precision)
+    nd = tuple(p.nd)
+    ck = np.ascontiguousarray(deltak, dtype=R).copy()
+    RC = _rc(precision)
+    sfx = _sfx(precision)
+    getattr(lib(), "ora_poisson_k" + sfx)(C.byref(p), _ptr(ck), RC(k_smooth), C.c_int(0),
+                                          RC(1.0))
+    phi = np.ascontiguousarray(c2r(ck, nd).astype(R))
+    S1 = np.zeros((int(np.prod(nd)), 3), dtype=R)
+    getattr(lib(), "ora_lpt1_disp" + sfx)(C.byref(p), _ptr(phi), _ptr(S1))
+    if not do_2lpt:
+        return S1, None
+    src = np.zeros(nd, dtype=R)
+    getattr(lib(), "ora_lpt2_source" + sfx)(C.byref(p), _ptr(S1), _ptr(src))
+    ck2 = np.ascontiguousarray(r2c(src).astype(R))
+    getattr(lib(), "ora_poisson_k" + sfx)(C.byref(p), _ptr(ck2), RC(k_smooth), C.c_int(1),
+                                          RC(p.Normk))
+    phi2 = np.ascontiguousarray(c2r(ck2, nd).astype(R))
+    S2 = np.zeros_like(S1)
+    getattr(lib(), "ora_lpt2_disp" + sfx)(C.byref(p), _ptr(phi2), _ptr(S2))
+    return S1, S2
+
+
+def positions(S, p, precision=8):
+    """Compute_Pos: q + S wrapped once into the box."""
+    R = _real(precision)
+    out = np.ascontiguousarray(S, dtype=R).copy()
+    getattr(lib(), "ora_lpt_positions" + _sfx(precision))(C.byref(p), _ptr(out))
+    return out
+
+
+def halo_positions(p, flag, S1, S2, halos, precision=8, with_vel=False):
+    R = _real(precision)
+    nh = len(halos["count"])
+    posh = np.zeros((max(nh, 1), 3), dtype=R)
+    velh = np.zeros((max(nh, 1), 3), dtype=R) if with_vel else None
+    getattr(lib(), "ora_halo_disp" + _sfx(precision))(
+        C.byref(p), _ptr(np.ascontiguousarray(flag.reshape(-1), dtype=np.int64)),
+        _ptr(np.ascontiguousarray(S1, dtype=R)),
+        _ptr(None if S2 is None else np.ascontiguousarray(S2, dtype=R)), C.c_longlong(nh),
+        _ptr(np.ascontiguousarray(halos["x"], dtype=np.int32)),
+        _ptr(np.ascontiguousarray(halos["count"], dtype=np.int32)), _ptr(posh), _ptr(velh))
+    return posh[:nh], (velh[:nh] if with_vel else None)
+
+
+def halos_box_from_pk(k, P, nd, Lc, noise, Om0=0.31, z=0.0, k_smooth=1e4, dc=-1.0,
+                      Nmin=1, a=1.0, beta=0.0, alpha=0.0, seed=12345, do_2lpt=False,
+                      out_lpt=False, precision=8):
+    """Generate_Halos_Box_from_Pk (box.c:4-68) with explicit noise modes."""
+    R = _real(precision)
+    p = params(nd, Lc, Om0, z, dc, precision)
+    dk = field_modes(p, k, P, noise, precision, 1e5)
+    delta = (c2r(dk, tuple(p.nd)).astype(R) * R(p.Normx)).astype(R)
+    hal = find_halos(delta, k, P, Lc, Om0, z, dc, Nmin, a, beta, alpha, seed, precision, p)
+    S1, S2 = lpt(dk, p, k_smooth, do_2lpt, precision)
+    posh, _ = halo_positions(p, hal["flag"], S1, S2, hal, precision)
+    out = dict(posh=posh, Mh=hal["Mh"], halos=hal, delta=delta, deltak=dk, params=p)
+    if out_lpt:
+        S = S1 if S2 is None else (S1 + S2).astype(R)
+        out["pos"] = positions(S, p, precision)
+    return out
+
+
+# --------------------------------------------------------------------------
+# deposit and power spectrum
+# --------------------------------------------------------------------------
+def grid_compute(pos, nd, L, window=2, interlacing=False, mass=None, types=None,
+                 ntype=1, direction=-1, vel=None, Om0=0.31, z=0.0, folds=1, precision=8):
+    """Tracer_Grid: returns grid [ntype][1+interlacing][nd][nd][nd]."""
+    R = _real(precision)
+    pos = np.ascontiguousarray(pos, dtype=R).copy()
+    npart = pos.shape[0]
+    grid = np.zeros((ntype, 1 + int(interlacing), nd, nd, nd), dtype=R)
+    mass = None if mass is None else np.ascontiguousarray(mass, dtype=R)
+    types = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
+    vel = None if vel is None else np.ascontiguousarray(vel, dtype=R)
+    RC = _rc(precision)
+    rc = getattr(lib(), "ora_tracer_grid" + _sfx(precision))(
+        _ptr(grid), C.c_int(nd), RC(L), C.c_int(direction), _ptr(pos), _ptr(vel),
+        C.c_longlong(npart), _ptr(mass), _ptr(types), C.c_int(ntype), C.c_int(window),
+        C.c_int(int(interlacing)), RC(Om0), RC(z), C.c_int(folds))
+    if rc != 0:
+        raise ValueError(f"oracle tracer_grid failed ({rc})")
+    return grid
+
+
+def power_compute(grid, ntype, nd, L, window=2, interlacing=False, Nk=25, k_min=None,
+                  k_max=None, l_max=0, direction=-1, precision=8):
+    """Compute_gridk + Power_Spectrum.  grid: [ntype][1+il][nd][nd][nd]."""
+    R = _real(precision)
+    grid = np.asarray(grid, dtype=R).reshape(ntype, 1 + int(interlacing), nd, nd, nd)
+    if k_min is None:
+        k_min = 2.0 * np.pi / L
+    if k_max is None:
+        k_max = np.pi / L * nd
+    RC = _rc(precision)
+    sfx = _sfx(precision)
+    gks = []
+    for t in range(ntype):
+        f1 = np.ascontiguousarray(r2c(grid[t, 0]).astype(R))
+        f2 = np.ascontiguousarray(r2c(grid[t, 1]).astype(R)) if interlacing else None
+        out = np.zeros_like(f1)
+        getattr(lib(), "ora_gridk" + sfx)(C.c_int(nd), RC(L), C.c_int(window), _ptr(f1),
+                                          _ptr(f2), _ptr(out))
+        gks.append(out)
+    arr = (C.c_void_p * ntype)(*[g.ctypes.data for g in gks])
+    NPs = ntype * (ntype + 1) // 2
+    ls = l_max // 2 + 1
+    Pk = np.zeros((NPs, ls, Nk), dtype=np.float64)
+    Km = np.zeros(Nk, dtype=np.float64)
+    cnt = np.zeros(Nk, dtype=np.int64)
+    getattr(lib(), "ora_power_bins" + sfx)(
+        arr, C.c_int(ntype), C.c_int(nd), RC(L), C.c_int(Nk), RC(R(k_min)), RC(R(k_max)),
+        C.c_int(l_max), C.c_int(direction), _ptr(Pk), _ptr(Km), _ptr(cnt))
+    return dict(k=Km.astype(R), Pk=Pk.astype(R), Nk=cnt, Pk64=Pk, k64=Km)
