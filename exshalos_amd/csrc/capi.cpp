@@ -33,7 +33,8 @@ static void check_ctx(exh_ctx* c) {
 static void check_box(const exh_box* b) {
   EXH_REQUIRE(b != nullptr, EXH_ERR_ARG, "null box");
   EXH_REQUIRE(b->precision == 4 || b->precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
-  EXH_REQUIRE(b->nd[0] >= 2 && b->nd[1] >= 2 && b->nd[2] >= 2, EXH_ERR_ARG, "grid too small");
+  EXH_REQUIRE(b->nd[0] >= 4 && b->nd[1] >= 4 && b->nd[2] >= 4, EXH_ERR_ARG,
+              "grid too small (at least 4 cells per side for the 7-point stencils)");
   EXH_REQUIRE(b->nd[0] % 2 == 0 && b->nd[1] % 2 == 0 && b->nd[2] % 2 == 0, EXH_ERR_ARG,
               "grid sizes must be even");
 }

@@ -200,6 +200,11 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
       return;
     }
     if (!(den < (double)a.Bt[count] && count > 0)) break;
+    if (grows < 0) {  // cannot happen while count > 0 on an unclamped table
+      atomicOr(a.err, 32);
+      a.G[rank] = -1;
+      return;
+    }
     den_tmp = den;
     count_tmp = count;
     den = den * ((double)count);

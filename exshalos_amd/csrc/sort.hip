@@ -180,7 +180,8 @@ __global__ void k_tile_chase(const K* keysA, const int* valsA, K* keysB, int* va
     K kv = keysA[j];
     if (kv <= pivot) continue;
     long long s = s0[j];
-    while (s < g.r) s += arot[g.lo + s];
+    // every arot entry is >= 1, so at most r steps (bounded regardless)
+    for (long long guard = 0; s < g.r && guard <= g.r; guard++) s += max(arot[g.lo + s], 1LL);
     long long dst = g.lo + g.c + 1 + (s - g.r);
     keysB[dst] = kv;
     valsB[dst] = valsA[j];
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(64) void k_small_sort(K* keys, int* vals, const lon
       int j = b + lane;
       if (j < hi && !(sk[j] <= pivot)) {
         int s = s0[j];
-        while (s < r) s += arot[s];
+        for (int guard = 0; s < r && guard <= r; guard++) s += max(arot[s], 1);
         tk[lo + c + 1 + (s - r)] = sk[j];
         tv[lo + c + 1 + (s - r)] = sv[j];
       }

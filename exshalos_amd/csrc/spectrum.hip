@@ -25,8 +25,15 @@ namespace exh {
 // ---------------------------------------------------------------------------
 // deposit
 // ---------------------------------------------------------------------------
+// Periodic wrap into [0, L).  Values within one box length of the range
+// take the reference's subtract/add loop (spectrum_h.c:18-28); farther
+// values are reduced with floor first, so that the loop is bounded (the
+// reference's loop does not terminate for huge or infinite values).
+// Non-finite input returns -1, which the callers skip.
 template <typename T>
 __device__ inline T wrap_while(T x, T L) {
+  if (!isfinite(x)) return (T)-1;
+  if (x >= (T)2 * L || x < -L) x = x - floor(x / L) * L;
   while (x >= L) x -= L;
   while (x < (T)0) x += L;
   return x;
@@ -120,6 +127,7 @@ __global__ void k_deposit(const T* pos, const T* vel, const T* mass, const int* 
     T m = mass ? mass[j] : (T)1;
     T p[3];
     for (int a = 0; a < 3; a++) p[a] = wrap_while<T>(x[a], L);
+    if (p[0] < (T)0 || p[1] < (T)0 || p[2] < (T)0) continue;  // non-finite position
     T* g = grid + (size_t)t * nsub * ng;
     if (WIN == 1) dep_ngp(g, p, nd, Ld, m);
     if (WIN == 2) dep_cic(g, p, nd, Ld, m);
