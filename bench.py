@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark: Mcells/s of field -> 2LPT -> halos -> CIC -> P(k) on one MI355X.
+
+Workload (the precision form the metric is quoted on): nd = 512, Lc = 4
+Mpc/h (L = 2048 Mpc/h), Om0 = 0.31, synthetic BBKS P(k) on 500 log-spaced k
+normalised to sigma8 = 0.81, unit white-noise modes from seed 1.  One step
+= white-noise modes already in HBM -> density grid (FP32 rocFFT) ->
+excursion-set halos (peaks, exact Lomuto sort, growth rounds, masses) ->
+1LPT + 2LPT displacements of the 512^3 particles -> interlaced CIC onto two
+512^3 grids -> r2c -> P(k) in 256 bins with FP64 accumulators, copied to
+the host.  value = nd^3 / (seconds per step) / 1e6.
+
+Also reported: the same pipeline in FP64 (the "plain FP64 GPU path"), the
+relative P(k) difference FP32 vs FP64, per-kernel HIP-event times with
+their HBM roofline fractions, and the CPU oracle timed on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(name: str, nd: int, npk: int, interlacing: int, prec: int) -> float | None:
+    """HBM bytes per launch a kernel must move at minimum (its inputs read once, outputs written once)."""
+    r = prec
+    ng = nd**3
+    nh = nd * nd * (nd // 2 + 1)
+    c = 2 * r
+    nsub = 1 + interlacing
+    table = {
+        "field_modes": 2 * c * nh,                  # read noise, write delta_k
+        "lpt1_kspace": 2 * c * nh,                  # read noise, write phi_k
+        "lpt2_kspace": 2 * c * nh,                  # read/write in place
+        "peaks": r * ng,                            # read delta (count pass)
+        "peaks_write": r * ng,                      # read delta again, write peaks
+        "lpt1_grad": r * ng + 3 * r * ng,           # read phi, write S1
+        "lpt2_source": 3 * r * ng + r * ng,         # read S1, write source
+        "lpt2_grad": r * ng + 6 * r * ng,           # read phi2, read+write S
+        "positions": 6 * r * ng,
+        "grid_zero": r * ng * nsub,
+        "cic_deposit": 3 * r * ng + r * ng * nsub,  # 12 Np + 4 ng (1 + il)
+        "grid_normalise": 2 * r * ng * nsub,
+        "power_bins": c * nh * nsub,                # 8 nh ntracer (1 + il)
+    }
+    return table.get(name)
+
+
+def run_cpu_baseline(k, P, nd=128, reps=1):
+    """The CPU oracle (a C + numpy restatement of the reference's CPU path,
+    single-threaded C loops and numpy FFTs) on a bounded sample: the same
+    pipeline at nd=128, FP32 mode."""
+    from oracle import pipeline as O
+    from exshalos_amd import synth
+
+    noise = synth.white_noise(nd, 1, np.float32)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = O.halos_box_from_pk(k, P, nd, 4.0, noise, k_smooth=1e4, seed=1, do_2lpt=True,
+                                  out_lpt=True, precision=4)
+        g = O.grid_compute(out["pos"], nd, 4.0 * nd, window=2, interlacing=True, precision=4)
+        O.power_compute(g, 1, nd, 4.0 * nd, window=2, interlacing=True, Nk=nd // 2, precision=4)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": nd**3 / dt / 1e6, "unit": "Mcells/s", "cores": 1, "kind": "port",
+            "sample": f"oracle pipeline field->halos->2LPT->CIC(interlaced)->P(k) at nd={nd}, "
+                      f"Lc=4, FP32 mode, {reps} run(s), {dt:.2f} s each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nd", type=int, default=512)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-nd", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(independent boxes shard one per GPU with no communication)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from exshalos_amd import synth
+    from exshalos_amd.pipeline import BoxPipeline
+
+    nd = args.nd
+    k, P = synth.bbks_spectrum()
+    t_noise = time.perf_counter()
+    noise64 = synth.white_noise(nd, 1, np.float64)
+    print(f"[bench] noise {nd}^3 modes drawn in {time.perf_counter() - t_noise:.1f} s", flush=True)
+
+    results = {}
+    for prec, steps, warm in ((4, args.steps, args.warmup), (8, args.fp64_steps, 1)):
+        pipe = BoxPipeline(k, P, nd=nd, Lc=4.0, precision=prec, do_2lpt=True, Nk=256,
+                           kbin_precision=8)
+        pipe.set_modes(noise64.astype(np.float32) if prec == 4 else noise64)
+        for _ in range(warm):
+            pipe.run()
+        print(f"[bench] prec {prec}: warmup done, nh = {pipe.nh}", flush=True)
+        torch.cuda.synchronize()
+        pipe.timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            kk, Pk, cnt, nh = pipe.run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        tim = pipe.timings()
+        pipe.timing(False)
+        results[prec] = dict(ms=dt * 1e3, k=kk, Pk=Pk, cnt=cnt, nh=nh, timings=tim,
+                             stats=pipe.halo_stats())
+        print(f"[bench] prec {prec}: {dt * 1e3:.1f} ms/step, nh = {nh}, "
+              f"{nd**3 / dt / 1e6:.1f} Mcells/s", flush=True)
+        del pipe
+        torch.cuda.empty_cache()
+
+    r32, r64 = results[4], results[8]
+    ok = (r64["cnt"] > 0) & (r64["Pk"] != 0)
+    pk_err = float(np.max(np.abs(r32["Pk"][ok] - r64["Pk"][ok]) / np.abs(r64["Pk"][ok])))
+
+    # per-kernel HIP-event times of the FP32 timed region
+    steps = args.steps
+    agg = {}
+    for name, ms in r32["timings"]:
+        a = agg.setdefault(name, [0.0, 0])
+        a[0] += ms
+        a[1] += 1
+    kernels = {}
+    npk = r32["stats"]["peaks"]
+    for name, (tot, cnt) in agg.items():
+        avg = tot / cnt
+        b = algorithmic_bytes(name, nd, npk, 1, 4)
+        e = {"avg_ms": round(avg, 4), "launches_per_step": cnt / steps,
+             "ms_per_step": round(tot / steps, 4)}
+        if b is not None and avg > 0:
+            e["GB/s"] = round(b / (avg * 1e-3) / 1e9, 1)
+            e["hbm_frac"] = round(b / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)
+        kernels[name] = e
+    # dominant kernel with a byte model: CIC deposit (the grid kernel the
+    # metric singles out) unless another modelled kernel takes longer
+    modelled = {n: e for n, e in kernels.items() if "GB/s" in e}
+    dom = max(modelled, key=lambda n: modelled[n]["ms_per_step"]) if modelled else None
+    roof = None
+    if dom:
+        e = modelled[dom]
+        roof = {"kernel": dom, "bound": "hbm", "achieved": e["GB/s"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": e["hbm_frac"], "traffic": None,
+                "bytes_per_launch": algorithmic_bytes(dom, nd, npk, 1, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU oracle sample", flush=True)
+        cpu = run_cpu_baseline(k, P, nd=args.cpu_nd)
+
+    ms = r32["ms"]
+    line = {
+        "metric": "Mcells/s field->2LPT->halos->P(k) at nd=512, FP32 vs FP64 GPU at <=1e-4 P(k) err",
+        "value": round(nd**3 / (ms * 1e-3) / 1e6, 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (BBKS P(k), numpy default_rng(1) white-noise modes)",
+        "config": {"workload": f"nd={nd} Lc=4 L={4 * nd} 2LPT halos CIC-interlaced P(k) Nk=256",
+                   "nd": nd, "Lc": 4.0, "Nk": 256, "window": "CIC", "interlacing": True,
+                   "bins": "FP64 accumulators, FP64 bin index"},
+        "fp64": {"ms_per_step": round(r64["ms"], 3),
+                 "value": round(nd**3 / (r64["ms"] * 1e-3) / 1e6, 2),
+                 "speedup_fp32_over_fp64": round(r64["ms"] / ms, 3)},
+        "pk_rel_err_fp32_vs_fp64": pk_err,
+        "halos": {"fp32": int(r32["nh"]), "fp64": int(r64["nh"]), **r32["stats"]},
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

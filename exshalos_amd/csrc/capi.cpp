@@ -155,13 +155,14 @@ int exh_ctx_timings(exh_ctx* c, char* names, int names_len, double* ms, int max)
   return n;
 }
 
-int exh_ctx_halo_stats(exh_ctx* c, long long* out4) {
+int exh_ctx_halo_stats(exh_ctx* c, long long* out5) {
   EXH_API_BEGIN
-  EXH_REQUIRE(c && out4, EXH_ERR_ARG, "null argument");
-  out4[0] = c->stat_peaks;
-  out4[1] = c->stat_candidates;
-  out4[2] = c->stat_rounds;
-  out4[3] = c->stat_sort_levels;
+  EXH_REQUIRE(c && out5, EXH_ERR_ARG, "null argument");
+  out5[0] = c->stat_peaks;
+  out5[1] = c->stat_candidates;
+  out5[2] = c->stat_rounds;
+  out5[3] = c->stat_sort_levels;
+  out5[4] = c->stat_sort_doubling;
   EXH_API_END
 }
 

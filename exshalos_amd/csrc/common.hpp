@@ -58,7 +58,8 @@ enum Slot {
   BUF_V,        // velocity field [ng][3]
   BUF_FLAG,     // int32 halo flags
   BUF_CLAIM,    // uint64 per-cell claims used by the halo rounds
-  BUF_PEAK_IDX, // peak cell indices
+  BUF_PEAK_IDX, // lexicographic peak indices (sort values)
+  BUF_PEAK_CELL,// peak cell indices in lexicographic order
   BUF_PEAK_KEY, // peak densities
   BUF_SORT_K,   // sort ping-pong keys
   BUF_SORT_V,   // sort ping-pong values
@@ -138,7 +139,8 @@ struct exh_ctx {
   void* pinned = nullptr;
   size_t pinned_cap = 0;
   // halo finder statistics of the last run
-  long long stat_peaks = 0, stat_candidates = 0, stat_rounds = 0, stat_sort_levels = 0;
+  long long stat_peaks = 0, stat_candidates = 0, stat_rounds = 0, stat_sort_levels = 0,
+            stat_sort_doubling = 0;
 };
 
 namespace exh {

@@ -68,7 +68,7 @@ __global__ void k_peak_count(const T* d, int n0, int n1, int n2, long long* bcnt
 
 template <typename T>
 __global__ void k_peak_write(const T* d, int n0, int n1, int n2, const long long* boff,
-                             T* key, int* cell) {
+                             T* key, int* cell, int* idx) {
   __shared__ long long sh[4];
   size_t ng = (size_t)n0 * n1 * n2;
   size_t base = (size_t)blockIdx.x * PK_TILE + (size_t)threadIdx.x * PK_ITEMS;
@@ -84,8 +84,55 @@ __global__ void k_peak_write(const T* d, int n0, int n1, int n2, const long long
     if (pk[q]) {
       key[o] = d[base + q];
       cell[o] = (int)(base + q);
+      idx[o] = (int)o;
       o++;
     }
+}
+
+// Candidates by rank (0 = densest): rank r <-> sorted position np-1-r,
+// kept for sorted positions >= lb (density >= the minimum barrier, and
+// never position 0, which the reference does not visit).  inv maps the
+// lexicographic peak index to its rank (or -1).
+template <typename T>
+__global__ void k_cand_setup(const T* skey, const int* sidx, const int* pcell, long long np,
+                             long long lb, int* ccell, T* cden, int* inv) {
+  for (long long l = blockIdx.x * (long long)blockDim.x + threadIdx.x; l < np;
+       l += (long long)gridDim.x * blockDim.x) {
+    int orig = sidx[l];
+    if (l >= lb) {
+      long long r = np - 1 - l;
+      ccell[r] = pcell[orig];
+      cden[r] = skey[l];
+      inv[orig] = (int)r;
+    } else {
+      inv[orig] = -1;
+    }
+  }
+}
+
+// Ranks of the candidates listed in cell order (stable compaction of inv):
+// the growth rounds then walk the grid in memory order.
+__global__ void k_nonneg_count(const int* v, long long n, long long* bcnt) {
+  __shared__ long long sh[4];
+  long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
+  long long cnt = 0;
+  for (int q = 0; q < PK_ITEMS; q++)
+    if (base + q < n && v[base + q] >= 0) cnt++;
+  long long tot;
+  block_exclusive_256<long long>(cnt, sh, &tot);
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
+}
+
+__global__ void k_nonneg_write(const int* v, long long n, const long long* boff, int* out) {
+  __shared__ long long sh[4];
+  long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
+  long long cnt = 0;
+  for (int q = 0; q < PK_ITEMS; q++)
+    if (base + q < n && v[base + q] >= 0) cnt++;
+  long long tot;
+  long long o = block_exclusive_256<long long>(cnt, sh, &tot) + boff[blockIdx.x];
+  for (int q = 0; q < PK_ITEMS; q++)
+    if (base + q < n && v[base + q] >= 0) out[o++] = v[base + q];
 }
 
 // first index with key >= Bmin in the ascending array (one thread)
@@ -134,9 +181,8 @@ struct GrowArgs {
   int nB;
   T Bmin;
   T negMtot;
-  const T* pkey;   // sorted ascending peak densities
-  const int* pcell;
-  long long np;    // candidate rank r <-> sorted index np - 1 - r
+  const T* cden;   // candidate density by rank (0 = densest)
+  const int* ccell;  // candidate centre cell by rank
   int Nmin;
   // per-rank results
   int* G;          // first failing shell (read radius^2); -1 = skipped
@@ -159,8 +205,7 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npend) return;
   int rank = pend[t];
-  long long l = a.np - 1 - rank;
-  int cell = a.pcell[l];
+  int cell = a.ccell[rank];
   if (a.flag[cell] != FLAG_FREE) {  // already inside a denser halo
     a.G[rank] = -1;
     a.kept[rank] = 0;
@@ -169,7 +214,7 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
   int ci, cj, ck;
   decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
   const double Bmin = (double)a.Bmin;
-  double den = (double)a.pkey[l], den_tmp = den;
+  double den = (double)a.cden[rank], den_tmp = den;
   int count = 0, count_tmp = 1, grows = 0, grows_tmp = 0;
   while (den_tmp >= Bmin) {
     if (count < count_tmp) grows = grows_tmp;
@@ -235,8 +280,7 @@ __global__ void k_commit(GrowArgs<T> a, const int* pend, int npend, unsigned lon
   int rank = pend[t];
   int G = a.G[rank];
   if (G < 0) return;  // skipped or failed
-  long long l = a.np - 1 - rank;
-  int cell = a.pcell[l];
+  int cell = a.ccell[rank];
   int ci, cj, ck;
   decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
   // safe iff no denser pending candidate claims a cell of the read ball
@@ -280,13 +324,13 @@ __global__ void k_kept_to_int(const unsigned char* kept, long long n, long long*
 
 template <typename T>
 __global__ void k_gather_halos(const unsigned char* kept, const long long* hidx, long long nc,
-                               const int* pcell, long long np, const int* cnt, const int* r2,
-                               int n1, int n2, int* hx, int* hcount, int* hr2) {
+                               const int* ccell, const int* cnt, const int* r2, int n1, int n2,
+                               int* hx, int* hcount, int* hr2) {
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nc;
        r += (long long)gridDim.x * blockDim.x) {
     if (!kept[r]) continue;
     long long h = hidx[r];
-    int cell = pcell[np - 1 - r];
+    int cell = ccell[r];
     int i, j, k;
     decode_cell<T>(cell, n1, n2, i, j, k);
     hx[3 * h] = i;
@@ -520,15 +564,16 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   const long long np = hbuf[0];
   c->stat_peaks = np;
   T* pkey = (T*)ctx_buf(c, BUF_PEAK_KEY, (size_t)std::max(np, 1LL) * sizeof(T));
-  int* pcell = (int*)ctx_buf(c, BUF_PEAK_IDX, (size_t)std::max(np, 1LL) * sizeof(int));
+  int* pidx = (int*)ctx_buf(c, BUF_PEAK_IDX, (size_t)std::max(np, 1LL) * sizeof(int));
+  int* pcell = (int*)ctx_buf(c, BUF_PEAK_CELL, (size_t)std::max(np, 1LL) * sizeof(int));
   {
     TimedScope ts(c, "peaks_write");
     hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
-                       n1, n2, bc + nblk + 1, pkey, pcell);
+                       n1, n2, bc + nblk + 1, pkey, pcell, pidx);
     EXH_LAUNCH_CHECK();
   }
-  // 2. sort
-  lomuto_sort<T>(c, pkey, pcell, np);
+  // 2. sort (keys with the lexicographic peak index as value)
+  lomuto_sort<T>(c, pkey, pidx, np);
 
   // 3. candidates
   const T Bmin = Bt[Ncells - 1];
@@ -548,10 +593,17 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     return r;
   };
   size_t ncs = (size_t)std::max(nc, 1LL);
+  size_t nps = (size_t)std::max(np, 1LL);
+  long long nblk_p = (np + PK_TILE - 1) / PK_TILE;
   size_t o_G = ctake(ncs * 4), o_c = ctake(ncs * 4), o_r = ctake(ncs * 4), o_k = ctake(ncs),
          o_p0 = ctake(ncs * 4), o_p1 = ctake(ncs * 4), o_kl = ctake((ncs + 1) * 8),
-         o_hid = ctake((ncs + 1) * 8);
+         o_hid = ctake((ncs + 1) * 8), o_cc = ctake(ncs * 4), o_cd = ctake(ncs * sizeof(T)),
+         o_inv = ctake(nps * 4), o_pb = ctake((size_t)(2 * nblk_p + 2) * 8);
   char* cand = (char*)ctx_buf(c, BUF_CAND, coff);
+  int* ccell = (int*)(cand + o_cc);
+  T* cden = (T*)(cand + o_cd);
+  int* inv = (int*)(cand + o_inv);
+  long long* pb = (long long*)(cand + o_pb);
   int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
   unsigned long long* claim = (unsigned long long*)ctx_buf(c, BUF_CLAIM, (size_t)ng * 8);
   S.flag = flag;
@@ -574,9 +626,8 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   ga.nB = nBt;
   ga.Bmin = Bmin;
   ga.negMtot = -(T)b->Mtot;
-  ga.pkey = pkey;
-  ga.pcell = pcell;
-  ga.np = np;
+  ga.cden = cden;
+  ga.ccell = ccell;
   ga.Nmin = o->Nmin;
   ga.G = (int*)(cand + o_G);
   ga.cnt = (int*)(cand + o_c);
@@ -586,8 +637,18 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   int* pend = (int*)(cand + o_p0);
   int* pnext = (int*)(cand + o_p1);
   int* dnext = (int*)(dcnt + 1);
-  hipLaunchKernelGGL(k_iota, dim3(grid_for(nc, 256 * 8)), dim3(256), 0, c->stream, pend, nc);
-  EXH_LAUNCH_CHECK();
+  {
+    TimedScope ts(c, "halo_candidates");
+    hipLaunchKernelGGL(k_cand_setup<T>, dim3(grid_for(np, 256 * 4)), dim3(256), 0, c->stream,
+                       pkey, pidx, pcell, np, lb, ccell, cden, inv);
+    hipLaunchKernelGGL(k_nonneg_count, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
+                       c->stream, inv, np, pb);
+    EXH_LAUNCH_CHECK();
+    exclusive_scan<long long>(c, pb, pb + nblk_p + 1, nblk_p);
+    hipLaunchKernelGGL(k_nonneg_write, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
+                       c->stream, inv, np, pb + nblk_p + 1, pend);
+    EXH_LAUNCH_CHECK();
+  }
   long long npend = nc;
   int round = 0;
   {
@@ -646,7 +707,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   S.posh = hal + o_ph;
   S.velh = hal + o_vh;
   hipLaunchKernelGGL(k_gather_halos<T>, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream,
-                     ga.kept, hidx, nc, pcell, np, ga.cnt, ga.r2, n1, n2, S.hx, S.hcount, S.hr2);
+                     ga.kept, hidx, nc, ga.ccell, ga.cnt, ga.r2, n1, n2, S.hx, S.hcount, S.hr2);
   EXH_LAUNCH_CHECK();
   // 5. masses
   if (nh > 0) {

@@ -49,6 +49,11 @@ __global__ void k_seg_ntiles(const long long* seg_lo, const long long* seg_hi, i
   j0[s] = seg_hi[s];  // "no > element" until found
 }
 
+__global__ void k_fill_int(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __device__ inline int find_seg(const long long* tile_off, int nseg, long long t) {
   int lo = 0, hi = nseg;  // tile_off[lo] <= t < tile_off[hi]
   while (hi - lo > 1) {
@@ -165,10 +170,17 @@ __global__ void k_tile_scatter(const K* keysA, const int* valsA, K* keysB, int* 
   }
 }
 
+// Chase with at most `cap` dependent steps.  Elements that finish are
+// placed and their s0 set to LLONG_MAX; the others keep their progress in
+// s0 and flag their segment for pointer doubling (an extreme pivot makes
+// the queue short and its elements rotate many times).
+static constexpr int CHASE_CAP = 48;
+
 template <typename K>
 __global__ void k_tile_chase(const K* keysA, const int* valsA, K* keysB, int* valsB,
                              const long long* tile_off, int nseg, const SegInfo* info,
-                             const long long* arot, const long long* s0) {
+                             const long long* arot, long long* s0, int* seg_need,
+                             int* need_any) {
   long long t = blockIdx.x;
   int sg = find_seg(tile_off, nseg, t);
   SegInfo g = info[sg];
@@ -180,10 +192,79 @@ __global__ void k_tile_chase(const K* keysA, const int* valsA, K* keysB, int* va
     K kv = keysA[j];
     if (kv <= pivot) continue;
     long long s = s0[j];
-    // every arot entry is >= 1, so at most r steps (bounded regardless)
-    for (long long guard = 0; s < g.r && guard <= g.r; guard++) s += max(arot[g.lo + s], 1LL);
+    int steps = 0;
+    while (s < g.r && steps < CHASE_CAP) {
+      s += max(arot[g.lo + s], 1LL);  // every entry is >= 1
+      steps++;
+    }
+    if (s < g.r) {
+      s0[j] = s;
+      seg_need[sg] = 1;
+      *need_any = 1;
+      continue;
+    }
+    s0[j] = LLONG_MAX;
     long long dst = g.lo + g.c + 1 + (s - g.r);
     keysB[dst] = kv;
+    valsB[dst] = valsA[j];
+  }
+}
+
+// Pointer doubling on nxt(q) = q + A(q) over the stream indices of flagged
+// segments: after k rounds one entry covers 2^k chase steps.  Values are
+// stored relative to the segment start.
+__global__ void k_double_init(const long long* tile_off, int nseg, const SegInfo* info,
+                              const int* seg_need, const long long* arot, int* nxt) {
+  long long t = blockIdx.x;
+  int sg = find_seg(tile_off, nseg, t);
+  if (!seg_need[sg]) return;
+  SegInfo g = info[sg];
+  long long q0 = (t - g.first_tile) * TILE;
+  bool last = t == tile_off[sg + 1] - 1;
+  long long q1 = last ? g.r : min(q0 + TILE, g.r);
+  for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
+    nxt[g.lo + q] = (int)(q + max(arot[g.lo + q], 1LL));  // >= r: left the stream prefix
+  }
+}
+
+__global__ void k_double_round(const long long* tile_off, int nseg, const SegInfo* info,
+                               const int* seg_need, const int* src, int* dst, int* changed) {
+  long long t = blockIdx.x;
+  int sg = find_seg(tile_off, nseg, t);
+  if (!seg_need[sg]) return;
+  SegInfo g = info[sg];
+  long long q0 = (t - g.first_tile) * TILE;
+  bool last = t == tile_off[sg + 1] - 1;
+  long long q1 = last ? g.r : min(q0 + TILE, g.r);
+  bool open = false;
+  for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
+    int v = src[g.lo + q];
+    int w = v < g.r ? src[g.lo + v] : v;
+    dst[g.lo + q] = w;
+    open |= w < g.r;
+  }
+  if (open) *changed = 1;
+}
+
+template <typename K>
+__global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* valsB,
+                              const long long* tile_off, int nseg, const SegInfo* info,
+                              const int* seg_need, const int* nxt, const long long* s0) {
+  long long t = blockIdx.x;
+  int sg = find_seg(tile_off, nseg, t);
+  if (!seg_need[sg]) return;
+  SegInfo g = info[sg];
+  const K pivot = keysA[g.hi];
+  long long start = g.lo + (t - g.first_tile) * TILE;
+  for (int q = 0; q < PER_THREAD; q++) {
+    long long j = start + (long long)q * 256 + threadIdx.x;
+    if (j >= g.hi) continue;
+    if (keysA[j] <= pivot) continue;
+    long long s = s0[j];
+    if (s >= g.r) continue;  // placed by the capped chase
+    for (int guard = 0; s < g.r && guard < 64; guard++) s = nxt[g.lo + s];
+    long long dst = g.lo + g.c + 1 + (s - g.r);
+    keysB[dst] = keysA[j];
     valsB[dst] = valsA[j];
   }
 }
@@ -215,12 +296,75 @@ __global__ void k_tile_copyback(K* keysA, int* valsA, const K* keysB, const int*
 // ---------------------------------------------------------------------------
 // small segments: one wavefront each, LDS resident
 // ---------------------------------------------------------------------------
+static constexpr int TINY = 64;
+
+// Scalar Lomuto quicksort of sk/sv[lo..hi] (LDS) by one lane: the
+// reference's partition verbatim.  The smaller side is sorted next and the
+// larger pushed, so the stack depth stays below log2(TINY) + 1.
+template <typename K>
+__device__ inline void lane_quicksort(K* sk, int* sv, int lo, int hi) {
+  int st_lo[8], st_hi[8];
+  int top = 0;
+  for (;;) {
+    while (lo < hi) {
+      const K pivot = sk[hi];
+      int i = lo - 1;
+      for (int j = lo; j < hi; j++) {
+        const K kj = sk[j];
+        if (kj <= pivot) {
+          i++;
+          const K ki = sk[i];
+          sk[i] = kj;
+          sk[j] = ki;
+          const int vi = sv[i];
+          sv[i] = sv[j];
+          sv[j] = vi;
+        }
+      }
+      {
+        const K t = sk[i + 1];
+        sk[i + 1] = sk[hi];
+        sk[hi] = t;
+        const int v = sv[i + 1];
+        sv[i + 1] = sv[hi];
+        sv[hi] = v;
+      }
+      const int p = i + 1;
+      int al = lo, ah = p - 1, bl = p + 1, bh = hi;
+      if (ah - al > bh - bl) {  // [al, ah] becomes the smaller side
+        int x = al, y = ah;
+        al = bl;
+        ah = bh;
+        bl = x;
+        bh = y;
+      }
+      if (bh > bl) {
+        st_lo[top] = bl;
+        st_hi[top] = bh;
+        top++;
+      }
+      lo = al;
+      hi = ah;
+    }
+    if (top == 0) break;
+    top--;
+    lo = st_lo[top];
+    hi = st_hi[top];
+  }
+}
+
+// One wavefront per segment of 2..SMALL elements, resident in LDS.
+// Sub-segments longer than TINY are partitioned by the whole wave with the
+// closed-form Lomuto rule (see the file comment); shorter ones are queued
+// and then sorted one per lane by lane_quicksort.
 template <typename K, int SMALL>
 __global__ __launch_bounds__(64) void k_small_sort(K* keys, int* vals, const long long* seg_lo,
                                                    const long long* seg_hi) {
   __shared__ K sk[SMALL], tk[SMALL];
   __shared__ int sv[SMALL], tv[SMALL], arot[SMALL], s0[SMALL];
   __shared__ int st_lo[64], st_hi[64];
+  __shared__ short tl_lo[SMALL / 2 + 1], tl_hi[SMALL / 2 + 1];
+  __shared__ int ntiny;
   const int lane = threadIdx.x;
   const long long base = seg_lo[blockIdx.x];
   const int n = (int)(seg_hi[blockIdx.x] - base + 1);
@@ -230,16 +374,22 @@ __global__ __launch_bounds__(64) void k_small_sort(K* keys, int* vals, const lon
   }
   int top = 0;
   if (lane == 0) {
-    st_lo[0] = 0;
-    st_hi[0] = n - 1;
+    ntiny = 0;
+    if (n > TINY) {
+      st_lo[0] = 0;
+      st_hi[0] = n - 1;
+    } else {
+      tl_lo[0] = 0;
+      tl_hi[0] = (short)(n - 1);
+      ntiny = 1;
+    }
   }
-  top = 1;
+  top = n > TINY ? 1 : 0;
   __syncthreads();
   while (top > 0) {
     top--;
     const int lo = st_lo[top], hi = st_hi[top];
     __syncthreads();
-    if (hi - lo < 1) continue;
     const K pivot = sk[hi];
     // pass 1: count "<=" and find the first ">"
     int c = 0, j0 = hi;
@@ -292,7 +442,8 @@ __global__ __launch_bounds__(64) void k_small_sort(K* keys, int* vals, const lon
       sk[i] = tk[i];
       sv[i] = tv[i];
     }
-    // children: push the larger first so the smaller is sorted next
+    // children: long ones on the stack (larger first, so the smaller is
+    // partitioned next), short ones to the per-lane list
     int al = lo, ah = lo + c - 1, bl = lo + c + 1, bh = hi;
     if (ah - al > bh - bl) {
       int x = al, y = ah;
@@ -301,23 +452,35 @@ __global__ __launch_bounds__(64) void k_small_sort(K* keys, int* vals, const lon
       bl = x;
       bh = y;
     }
-    // now [al, ah] is the smaller child
+    int pushed = 0;
     if (lane == 0) {
       int tt = top;
-      if (bh - bl >= 1) {
+      if (bh - bl + 1 > TINY) {
         st_lo[tt] = bl;
         st_hi[tt] = bh;
         tt++;
+      } else if (bh > bl) {
+        tl_lo[ntiny] = (short)bl;
+        tl_hi[ntiny] = (short)bh;
+        ntiny++;
       }
-      if (ah - al >= 1) {
+      if (ah - al + 1 > TINY) {
         st_lo[tt] = al;
         st_hi[tt] = ah;
         tt++;
+      } else if (ah > al) {
+        tl_lo[ntiny] = (short)al;
+        tl_hi[ntiny] = (short)ah;
+        ntiny++;
       }
     }
-    top += (bh - bl >= 1) + (ah - al >= 1);
+    pushed = (bh - bl + 1 > TINY) + (ah - al + 1 > TINY);
+    top += pushed;
     __syncthreads();
   }
+  __syncthreads();
+  for (int q = lane; q < ntiny; q += 64) lane_quicksort<K>(sk, sv, tl_lo[q], tl_hi[q]);
+  __syncthreads();
   for (int i = lane; i < n; i += 64) {
     keys[base + i] = sk[i];
     vals[base + i] = sv[i];
@@ -340,6 +503,7 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
   TimedScope ts(c, "peak_sort");
   constexpr int SMALL = SmallCfg<K>::N;
   c->stat_sort_levels = 0;
+  c->stat_sort_doubling = 0;
   if (n < 2) return;
   // scratch layout in BUF_SORT_AUX
   const long long maxseg = n / (SMALL + 1) * 2 + 4;
@@ -357,7 +521,8 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
          o_tle = take((maxtiles + 1) * 8), o_tpre = take((maxtiles + 2) * 8),
          o_info = take(maxseg * sizeof(SegInfo)), o_slo = take(maxsmall * 8),
          o_shi = take(maxsmall * 8), o_cnt = take(64), o_arot = take((size_t)n * 8),
-         o_s0 = take((size_t)n * 8);
+         o_s0 = take((size_t)n * 8), o_need = take(maxseg * 4 + 64), o_nxa = take((size_t)n * 4 + 64),
+         o_nxb = take((size_t)n * 4 + 64);
   char* aux = (char*)ctx_buf(c, BUF_SORT_AUX, bytes);
   K* kB = (K*)ctx_buf(c, BUF_SORT_K, (size_t)n * sizeof(K));
   int* vB = (int*)ctx_buf(c, BUF_SORT_V, (size_t)n * sizeof(int));
@@ -381,7 +546,7 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
   int nsmall_host = 0;
   int nseg = 0;
   long long init_lo = 0, init_hi = n - 1;
-  EXH_CHECK_HIP(hipMemsetAsync(cnt, 0, 16, c->stream));
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 16, 0);
   if (n > SMALL) {
     EXH_CHECK_HIP(hipMemcpyAsync(cur_lo, &init_lo, 8, hipMemcpyHostToDevice, c->stream));
     EXH_CHECK_HIP(hipMemcpyAsync(cur_hi, &init_hi, 8, hipMemcpyHostToDevice, c->stream));
@@ -392,6 +557,10 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
     nsmall_host = 1;
   }
   EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  int* seg_need = (int*)(aux + o_need);
+  int* nxa = (int*)(aux + o_nxa);
+  int* nxb = (int*)(aux + o_nxb);
+  int* flags = cnt + 2;  // [2] any segment needs doubling, [3] doubling not converged
   while (nseg > 0) {
     c->stat_sort_levels++;
     hipLaunchKernelGGL(k_seg_ntiles, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, cur_lo,
@@ -405,7 +574,10 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
                        keys, cur_lo, cur_hi, toff, nseg, tle, j0);
     EXH_LAUNCH_CHECK();
     exclusive_scan<long long>(c, tle, tpre, ntile_host);
-    EXH_CHECK_HIP(hipMemsetAsync(cnt, 0, 4, c->stream));  // next-level count
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 1, 0);  // next count
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, flags, 2, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, seg_need,
+                       nseg, 0);
     hipLaunchKernelGGL(k_seg_final<K>, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, keys,
                        vals, kB, vB, cur_lo, cur_hi, toff, tpre, j0, nseg, info, arot,
                        (long long)SMALL, nxt_lo, nxt_hi, cnt, slo, shi, cnt + 1);
@@ -414,13 +586,34 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
                        keys, vals, kB, vB, toff, tpre, nseg, info, arot, s0);
     EXH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_tile_chase<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
-                       keys, vals, kB, vB, toff, nseg, info, arot, s0);
+                       keys, vals, kB, vB, toff, nseg, info, arot, s0, seg_need, flags);
     EXH_LAUNCH_CHECK();
+    EXH_CHECK_HIP(hipMemcpyAsync(hc, cnt, 12, hipMemcpyDeviceToHost, c->stream));
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (hc[2]) {
+      // extreme pivots: finish their chases by pointer doubling
+      c->stat_sort_doubling++;
+      hipLaunchKernelGGL(k_double_init, dim3((unsigned)ntile_host), dim3(256), 0, c->stream, toff,
+                         nseg, info, seg_need, arot, nxa);
+      EXH_LAUNCH_CHECK();
+      for (int round = 0; round < 40; round++) {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, flags + 1, 1, 0);
+        hipLaunchKernelGGL(k_double_round, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
+                           toff, nseg, info, seg_need, nxa, nxb, flags + 1);
+        EXH_LAUNCH_CHECK();
+        std::swap(nxa, nxb);
+        EXH_CHECK_HIP(hipMemcpyAsync(hc + 3, flags + 1, 4, hipMemcpyDeviceToHost, c->stream));
+        EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+        if (!hc[3]) break;
+        EXH_REQUIRE(round < 39, EXH_ERR_INTERNAL, "peak sort: pointer doubling did not converge");
+      }
+      hipLaunchKernelGGL(k_tile_chase2<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
+                         keys, vals, kB, vB, toff, nseg, info, seg_need, nxa, s0);
+      EXH_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_tile_copyback<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
                        keys, vals, kB, vB, toff, nseg, info);
     EXH_LAUNCH_CHECK();
-    EXH_CHECK_HIP(hipMemcpyAsync(hc, cnt, 8, hipMemcpyDeviceToHost, c->stream));
-    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
     nseg = hc[0];
     nsmall_host = hc[1];
     EXH_REQUIRE(nseg <= maxseg && nsmall_host <= maxsmall, EXH_ERR_INTERNAL,

@@ -39,9 +39,13 @@ __device__ inline T wrap_while(T x, T L) {
   return x;
 }
 
+// Hardware float/double atomic add (global_atomic_add_f32/f64).  HIP's
+// atomicAdd(float*) defaults to a compare-and-swap loop, which serialises
+// under the contention of neighbouring particles; the grids live in
+// coarse-grained hipMalloc memory, where the hardware instruction is exact.
 template <typename T>
 __device__ inline void atomic_add_t(T* p, T v) {
-  atomicAdd(p, v);
+  unsafeAtomicAdd(p, v);
 }
 
 template <typename T>
@@ -119,6 +123,15 @@ __global__ void k_deposit(const T* pos, const T* vel, const T* mass, const int* 
                           T* grid, unsigned long long* counts) {
   const size_t ng = (size_t)nd * nd * nd;
   const T Ld = L / (T)nd;
+  // particle counts per type (the normalisation's total mass): registers
+  // for up to 4 types, an LDS histogram beyond; one atomic per block and
+  // type at the end instead of one per particle on a single address
+  __shared__ unsigned long long sh_cnt[64];
+  unsigned long long reg_cnt[4] = {0, 0, 0, 0};
+  const bool count = !mass && counts;
+  const int ntype_reg = 4;
+  if (threadIdx.x < 64) sh_cnt[threadIdx.x] = 0;
+  __syncthreads();
   for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < np;
        j += (long long)gridDim.x * blockDim.x) {
     T x[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
@@ -139,7 +152,223 @@ __global__ void k_deposit(const T* pos, const T* vel, const T* mass, const int* 
       if (WIN == 2) dep_cic(g + ng, p, nd, Ld, m);
       if (WIN == 5) dep_tsc(g + ng, p, nd, Ld, m);
     }
-    if (!mass && counts) atomicAdd(&counts[t], 1ull);
+    if (count) {
+      if (t < ntype_reg) {
+        for (int q = 0; q < ntype_reg; q++) reg_cnt[q] += (q == t);
+      } else {
+        atomicAdd(&sh_cnt[t], 1ull);
+      }
+    }
+  }
+  if (!count) return;
+  for (int q = 0; q < ntype_reg; q++)
+    if (reg_cnt[q]) atomicAdd(&sh_cnt[q], reg_cnt[q]);
+  __syncthreads();
+  if (threadIdx.x < 64 && sh_cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sh_cnt[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-privatised deposit for particles stored in Lagrangian lattice order
+// (particle (i, j, k) of an nL^3 lattice at index (i nL + j) nL + k, as the
+// LPT positions are).  One workgroup takes a TL^3 tile of the lattice: its
+// particles, displaced by a smooth field, land in a compact Eulerian box.
+// The block finds that box (cells relative to the tile centre, unwrapped
+// across the periodic boundary), accumulates the window weights there with
+// LDS atomics, and adds each touched cell to the grid with one coalesced
+// global atomic; per particle that is ~2 global atomics instead of 8
+// (CIC).  Tiles whose box exceeds the LDS capacity deposit directly.
+// Weights are computed exactly as in dep_ngp/dep_cic/dep_tsc.
+// ---------------------------------------------------------------------------
+__device__ inline int rel_cell(int c, int qc, int nd) {
+  int d = c - qc;
+  if (d >= nd / 2) d -= nd;
+  if (d < -nd / 2) d += nd;
+  return d;
+}
+
+__device__ inline int wrap_cell(int c, int nd) {
+  c %= nd;
+  return c < 0 ? c + nd : c;
+}
+
+// Cells and weights of one particle along one axis for window WIN.
+template <typename T, int WIN>
+__device__ inline int axis_stencil(T x, T Ld, int nd, int* cell, T* w) {
+  if (WIN == 1) {
+    int xt = (int)floor(x / Ld);
+    if (xt == nd) xt -= 1;
+    cell[0] = xt;
+    w[0] = (T)1;
+    return 1;
+  } else if (WIN == 2) {
+    T q = x / Ld;
+    int xt = (int)floor(q);
+    if (xt == nd) xt -= 1;
+    T dx = (T)((double)q - ((double)xt + 0.5));
+    int o;
+    if (dx >= (T)0) {
+      o = xt + 1 == nd ? 0 : xt + 1;
+    } else {
+      o = xt == 0 ? nd - 1 : xt - 1;
+      dx = -dx;
+    }
+    cell[0] = xt;
+    w[0] = (T)(1.0 - (double)dx);
+    cell[1] = o;
+    w[1] = dx;
+    return 2;
+  } else {
+    double u = (double)x / (double)Ld - 0.5;
+    double cf = floor(u + 0.5);
+    double d = u - cf;
+    int c = (int)cf;
+    cell[0] = wrap_cell(c - 1, nd);
+    cell[1] = wrap_cell(c, nd);
+    cell[2] = wrap_cell(c + 1, nd);
+    w[0] = (T)(0.5 * (0.5 - d) * (0.5 - d));
+    w[1] = (T)(0.75 - d * d);
+    w[2] = (T)(0.5 * (0.5 + d) * (0.5 + d));
+    return 3;
+  }
+}
+
+template <int N>
+__device__ inline void block_minmax(int* v_min, int* v_max, int* sh) {
+  // sh: 2 * N * (blockDim/64) ints; reduces N axes at once
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int a = 0; a < N; a++) {
+    int mn = v_min[a], mx = v_max[a];
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = min(mn, __shfl_xor(mn, o, 64));
+      mx = max(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) {
+      sh[(2 * a) * nw + wave] = mn;
+      sh[(2 * a + 1) * nw + wave] = mx;
+    }
+  }
+  __syncthreads();
+  for (int a = 0; a < N; a++) {
+    int mn = sh[(2 * a) * nw], mx = sh[(2 * a + 1) * nw];
+    for (int w = 1; w < nw; w++) {
+      mn = min(mn, sh[(2 * a) * nw + w]);
+      mx = max(mx, sh[(2 * a + 1) * nw + w]);
+    }
+    v_min[a] = mn;
+    v_max[a] = mx;
+  }
+  __syncthreads();
+}
+
+static constexpr int TILE_L = 16;        // lattice tile side
+static constexpr int TILE_THREADS = 1024;
+static constexpr int TILE_PPT = TILE_L * TILE_L * TILE_L / TILE_THREADS;  // particles per thread
+
+template <typename T, int WIN>
+__global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
+    const T* pos, const T* vel, const T* mass, int nL, int nd, T L, T Lb, int direction, T rsd,
+    int nsub, int cap, T* grid, unsigned long long* counts) {
+  extern __shared__ unsigned char lds_raw[];
+  T* box = (T*)lds_raw;
+  __shared__ int red[2 * 3 * (TILE_THREADS / 64)];
+  __shared__ unsigned long long nvalid;
+  constexpr int NW = WIN == 1 ? 1 : (WIN == 2 ? 2 : 3);
+  const size_t ng = (size_t)nd * nd * nd;
+  const T Ld = L / (T)nd;
+  const int nt = nL / TILE_L;
+  const int tz = blockIdx.x % nt, ty = (blockIdx.x / nt) % nt, tx = blockIdx.x / (nt * nt);
+  // tile centre in grid cells (Lagrangian lattice mapped onto the grid)
+  const double scale = (double)nd / (double)nL;
+  const int qc[3] = {(int)((tx * TILE_L + TILE_L / 2) * scale) % nd,
+                     (int)((ty * TILE_L + TILE_L / 2) * scale) % nd,
+                     (int)((tz * TILE_L + TILE_L / 2) * scale) % nd};
+  if (threadIdx.x == 0) nvalid = 0;
+  // particles of this thread: lanes along k (coalesced rows)
+  T xs[TILE_PPT][3];
+  T ms[TILE_PPT];
+  bool ok[TILE_PPT];
+  unsigned long long myvalid = 0;
+  for (int p = 0; p < TILE_PPT; p++) {
+    int lin = threadIdx.x + p * TILE_THREADS;  // 0..4095
+    int c = lin % TILE_L, b = (lin / TILE_L) % TILE_L, a = lin / (TILE_L * TILE_L);
+    size_t j = ((size_t)(tx * TILE_L + a) * nL + (ty * TILE_L + b)) * nL + (tz * TILE_L + c);
+    T x[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    if (direction >= 0) x[direction] = wrap_while<T>(x[direction] + vel[3 * j + direction] * rsd, L);
+    ok[p] = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
+    for (int a2 = 0; a2 < 3; a2++) xs[p][a2] = x[a2];
+    ms[p] = mass ? mass[j] : (T)1;
+    myvalid += ok[p];
+  }
+  for (int s = 0; s < nsub; s++) {
+    T* g = grid + (size_t)s * ng;
+    const T sh = s == 0 ? (T)0 : (T)(-(double)Lb / 2.0);
+    // box extent of this sub-grid's stencils
+    int vmin[3] = {1 << 30, 1 << 30, 1 << 30}, vmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    for (int p = 0; p < TILE_PPT; p++) {
+      if (!ok[p]) continue;
+      for (int a = 0; a < 3; a++) {
+        T pa = wrap_while<T>(xs[p][a] + sh, L);
+        int cell[3];
+        T w[3];
+        int n = axis_stencil<T, WIN>(pa, Ld, nd, cell, w);
+        for (int q = 0; q < n; q++) {
+          int r = rel_cell(cell[q], qc[a], nd);
+          vmin[a] = min(vmin[a], r);
+          vmax[a] = max(vmax[a], r);
+        }
+      }
+    }
+    block_minmax<3>(vmin, vmax, red);
+    const int e0 = vmax[0] - vmin[0] + 1, e1 = vmax[1] - vmin[1] + 1, e2 = vmax[2] - vmin[2] + 1;
+    const bool fits = vmax[0] >= vmin[0] && (long long)e0 * e1 * e2 <= cap;
+    if (fits) {
+      const int vol = e0 * e1 * e2;
+      for (int v = threadIdx.x; v < vol; v += TILE_THREADS) box[v] = (T)0;
+      __syncthreads();
+    }
+    for (int p = 0; p < TILE_PPT; p++) {
+      if (!ok[p]) continue;
+      int cell[3][3];
+      T w[3][3];
+      for (int a = 0; a < 3; a++) {
+        T pa = wrap_while<T>(xs[p][a] + sh, L);
+        axis_stencil<T, WIN>(pa, Ld, nd, cell[a], w[a]);
+      }
+      for (int a = 0; a < NW; a++) {
+        for (int b = 0; b < NW; b++) {
+          T wab = ms[p] * w[0][a] * w[1][b];
+          for (int c = 0; c < NW; c++) {
+            T val = wab * w[2][c];
+            if (fits) {
+              int lx = rel_cell(cell[0][a], qc[0], nd) - vmin[0];
+              int ly = rel_cell(cell[1][b], qc[1], nd) - vmin[1];
+              int lz = rel_cell(cell[2][c], qc[2], nd) - vmin[2];
+              atomicAdd(&box[(lx * e1 + ly) * e2 + lz], val);
+            } else {
+              atomic_add_t(g + ((size_t)cell[0][a] * nd + cell[1][b]) * nd + cell[2][c], val);
+            }
+          }
+        }
+      }
+    }
+    if (fits) {
+      __syncthreads();
+      const int vol = e0 * e1 * e2;
+      for (int v = threadIdx.x; v < vol; v += TILE_THREADS) {
+        T val = box[v];
+        if (val == (T)0) continue;
+        int lz = v % e2, ly = (v / e2) % e1, lx = v / (e1 * e2);
+        int cx = wrap_cell(qc[0] + vmin[0] + lx, nd), cy = wrap_cell(qc[1] + vmin[1] + ly, nd),
+            cz = wrap_cell(qc[2] + vmin[2] + lz, nd);
+        atomic_add_t(g + ((size_t)cx * nd + cy) * nd + cz, val);
+      }
+      __syncthreads();
+    }
+  }
+  if (!mass && counts) {
+    atomicAdd(&nvalid, myvalid);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&counts[0], nvalid);
   }
 }
 
@@ -180,23 +409,29 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   T rsd = (T)(std::pow(1.0 + (T)z, 3.0) / (double)Hz);
   unsigned long long* counts = (unsigned long long*)ctx_buf(c, BUF_MISC, 64 * 8);
   EXH_REQUIRE(ntype <= 64, EXH_ERR_ARG, "at most 64 tracer types");
-  TimedScope ts(c, "deposit");
-  hipLaunchKernelGGL(k_zero<T>, dim3(grid_for(ng * ntype * nsub, 256 * 8)), dim3(256), 0,
-                     c->stream, grid, ng * ntype * nsub);
-  hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(1), dim3(64), 0, c->stream, counts,
-                     (size_t)64);
-  EXH_LAUNCH_CHECK();
+  {
+    TimedScope ts(c, "grid_zero");
+    hipLaunchKernelGGL(k_zero<T>, dim3(grid_for(ng * ntype * nsub, 256 * 8)), dim3(256), 0,
+                       c->stream, grid, ng * ntype * nsub);
+    hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(1), dim3(64), 0, c->stream, counts,
+                       (size_t)64);
+    EXH_LAUNCH_CHECK();
+  }
   unsigned gsz = grid_for(np, 256);
   if (gsz > 65536 * 4) gsz = 65536 * 4;
+  {
+    TimedScope ts(c, window == 2 ? "cic_deposit" : (window == 1 ? "ngp_deposit" : "tsc_deposit"));
 #define EXH_DEP(W)                                                                         \
   hipLaunchKernelGGL((k_deposit<T, W>), dim3(gsz), dim3(256), 0, c->stream, pos, vel, mass, \
                      type, np, nd, Lf, Lb, direction, rsd, nsub, grid, counts)
-  if (window == 1) EXH_DEP(1);
-  if (window == 2) EXH_DEP(2);
-  if (window == 5) EXH_DEP(5);
+    if (window == 1) EXH_DEP(1);
+    if (window == 2) EXH_DEP(2);
+    if (window == 5) EXH_DEP(5);
 #undef EXH_DEP
-  EXH_LAUNCH_CHECK();
+    EXH_LAUNCH_CHECK();
+  }
   if (!mass) {
+    TimedScope ts(c, "grid_normalise");
     hipLaunchKernelGGL(k_normalise<T>, dim3(grid_for(ng * ntype * nsub, 256 * 8)), dim3(256), 0,
                        c->stream, grid, ng, ntype * nsub, nsub, counts);
     EXH_LAUNCH_CHECK();
@@ -357,18 +592,37 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
   }
 }
 
+// One block per output value: 256 threads stride over the block partials,
+// then a fixed-shape tree; the result does not depend on timing.
 __global__ void k_power_reduce(const double* part, const unsigned* pcnt, int nblocks, int Nk,
                                int NV, double* sum, unsigned long long* cnt) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < Nk * NV) {
-    double s = 0.0;
-    for (int bl = 0; bl < nblocks; bl++) s += part[(size_t)bl * Nk * NV + q];
-    sum[q] = s;
+  __shared__ double sd[256];
+  __shared__ unsigned long long sc[256];
+  const int q = blockIdx.x;
+  const bool is_cnt = q >= Nk * NV;
+  double s = 0.0;
+  unsigned long long u = 0;
+  for (int bl = threadIdx.x; bl < nblocks; bl += 256) {
+    if (is_cnt)
+      u += pcnt[(size_t)bl * Nk + (q - Nk * NV)];
+    else
+      s += part[(size_t)bl * Nk * NV + q];
   }
-  if (q < Nk) {
-    unsigned long long s = 0;
-    for (int bl = 0; bl < nblocks; bl++) s += pcnt[(size_t)bl * Nk + q];
-    cnt[q] = s;
+  sd[threadIdx.x] = s;
+  sc[threadIdx.x] = u;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sd[threadIdx.x] += sd[threadIdx.x + w];
+      sc[threadIdx.x] += sc[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (is_cnt)
+      cnt[q - Nk * NV] = sc[0];
+    else
+      sum[q] = sd[0];
   }
 }
 
@@ -486,7 +740,10 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
     hipLaunchKernelGGL(k_power_bins<T>, dim3(nblocks), dim3(64 * waves), per_wave * waves,
                        c->stream, a, gk, part, pcnt);
     EXH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_power_reduce, dim3(grid_for((long long)Nk * NV, 256)), dim3(256), 0,
+  }
+  {
+    TimedScope ts(c, "power_reduce");
+    hipLaunchKernelGGL(k_power_reduce, dim3((unsigned)(Nk * NV + Nk)), dim3(256), 0,
                        c->stream, part, pcnt, nblocks, Nk, NV, sum, cnt);
     EXH_LAUNCH_CHECK();
   }

@@ -188,6 +188,7 @@ def timings(ctx):
 
 
 def halo_stats(ctx):
-    out = (C.c_longlong * 4)()
+    out = (C.c_longlong * 5)()
     check(load().exh_ctx_halo_stats(ctx, out), "exh_ctx_halo_stats")
-    return dict(peaks=out[0], candidates=out[1], rounds=out[2], sort_levels=out[3])
+    return dict(peaks=out[0], candidates=out[1], rounds=out[2], sort_levels=out[3],
+                sort_doubling=out[4])

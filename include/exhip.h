@@ -109,9 +109,10 @@ int exh_ctx_synchronize(exh_ctx* ctx);
  * names separated by '\n' in `names`. */
 int exh_ctx_set_timing(exh_ctx* ctx, int enable);
 int exh_ctx_timings(exh_ctx* ctx, char* names, int names_len, double* ms, int max);
-/* Halo-finder statistics of the last run: peaks, candidates, growth rounds,
- * sort levels. */
-int exh_ctx_halo_stats(exh_ctx* ctx, long long* out4);
+/* Halo-finder statistics of the last run (5 values): peaks, candidates,
+ * growth rounds, device-wide sort levels, sort levels that needed pointer
+ * doubling. */
+int exh_ctx_halo_stats(exh_ctx* ctx, long long* out5);
 
 /* set_cosmology + set_box (exshalos_h.c:43-106).  Lc_or_Mc < 100 is a cell
  * size in Mpc/h, > 1e5 a cell mass in Msun/h, anything else EXH_ERR_ARG. */

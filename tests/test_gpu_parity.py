@@ -186,3 +186,33 @@ def test_halos_box_from_pk_config1(bbks, prec):
     d = np.abs(out["pos"].astype(np.float64) - pos)
     d = np.minimum(d, 256.0 - d)
     assert d.max() < (2e-3 if prec == 4 else 1e-8)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_sort_extreme_pivots_and_ties(bbks, prec):
+    """Isolated one-cell spikes on every other cell, heights nearly sorted in
+    lexicographic order and quantised: Lomuto's last-element pivots are then
+    extreme (long queue rotations, the pointer-doubling path) and ties are
+    everywhere.  Every spike is its own one-cell halo, so the catalogue order
+    is the sort order: it must equal the oracle's exactly."""
+    from exshalos_amd.lib import _native as N
+
+    k, P = bbks
+    set_precision(prec)
+    nd = 64
+    R = np.float32 if prec == 4 else np.float64
+    rng = np.random.default_rng(5)
+    n = (nd // 2) ** 3
+    v = 1.7 + 1.3 * np.arange(n) / n + rng.normal(0, 0.02, n)
+    v = np.round(v * 4096) / 4096
+    delta = np.zeros((nd, nd, nd), dtype=R)
+    delta[::2, ::2, ::2] = v.reshape(nd // 2, nd // 2, nd // 2)
+    ref = O.find_halos(delta, k, P, LC, seed=12345, precision=prec)
+    out = X.find_halos(delta, k, P, LC, 0.31, 0.0, -1.0, 1, 1.0, 0.0, 0.0, 1, 0, 0)
+    cat = X.halo_catalogue()
+    stats = N.halo_stats(N.context())
+    assert stats["sort_doubling"] > 0  # the extreme-pivot path ran
+    assert len(cat["count"]) == len(ref["count"]) > n // 2
+    np.testing.assert_array_equal(cat["x"], ref["x"])
+    np.testing.assert_array_equal(out["Mh"], ref["Mh"])
+    np.testing.assert_array_equal(out["flag"].reshape(-1), ref["flag"].reshape(-1))
