@@ -20,6 +20,9 @@
 #include "kernels.hpp"
 #include "util.hpp"
 
+#include <cmath>
+#include <cstdlib>
+
 namespace exh {
 
 // ---------------------------------------------------------------------------
@@ -268,7 +271,7 @@ template <typename T, int WIN>
 __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
     const T* pos, const T* vel, const T* mass, int nL, int nd, T L, T Lb, int direction, T rsd,
     int nsub, int cap, T* grid, unsigned long long* counts) {
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   T* box = (T*)lds_raw;
   __shared__ int red[2 * 3 * (TILE_THREADS / 64)];
   __shared__ unsigned long long nvalid;
@@ -339,10 +342,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
           T wab = ms[p] * w[0][a] * w[1][b];
           for (int c = 0; c < NW; c++) {
             T val = wab * w[2][c];
-            if (fits) {
-              int lx = rel_cell(cell[0][a], qc[0], nd) - vmin[0];
-              int ly = rel_cell(cell[1][b], qc[1], nd) - vmin[1];
-              int lz = rel_cell(cell[2][c], qc[2], nd) - vmin[2];
+            const int lx = rel_cell(cell[0][a], qc[0], nd) - vmin[0];
+            const int ly = rel_cell(cell[1][b], qc[1], nd) - vmin[1];
+            const int lz = rel_cell(cell[2][c], qc[2], nd) - vmin[2];
+            // the box bounds were taken over these same stencils; the
+            // range test only guards the LDS index
+            if (fits && (unsigned)lx < (unsigned)e0 && (unsigned)ly < (unsigned)e1 &&
+                (unsigned)lz < (unsigned)e2) {
               atomicAdd(&box[(lx * e1 + ly) * e2 + lz], val);
             } else {
               atomic_add_t(g + ((size_t)cell[0][a] * nd + cell[1][b]) * nd + cell[2][c], val);
@@ -391,6 +397,28 @@ __global__ void k_normalise(T* grid, size_t ng, int ngrids, int nsub,
   }
 }
 
+// Side of the particle lattice when np is a perfect cube, else 0.
+static int lattice_side(long long np) {
+  if (np <= 0) return 0;
+  long long n = (long long)std::llround(std::cbrt((double)np));
+  for (long long m = n > 1 ? n - 1 : 1; m <= n + 1; m++)
+    if (m * m * m == np) return (int)m;
+  return 0;
+}
+
+// LDS bytes of the tiled deposit's Eulerian box: EXH_DEPOSIT_LDS_KB (0
+// disables the tiled path), default 144 KB of the 160 KB per workgroup.
+static int deposit_lds_bytes() {
+  static int v = [] {
+    const char* e = std::getenv("EXH_DEPOSIT_LDS_KB");
+    int kb = e ? std::atoi(e) : 144;
+    if (kb < 0) kb = 0;
+    if (kb > 156) kb = 156;
+    return kb * 1024;
+  }();
+  return v;
+}
+
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
@@ -419,15 +447,37 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   }
   unsigned gsz = grid_for(np, 256);
   if (gsz > 65536 * 4) gsz = 65536 * 4;
+  // particles on a lattice whose side is a multiple of the tile side are
+  // deposited tile by tile through LDS (correct for any order; fast when the
+  // array is in lattice order, as LPT particles are)
+  const int nL = lattice_side(np);
+  const bool tiles = type == nullptr && nL >= TILE_L && nL % TILE_L == 0 && deposit_lds_bytes() > 0;
   {
     TimedScope ts(c, window == 2 ? "cic_deposit" : (window == 1 ? "ngp_deposit" : "tsc_deposit"));
+    if (tiles) {
+      const int lds = deposit_lds_bytes();
+      const int cap = lds / (int)sizeof(T);
+      const unsigned nblk = (unsigned)((nL / TILE_L) * (nL / TILE_L) * (nL / TILE_L));
+#define EXH_DEPT(W)                                                                            \
+  do {                                                                                         \
+    EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_deposit_tiles<T, W>,                      \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));       \
+    hipLaunchKernelGGL((k_deposit_tiles<T, W>), dim3(nblk), dim3(TILE_THREADS), lds, c->stream, \
+                       pos, vel, mass, nL, nd, Lf, Lb, direction, rsd, nsub, cap, grid, counts); \
+  } while (0)
+      if (window == 1) EXH_DEPT(1);
+      if (window == 2) EXH_DEPT(2);
+      if (window == 5) EXH_DEPT(5);
+#undef EXH_DEPT
+    } else {
 #define EXH_DEP(W)                                                                         \
   hipLaunchKernelGGL((k_deposit<T, W>), dim3(gsz), dim3(256), 0, c->stream, pos, vel, mass, \
                      type, np, nd, Lf, Lb, direction, rsd, nsub, grid, counts)
-    if (window == 1) EXH_DEP(1);
-    if (window == 2) EXH_DEP(2);
-    if (window == 5) EXH_DEP(5);
+      if (window == 1) EXH_DEP(1);
+      if (window == 2) EXH_DEP(2);
+      if (window == 5) EXH_DEP(5);
 #undef EXH_DEP
+    }
     EXH_LAUNCH_CHECK();
   }
   if (!mass) {

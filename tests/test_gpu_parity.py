@@ -122,6 +122,39 @@ def test_grid_compute(prec, window, il):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("window", [1, 2, 5])
+@pytest.mark.parametrize("il", [0, 1])
+@pytest.mark.parametrize("nd", [32, 64])
+def test_grid_compute_lattice_tiles(prec, window, il, nd):
+    """Particles in lattice order (np = 32^3) take the LDS-tiled deposit.
+    nd = 32: tile boxes fit in LDS, including a tile whose box wraps the
+    whole periodic grid (an outlier at the antipode); nd = 64: the 16-particle
+    tiles span 32+ cells a side and take the direct path.  Weighted and
+    redshift-space variants run through the same tiles."""
+    set_precision(prec)
+    R = np.float32 if prec == 4 else np.float64
+    nL, L = 32, 128.0
+    rng = np.random.default_rng(13 + window + 2 * il + nd)
+    q = (np.indices((nL, nL, nL)).reshape(3, -1).T + 0.5) * (L / nL)
+    pos = q + rng.normal(0, 2 * L / nd, q.shape)
+    pos[0] += L / 2  # antipodal outlier in the first tile
+    pos[5 * nL * nL + 3 * nL + 7] -= (L / 3, 0.0, L / 4)
+    pos = np.mod(pos, L).astype(R)
+    ref = O.grid_compute(pos, nd, L, window=window, interlacing=bool(il), precision=prec)
+    g = SP.grid_compute(pos, None, None, 0, None, 1, nd, L, 0.31, 0.0, -1, window, 4.0, 5.0, il,
+                        1, 0, 1)
+    tol = 2e-5 if prec == 4 else 1e-12
+    assert _rel(g, ref) < tol
+    vel = (rng.normal(size=pos.shape) * 300).astype(R)
+    mass = rng.random(len(pos)).astype(R)
+    ref = O.grid_compute(pos, nd, L, window=window, interlacing=bool(il), mass=mass,
+                         direction=1, vel=vel, precision=prec)
+    g = SP.grid_compute(pos, vel, mass, 1, None, 1, nd, L, 0.31, 0.0, 1, window, 4.0, 5.0, il,
+                        1, 0, 1)
+    assert _rel(g, ref) < tol
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 def test_grid_compute_weighted_types_rsd(prec):
     set_precision(prec)
     rng = np.random.default_rng(4)
