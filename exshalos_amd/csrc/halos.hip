@@ -53,40 +53,58 @@ __device__ inline bool is_peak(const T* d, size_t c, int n0, int n1, int n2) {
 static constexpr int PK_ITEMS = 8;
 static constexpr int PK_TILE = 256 * PK_ITEMS;
 
+// Peaks are found over tiles of PK_TILE consecutive cells; thread x of a
+// block tests cells tile + q * 256 + x (q < PK_ITEMS), so every load row is
+// coalesced.  The write pass ranks peaks in cell order from per-row wave
+// ballots, which keeps the reference's lexicographic peak order.
 template <typename T>
-__global__ void k_peak_count(const T* d, int n0, int n1, int n2, long long* bcnt) {
-  __shared__ long long sh[4];
-  size_t ng = (size_t)n0 * n1 * n2;
-  size_t base = (size_t)blockIdx.x * PK_TILE + (size_t)threadIdx.x * PK_ITEMS;
-  long long cnt = 0;
-  for (int q = 0; q < PK_ITEMS; q++)
-    if (base + q < ng && is_peak(d, base + q, n0, n1, n2)) cnt++;
-  long long tot;
-  block_exclusive_256<long long>(cnt, sh, &tot);
-  if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
+__global__ __launch_bounds__(256) void k_peak_count(const T* d, int n0, int n1, int n2,
+                                                    long long* bcnt) {
+  __shared__ int sh[4];
+  const size_t ng = (size_t)n0 * n1 * n2;
+  const size_t base = (size_t)blockIdx.x * PK_TILE + threadIdx.x;
+  int cnt = 0;
+  for (int q = 0; q < PK_ITEMS; q++) {
+    const size_t c = base + (size_t)q * 256;
+    if (c < ng && is_peak(d, c, n0, n1, n2)) cnt++;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = (long long)sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 template <typename T>
-__global__ void k_peak_write(const T* d, int n0, int n1, int n2, const long long* boff,
-                             T* key, int* cell, int* idx) {
-  __shared__ long long sh[4];
-  size_t ng = (size_t)n0 * n1 * n2;
-  size_t base = (size_t)blockIdx.x * PK_TILE + (size_t)threadIdx.x * PK_ITEMS;
-  bool pk[PK_ITEMS];
-  long long cnt = 0;
+__global__ __launch_bounds__(256) void k_peak_write(const T* d, int n0, int n1, int n2,
+                                                    const long long* boff, T* key, int* cell,
+                                                    int* idx) {
+  __shared__ int wcnt[PK_ITEMS][4];
+  const size_t ng = (size_t)n0 * n1 * n2;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const size_t base = (size_t)blockIdx.x * PK_TILE + threadIdx.x;
+  unsigned long long bal[PK_ITEMS];
+  T v[PK_ITEMS];
   for (int q = 0; q < PK_ITEMS; q++) {
-    pk[q] = base + q < ng && is_peak(d, base + q, n0, n1, n2);
-    cnt += pk[q];
+    const size_t c = base + (size_t)q * 256;
+    const bool pk = c < ng && is_peak(d, c, n0, n1, n2);
+    v[q] = pk ? d[c] : (T)0;
+    bal[q] = __ballot(pk);
+    if (lane == 0) wcnt[q][w] = __popcll(bal[q]);
   }
-  long long tot;
-  long long o = block_exclusive_256<long long>(cnt, sh, &tot) + boff[blockIdx.x];
-  for (int q = 0; q < PK_ITEMS; q++)
-    if (pk[q]) {
-      key[o] = d[base + q];
-      cell[o] = (int)(base + q);
-      idx[o] = (int)o;
-      o++;
+  __syncthreads();
+  long long o = boff[blockIdx.x];
+  const unsigned long long below = lanemask_lt();
+  for (int q = 0; q < PK_ITEMS; q++) {
+    long long ow = o;
+    for (int u = 0; u < w; u++) ow += wcnt[q][u];
+    if ((bal[q] >> lane) & 1ull) {
+      const long long at = ow + __popcll(bal[q] & below);
+      key[at] = v[q];
+      cell[at] = (int)(base + (size_t)q * 256);
+      idx[at] = (int)at;
     }
+    o += wcnt[q][0] + wcnt[q][1] + wcnt[q][2] + wcnt[q][3];
+  }
 }
 
 // Candidates by rank (0 = densest): rank r <-> sorted position np-1-r,

@@ -18,6 +18,9 @@
 //    rotations before it) is re-appended at s + A(s) while s < r; its final
 //    slot is lo + c + 1 + (s - r), c = number of "<=" elements.
 // Every stream index below r is visited once, so a partition costs O(n).
+// A chase grows roughly geometrically (s -> s / f for a pivot at quantile
+// f), so chains are short except for early queue entries of segments whose
+// pivot is near the top; those finish by pointer doubling.
 //
 // Large segments (> SMALL) are processed level by level with tile kernels
 // across the whole device; segments of 2..SMALL elements are finished by one
@@ -25,6 +28,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "util.hpp"
+
+#include <cstdlib>
 
 namespace exh {
 
@@ -66,31 +71,57 @@ __device__ inline int find_seg(const long long* tile_off, int nseg, long long t)
   return lo;
 }
 
+// 64-bit wave reductions
+__device__ inline long long wave_min_ll(long long v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (long long)__shfl_xor(v, o, 64));
+  return v;
+}
+
+// Per tile: number of "<=" elements and the first ">" element.  Lanes
+// read consecutive elements; one atomic per block for the segment's j0.
 template <typename K>
-__global__ void k_tile_count(const K* keys, const long long* seg_lo, const long long* seg_hi,
-                             const long long* tile_off, int nseg, long long* tile_le,
-                             unsigned long long* j0) {
-  __shared__ long long sh[4];
-  long long t = blockIdx.x;
-  int s = find_seg(tile_off, nseg, t);
-  long long lo = seg_lo[s], hi = seg_hi[s];
-  K pivot = keys[hi];
-  long long start = lo + (t - tile_off[s]) * TILE;
-  long long cnt = 0, first_gt = hi;
+__global__ __launch_bounds__(256) void k_tile_count(const K* keys, const long long* seg_lo,
+                                                    const long long* seg_hi,
+                                                    const long long* tile_off, int nseg,
+                                                    long long* tile_le, unsigned long long* j0) {
+  __shared__ int shc[4];
+  __shared__ long long shj[4];
+  const long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) {  // the grid is an upper bound of the tile count
+    if (threadIdx.x == 0) tile_le[t] = 0;
+    return;
+  }
+  const int s = find_seg(tile_off, nseg, t);
+  const long long lo = seg_lo[s], hi = seg_hi[s];
+  const K pivot = keys[hi];
+  const long long start = lo + (t - tile_off[s]) * TILE;
+  int cnt = 0;
+  long long first_gt = hi;
   for (int q = 0; q < PER_THREAD; q++) {
-    long long j = start + (long long)threadIdx.x * PER_THREAD + q;
+    const long long j = start + (long long)q * 256 + threadIdx.x;
     if (j < hi) {
       if (keys[j] <= pivot)
         cnt++;
-      else if (j < first_gt)
-        first_gt = j;
+      else
+        first_gt = min(first_gt, j);
     }
   }
-  long long tot;
-  block_exclusive_256<long long>(cnt, sh, &tot);
-  if (threadIdx.x == 0) tile_le[t] = tot;
-  // first ">" element of the segment
-  if (first_gt < hi) atomicMin(&j0[s], (unsigned long long)first_gt);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  first_gt = wave_min_ll(first_gt);
+  if (lane == 0) {
+    shc[w] = cnt;
+    shj[w] = first_gt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    tile_le[t] = (long long)shc[0] + shc[1] + shc[2] + shc[3];
+    const long long fg = min(min(shj[0], shj[1]), min(shj[2], shj[3]));
+    // most tiles come after the segment's first ">": a plain read of the
+    // running minimum skips their atomic on this one address
+    if (fg < hi && (unsigned long long)fg < __atomic_load_n(&j0[s], __ATOMIC_RELAXED))
+      atomicMin(&j0[s], (unsigned long long)fg);
+  }
 }
 
 template <typename K>
@@ -98,9 +129,9 @@ __global__ void k_seg_final(const K* keysA, const int* valsA, K* keysB, int* val
                             const long long* seg_lo, const long long* seg_hi,
                             const long long* tile_off, const long long* tile_pre,
                             const unsigned long long* j0s, int nseg, SegInfo* info,
-                            long long* arot, long long small_max, long long* next_lo,
+                            int* arot, long long small_max, long long* next_lo,
                             long long* next_hi, int* n_next, long long* small_lo,
-                            long long* small_hi, int* n_small) {
+                            long long* small_hi, int* n_small, unsigned long long* next_elems) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nseg) return;
   long long lo = seg_lo[s], hi = seg_hi[s];
@@ -119,7 +150,7 @@ __global__ void k_seg_final(const K* keysA, const int* valsA, K* keysB, int* val
   info[s] = in;
   keysB[lo + c] = keysA[hi];
   valsB[lo + c] = valsA[hi];
-  if (m > 0) arot[lo + r - 1] = m;
+  if (m > 0) arot[lo + r - 1] = (int)m;
   // children
   long long cl[2] = {lo, lo + c + 1}, ch[2] = {lo + c - 1, hi};
   for (int q = 0; q < 2; q++) {
@@ -128,6 +159,7 @@ __global__ void k_seg_final(const K* keysA, const int* valsA, K* keysB, int* val
       int p = atomicAdd(n_next, 1);
       next_lo[p] = cl[q];
       next_hi[p] = ch[q];
+      atomicAdd(next_elems, (unsigned long long)len);
     } else if (len >= 2) {
       int p = atomicAdd(n_small, 1);
       small_lo[p] = cl[q];
@@ -136,86 +168,157 @@ __global__ void k_seg_final(const K* keysA, const int* valsA, K* keysB, int* val
   }
 }
 
+// Place the "<=" elements at lo + rank, record the rotation table A and
+// each ">" element's first stream index.  Wave w of a tile takes its
+// elements [w * 512, (w + 1) * 512) in eight coalesced rows of 64; ranks
+// come from ballots, the wave offsets from LDS.
 template <typename K>
-__global__ void k_tile_scatter(const K* keysA, const int* valsA, K* keysB, int* valsB,
-                               const long long* tile_off, const long long* tile_pre, int nseg,
-                               const SegInfo* info, long long* arot, long long* s0) {
-  __shared__ long long sh[4];
-  long long t = blockIdx.x;
-  int s = find_seg(tile_off, nseg, t);
-  SegInfo g = info[s];
-  K pivot = keysA[g.hi];
-  long long start = g.lo + (t - g.first_tile) * TILE + (long long)threadIdx.x * PER_THREAD;
-  bool le[PER_THREAD];
-  long long cnt = 0;
+__global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int* valsA, K* keysB,
+                                                      int* valsB, const long long* tile_off,
+                                                      const long long* tile_pre, int nseg,
+                                                      const SegInfo* info, int* arot, int* s0) {
+  __shared__ int shw[4];
+  const long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) return;
+  const int sg = find_seg(tile_off, nseg, t);
+  const SegInfo g = info[sg];
+  const K pivot = keysA[g.hi];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long long base = g.lo + (t - g.first_tile) * TILE + (long long)w * (TILE / 4);
+  K kv[PER_THREAD];
+  unsigned long long bal[PER_THREAD];
+  int wcnt = 0;
   for (int q = 0; q < PER_THREAD; q++) {
-    long long j = start + q;
-    le[q] = (j < g.hi) && (keysA[j] <= pivot);
-    cnt += le[q] ? 1 : 0;
+    const long long j = base + q * 64 + lane;
+    const bool in = j < g.hi;
+    kv[q] = in ? keysA[j] : pivot;
+    bal[q] = __ballot(in && kv[q] <= pivot);
+    wcnt += __popcll(bal[q]);
   }
-  long long tot;
-  long long rank = block_exclusive_256<long long>(cnt, sh, &tot) + tile_pre[t] - tile_pre[g.first_tile];
+  if (lane == 0) shw[w] = wcnt;
+  __syncthreads();
+  long long run = tile_pre[t] - tile_pre[g.first_tile];
+  for (int v = 0; v < w; v++) run += shw[v];
+  const unsigned long long below = lanemask_lt();
+  const long long jlead = g.j0 - g.lo;  // "<=" elements before the first ">"
   for (int q = 0; q < PER_THREAD; q++) {
-    long long j = start + q;
-    if (j >= g.hi) break;
-    if (le[q]) {
-      keysB[g.lo + rank] = keysA[j];
-      valsB[g.lo + rank] = valsA[j];
-      if (g.r > 0 && j > g.j0) arot[g.lo + rank - (g.j0 - g.lo)] = (j - g.lo) - rank;
-      rank++;
-    } else {
-      long long gt_before = (j - g.lo) - rank;
-      s0[j] = gt_before + (rank - (g.j0 - g.lo));
+    const long long j = base + q * 64 + lane;
+    if (j < g.hi) {
+      const long long rank = run + __popcll(bal[q] & below);
+      if ((bal[q] >> lane) & 1ull) {
+        keysB[g.lo + rank] = kv[q];
+        valsB[g.lo + rank] = valsA[j];
+        if (g.r > 0 && j > g.j0) arot[g.lo + rank - jlead] = (int)((j - g.lo) - rank);
+      } else {
+        const long long gt_before = (j - g.lo) - rank;
+        s0[j] = (int)(gt_before + (rank - jlead));
+      }
+    }
+    run += __popcll(bal[q]);
+  }
+}
+
+// Chase with at most `cap` dependent steps per element.  A thread advances
+// its eight elements' chains in lock-step, so their loads are in flight
+// together.  Elements that finish are placed and their s0 set to S0_DONE;
+// the others keep their progress in s0, and the block flags its segment
+// for pointer doubling (one write per block, not per element).
+static constexpr int S0_DONE = 0x7fffffff;
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_tile_chase(const K* keysA, const int* valsA, K* keysB,
+                                                    int* valsB, const long long* tile_off,
+                                                    int nseg, const SegInfo* info,
+                                                    const int* arot, int* s0, int* seg_need,
+                                                    int* need_any, unsigned long long* rmax,
+                                                    int cap) {
+  const long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) return;
+  const int sg = find_seg(tile_off, nseg, t);
+  const SegInfo g = info[sg];
+  const K pivot = keysA[g.hi];
+  const long long start = g.lo + (t - g.first_tile) * TILE;
+  const int r = (int)g.r;
+  const int* A = arot + g.lo;
+  K kv[PER_THREAD];
+  int s[PER_THREAD];
+  bool act[PER_THREAD];
+  for (int q = 0; q < PER_THREAD; q++) {
+    const long long j = start + (long long)q * 256 + threadIdx.x;
+    act[q] = false;
+    s[q] = 0;
+    if (j < g.hi) {
+      kv[q] = keysA[j];
+      act[q] = !(kv[q] <= pivot);
+      if (act[q]) s[q] = s0[j];
+    }
+  }
+  for (int step = 0; step < cap; step++) {
+    bool any = false;
+    for (int q = 0; q < PER_THREAD; q++) {
+      const bool run = act[q] && s[q] < r;
+      const int a = A[run ? s[q] : 0];  // r >= 1 whenever a ">" element exists
+      s[q] += run ? max(a, 1) : 0;       // every entry is >= 1
+      any |= run;
+    }
+    if (!any) break;
+  }
+  // steps still needed by an unfinished chain: every later step advances
+  // by at least A(s), since A never decreases along the stream
+  unsigned long long rem = 0;
+  for (int q = 0; q < PER_THREAD; q++) {
+    if (!act[q]) continue;
+    const long long j = start + (long long)q * 256 + threadIdx.x;
+    if (s[q] < r) {
+      s0[j] = s[q];
+      const int a = max(A[s[q]], 1);
+      rem = max(rem, (unsigned long long)((r - s[q] + a - 1) / a));
+      continue;
+    }
+    s0[j] = S0_DONE;
+    const long long dst = g.lo + g.c + 1 + (s[q] - r);
+    keysB[dst] = kv[q];
+    valsB[dst] = valsA[j];
+  }
+  __shared__ unsigned long long sh_rem[4];
+  for (int o = 32; o > 0; o >>= 1) rem = max(rem, (unsigned long long)__shfl_xor(rem, o, 64));
+  if ((threadIdx.x & 63) == 0) sh_rem[threadIdx.x >> 6] = rem;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    rem = max(max(sh_rem[0], sh_rem[1]), max(sh_rem[2], sh_rem[3]));
+    if (rem > 0) {
+      seg_need[sg] = 1;
+      if (!__atomic_load_n(need_any, __ATOMIC_RELAXED)) atomicExch(need_any, 1);
+      if (__atomic_load_n(rmax, __ATOMIC_RELAXED) < rem) atomicMax(rmax, rem);
     }
   }
 }
 
-// Chase with at most `cap` dependent steps.  Elements that finish are
-// placed and their s0 set to LLONG_MAX; the others keep their progress in
-// s0 and flag their segment for pointer doubling (an extreme pivot makes
-// the queue short and its elements rotate many times).
-static constexpr int CHASE_CAP = 48;
-
-template <typename K>
-__global__ void k_tile_chase(const K* keysA, const int* valsA, K* keysB, int* valsB,
-                             const long long* tile_off, int nseg, const SegInfo* info,
-                             const long long* arot, long long* s0, int* seg_need,
-                             int* need_any) {
-  long long t = blockIdx.x;
-  int sg = find_seg(tile_off, nseg, t);
-  SegInfo g = info[sg];
-  K pivot = keysA[g.hi];
-  long long start = g.lo + (t - g.first_tile) * TILE;
-  for (int q = 0; q < PER_THREAD; q++) {
-    long long j = start + (long long)q * 256 + threadIdx.x;
-    if (j >= g.hi) continue;
-    K kv = keysA[j];
-    if (kv <= pivot) continue;
-    long long s = s0[j];
-    int steps = 0;
-    while (s < g.r && steps < CHASE_CAP) {
-      s += max(arot[g.lo + s], 1LL);  // every entry is >= 1
-      steps++;
-    }
-    if (s < g.r) {
-      s0[j] = s;
-      seg_need[sg] = 1;
-      *need_any = 1;
-      continue;
-    }
-    s0[j] = LLONG_MAX;
-    long long dst = g.lo + g.c + 1 + (s - g.r);
-    keysB[dst] = kv;
-    valsB[dst] = valsA[j];
-  }
+// Chase cap: EXH_SORT_CHASE_CAP (default 16).
+static int chase_cap() {
+  static int v = [] {
+    const char* e = std::getenv("EXH_SORT_CHASE_CAP");
+    int c = e ? std::atoi(e) : 16;
+    return c < 1 ? 1 : (c > 4096 ? 4096 : c);
+  }();
+  return v;
 }
 
 // Pointer doubling on nxt(q) = q + A(q) over the stream indices of flagged
-// segments: after k rounds one entry covers 2^k chase steps.  Values are
-// stored relative to the segment start.
+// segments: after k rounds one entry covers at least 2^k chase steps.
+// Values are stored relative to the segment start.  Each entry is either a
+// stream index below r on the chain from q or the chain's first index >= r,
+// so updating the array in place is exact: a round that reads an entry
+// another block has already advanced only jumps further along the same
+// chain, never past its exit.  A round whose predecessor left no entry
+// below r (round_open[k-1] == 0) returns at once, so the host can launch a
+// fixed number of rounds without waiting for the device.
 __global__ void k_double_init(const long long* tile_off, int nseg, const SegInfo* info,
-                              const int* seg_need, const long long* arot, int* nxt) {
+                              const int* seg_need, const int* need_any, const int* arot,
+                              int* nxt) {
+  if (!*need_any) return;
   long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) return;
   int sg = find_seg(tile_off, nseg, t);
   if (!seg_need[sg]) return;
   SegInfo g = info[sg];
@@ -223,34 +326,51 @@ __global__ void k_double_init(const long long* tile_off, int nseg, const SegInfo
   bool last = t == tile_off[sg + 1] - 1;
   long long q1 = last ? g.r : min(q0 + TILE, g.r);
   for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
-    nxt[g.lo + q] = (int)(q + max(arot[g.lo + q], 1LL));  // >= r: left the stream prefix
+    nxt[g.lo + q] = (int)q + max(arot[g.lo + q], 1);  // >= r: left the stream prefix
   }
 }
 
 __global__ void k_double_round(const long long* tile_off, int nseg, const SegInfo* info,
-                               const int* seg_need, const int* src, int* dst, int* changed) {
+                               const int* seg_need, const int* need_any, int* nxt,
+                               int* round_open, int k) {
+  if (!*need_any || (k > 0 && !round_open[k - 1])) return;
   long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) return;
   int sg = find_seg(tile_off, nseg, t);
   if (!seg_need[sg]) return;
   SegInfo g = info[sg];
   long long q0 = (t - g.first_tile) * TILE;
   bool last = t == tile_off[sg + 1] - 1;
   long long q1 = last ? g.r : min(q0 + TILE, g.r);
-  bool open = false;
-  for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
-    int v = src[g.lo + q];
-    int w = v < g.r ? src[g.lo + v] : v;
-    dst[g.lo + q] = w;
-    open |= w < g.r;
+  // all loads of a thread's entries are issued before its stores
+  int* X = nxt + g.lo;
+  const int r = (int)g.r;
+  int v[PER_THREAD];
+  for (int e = 0; e < PER_THREAD; e++) {
+    const long long q = q0 + threadIdx.x + (long long)e * 256;
+    v[e] = q < q1 ? X[q] : r;
   }
-  if (open) *changed = 1;
+  int w[PER_THREAD];
+  for (int e = 0; e < PER_THREAD; e++) w[e] = v[e] < r ? X[v[e]] : v[e];
+  bool open = false;
+  for (int e = 0; e < PER_THREAD; e++) {
+    const long long q = q0 + threadIdx.x + (long long)e * 256;
+    if (v[e] < r) {
+      X[q] = w[e];
+      open |= w[e] < r;
+    }
+  }
+  if (__syncthreads_or(open) && threadIdx.x == 0) round_open[k] = 1;
 }
 
 template <typename K>
 __global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* valsB,
                               const long long* tile_off, int nseg, const SegInfo* info,
-                              const int* seg_need, const int* nxt, const long long* s0) {
+                              const int* seg_need, const int* need_any, const int* nxt,
+                              const int* s0, int* err) {
+  if (!*need_any) return;
   long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) return;
   int sg = find_seg(tile_off, nseg, t);
   if (!seg_need[sg]) return;
   SegInfo g = info[sg];
@@ -263,6 +383,10 @@ __global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* v
     long long s = s0[j];
     if (s >= g.r) continue;  // placed by the capped chase
     for (int guard = 0; s < g.r && guard < 64; guard++) s = nxt[g.lo + s];
+    if (s < g.r) {  // doubling rounds too few: report, write nothing
+      *err = 1;
+      continue;
+    }
     long long dst = g.lo + g.c + 1 + (s - g.r);
     keysB[dst] = keysA[j];
     valsB[dst] = valsA[j];
@@ -273,6 +397,7 @@ template <typename K>
 __global__ void k_tile_copyback(K* keysA, int* valsA, const K* keysB, const int* valsB,
                                 const long long* tile_off, int nseg, const SegInfo* info) {
   long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) return;
   int sg = find_seg(tile_off, nseg, t);
   SegInfo g = info[sg];
   long long start = g.lo + (t - g.first_tile) * TILE;
@@ -353,111 +478,138 @@ __device__ inline void lane_quicksort(K* sk, int* sv, int lo, int hi) {
   }
 }
 
-// One wavefront per segment of 2..SMALL elements, resident in LDS.
-// Sub-segments longer than TINY are partitioned by the whole wave with the
-// closed-form Lomuto rule (see the file comment); shorter ones are queued
-// and then sorted one per lane by lane_quicksort.
+// One workgroup of SMALL_WAVES wavefronts per segment of 2..SMALL
+// elements, resident in LDS.  Sub-segments longer than TINY are partitioned
+// by the whole workgroup with the closed-form Lomuto rule (see the file
+// comment): each wave takes a contiguous chunk, ranks come from ballots and
+// the chunk offsets from LDS.  Shorter sub-segments are queued and then
+// sorted one per lane by lane_quicksort.  Several waves share one segment's
+// LDS so that the LDS per CU, not per wave, bounds the occupancy.
+static constexpr int SMALL_WAVES = 4;
+
 template <typename K, int SMALL>
-__global__ __launch_bounds__(64) void k_small_sort(K* keys, int* vals, const long long* seg_lo,
-                                                   const long long* seg_hi) {
+__global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(K* keys, int* vals,
+                                                                 const long long* seg_lo,
+                                                                 const long long* seg_hi) {
+  static_assert(SMALL <= 32767, "LDS indices are 16-bit");
+  constexpr int NT = 64 * SMALL_WAVES;
   __shared__ K sk[SMALL], tk[SMALL];
-  __shared__ int sv[SMALL], tv[SMALL], arot[SMALL], s0[SMALL];
-  __shared__ int st_lo[64], st_hi[64];
+  __shared__ int sv[SMALL], tv[SMALL];
+  __shared__ short arot[SMALL];
   __shared__ short tl_lo[SMALL / 2 + 1], tl_hi[SMALL / 2 + 1];
-  __shared__ int ntiny;
-  const int lane = threadIdx.x;
+  __shared__ short st_lo[64], st_hi[64];
+  __shared__ int wc[SMALL_WAVES], wj[SMALL_WAVES];
+  __shared__ int ntiny, top_sh;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const unsigned long long below = lanemask_lt();
   const long long base = seg_lo[blockIdx.x];
   const int n = (int)(seg_hi[blockIdx.x] - base + 1);
-  for (int i = lane; i < n; i += 64) {
+  for (int i = tid; i < n; i += NT) {
     sk[i] = keys[base + i];
     sv[i] = vals[base + i];
   }
-  int top = 0;
-  if (lane == 0) {
+  if (tid == 0) {
     ntiny = 0;
+    top_sh = 0;
     if (n > TINY) {
       st_lo[0] = 0;
-      st_hi[0] = n - 1;
+      st_hi[0] = (short)(n - 1);
+      top_sh = 1;
     } else {
       tl_lo[0] = 0;
       tl_hi[0] = (short)(n - 1);
       ntiny = 1;
     }
   }
-  top = n > TINY ? 1 : 0;
   __syncthreads();
-  while (top > 0) {
-    top--;
-    const int lo = st_lo[top], hi = st_hi[top];
-    __syncthreads();
+  for (;;) {
+    const int top = top_sh;
+    if (top == 0) break;
+    const int lo = st_lo[top - 1], hi = st_hi[top - 1];
     const K pivot = sk[hi];
-    // pass 1: count "<=" and find the first ">"
-    int c = 0, j0 = hi;
-    for (int b = lo; b < hi; b += 64) {
-      int j = b + lane;
-      bool le = (j < hi) && (sk[j] <= pivot);
-      bool gt = (j < hi) && !le;
-      unsigned long long mle = __ballot(le), mgt = __ballot(gt);
-      if (j0 == hi && mgt) j0 = b + __ffsll((long long)mgt) - 1;
-      c += __popcll(mle);
+    const int L = hi - lo;
+    const int Q = ((L + SMALL_WAVES - 1) / SMALL_WAVES + 63) & ~63;
+    const int cb = lo + w * Q, ce = min(cb + Q, hi);
+    // pass 1: per-chunk count of "<=" and first ">"
+    int cnt = 0, j0w = hi;
+    for (int b = cb; b < ce; b += 64) {
+      const int j = b + lane;
+      const bool in = j < ce;
+      const bool le = in && sk[j] <= pivot;
+      const unsigned long long mle = __ballot(le), mgt = __ballot(in && !le);
+      if (j0w == hi && mgt) j0w = b + __ffsll((long long)mgt) - 1;
+      cnt += __popcll(mle);
     }
-    const int m = (hi - lo) - c;
+    if (lane == 0) {
+      wc[w] = cnt;
+      wj[w] = j0w;
+    }
+    __syncthreads();
+    int c = 0, j0 = hi, wbase = 0;
+    for (int v = 0; v < SMALL_WAVES; v++) {
+      if (v < w) wbase += wc[v];
+      c += wc[v];
+      j0 = min(j0, wj[v]);
+    }
+    const int m = L - c;
     const int r = m > 0 ? (c - (j0 - lo)) + 1 : 0;
-    // pass 2: place "<=" elements, record rotations and queue entries
-    int run = 0;
-    for (int b = lo; b < hi; b += 64) {
-      int j = b + lane;
-      bool le = (j < hi) && (sk[j] <= pivot);
-      unsigned long long mle = __ballot(le);
-      int rank = run + __popcll(mle & lanemask_lt());
-      if (j < hi) {
-        if (le) {
-          tk[lo + rank] = sk[j];
-          tv[lo + rank] = sv[j];
-          if (r > 0 && j > j0) arot[rank - (j0 - lo)] = (j - lo) - rank;
-        } else {
-          s0[j] = ((j - lo) - rank) + (rank - (j0 - lo));
-        }
+    const int jlead = j0 - lo;
+    // pass 2: place "<=" elements, record the rotation table
+    int run = wbase;
+    for (int b = cb; b < ce; b += 64) {
+      const int j = b + lane;
+      const bool le = j < ce && sk[j] <= pivot;
+      const unsigned long long mle = __ballot(le);
+      if (le) {
+        const int rank = run + __popcll(mle & below);
+        tk[lo + rank] = sk[j];
+        tv[lo + rank] = sv[j];
+        if (r > 0 && j > j0) arot[rank - jlead] = (short)((j - lo) - rank);
       }
       run += __popcll(mle);
     }
-    if (lane == 0) {
+    if (tid == 0) {
       tk[lo + c] = sk[hi];
       tv[lo + c] = sv[hi];
-      if (m > 0) arot[r - 1] = m;
+      if (m > 0) arot[r - 1] = (short)m;
     }
     __syncthreads();
-    // pass 3: chase the queue
-    for (int b = lo; b < hi; b += 64) {
-      int j = b + lane;
-      if (j < hi && !(sk[j] <= pivot)) {
-        int s = s0[j];
-        for (int guard = 0; s < r && guard <= r; guard++) s += max(arot[s], 1);
+    // pass 3: chase the queue from each ">" element's first stream index
+    run = wbase;
+    for (int b = cb; b < ce; b += 64) {
+      const int j = b + lane;
+      const bool in = j < ce;
+      const bool le = in && sk[j] <= pivot;
+      const unsigned long long mle = __ballot(le);
+      if (in && !le) {
+        const int rank = run + __popcll(mle & below);
+        int s = ((j - lo) - rank) + (rank - jlead);
+        for (int guard = 0; s < r && guard <= r; guard++) s += max((int)arot[s], 1);
         tk[lo + c + 1 + (s - r)] = sk[j];
         tv[lo + c + 1 + (s - r)] = sv[j];
       }
+      run += __popcll(mle);
     }
     __syncthreads();
-    for (int i = lo + lane; i <= hi; i += 64) {
+    for (int i = lo + tid; i <= hi; i += NT) {
       sk[i] = tk[i];
       sv[i] = tv[i];
     }
     // children: long ones on the stack (larger first, so the smaller is
     // partitioned next), short ones to the per-lane list
-    int al = lo, ah = lo + c - 1, bl = lo + c + 1, bh = hi;
-    if (ah - al > bh - bl) {
-      int x = al, y = ah;
-      al = bl;
-      ah = bh;
-      bl = x;
-      bh = y;
-    }
-    int pushed = 0;
-    if (lane == 0) {
-      int tt = top;
+    if (tid == 0) {
+      int al = lo, ah = lo + c - 1, bl = lo + c + 1, bh = hi;
+      if (ah - al > bh - bl) {
+        int x = al, y = ah;
+        al = bl;
+        ah = bh;
+        bl = x;
+        bh = y;
+      }
+      int tt = top - 1;
       if (bh - bl + 1 > TINY) {
-        st_lo[tt] = bl;
-        st_hi[tt] = bh;
+        st_lo[tt] = (short)bl;
+        st_hi[tt] = (short)bh;
         tt++;
       } else if (bh > bl) {
         tl_lo[ntiny] = (short)bl;
@@ -465,23 +617,21 @@ __global__ __launch_bounds__(64) void k_small_sort(K* keys, int* vals, const lon
         ntiny++;
       }
       if (ah - al + 1 > TINY) {
-        st_lo[tt] = al;
-        st_hi[tt] = ah;
+        st_lo[tt] = (short)al;
+        st_hi[tt] = (short)ah;
         tt++;
       } else if (ah > al) {
         tl_lo[ntiny] = (short)al;
         tl_hi[ntiny] = (short)ah;
         ntiny++;
       }
+      top_sh = tt;
     }
-    pushed = (bh - bl + 1 > TINY) + (ah - al + 1 > TINY);
-    top += pushed;
     __syncthreads();
   }
+  for (int q = tid; q < ntiny; q += NT) lane_quicksort<K>(sk, sv, tl_lo[q], tl_hi[q]);
   __syncthreads();
-  for (int q = lane; q < ntiny; q += 64) lane_quicksort<K>(sk, sv, tl_lo[q], tl_hi[q]);
-  __syncthreads();
-  for (int i = lane; i < n; i += 64) {
+  for (int i = tid; i < n; i += NT) {
     keys[base + i] = sk[i];
     vals[base + i] = sv[i];
   }
@@ -520,9 +670,8 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
          o_toff = take((maxseg + 2) * 8), o_j0 = take(maxseg * 8),
          o_tle = take((maxtiles + 1) * 8), o_tpre = take((maxtiles + 2) * 8),
          o_info = take(maxseg * sizeof(SegInfo)), o_slo = take(maxsmall * 8),
-         o_shi = take(maxsmall * 8), o_cnt = take(64), o_arot = take((size_t)n * 8),
-         o_s0 = take((size_t)n * 8), o_need = take(maxseg * 4 + 64), o_nxa = take((size_t)n * 4 + 64),
-         o_nxb = take((size_t)n * 4 + 64);
+         o_shi = take(maxsmall * 8), o_cnt = take(64 * 4), o_arot = take((size_t)n * 4),
+         o_s0 = take((size_t)n * 4), o_need = take(maxseg * 4 + 64), o_nxa = take((size_t)n * 4 + 64);
   char* aux = (char*)ctx_buf(c, BUF_SORT_AUX, bytes);
   K* kB = (K*)ctx_buf(c, BUF_SORT_K, (size_t)n * sizeof(K));
   int* vB = (int*)ctx_buf(c, BUF_SORT_V, (size_t)n * sizeof(int));
@@ -539,90 +688,110 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
   long long* slo = (long long*)(aux + o_slo);
   long long* shi = (long long*)(aux + o_shi);
   int* cnt = (int*)(aux + o_cnt);  // [0] next count, [1] small count
-  long long* arot = (long long*)(aux + o_arot);
-  long long* s0 = (long long*)(aux + o_s0);
+  int* arot = (int*)(aux + o_arot);
+  int* s0 = (int*)(aux + o_s0);
+  // level state read back once per level: [0] large children, [1] small
+  // segments so far, [2] some segment needs doubling, [4..5] most steps an
+  // unfinished chain still needs, [6..7] elements in large children; [8] doubling error (kept
+  // over all levels); [16..63] per-round flags of the doubling
   int* hc = (int*)ctx_pinned(c, 64);
+  int* seg_need = (int*)(aux + o_need);
+  int* nxt = (int*)(aux + o_nxa);
+  int* round_open = cnt + 16;
+  unsigned long long* rmax = (unsigned long long*)(cnt + 4);
+  unsigned long long* next_elems = (unsigned long long*)(cnt + 6);
 
   int nsmall_host = 0;
   int nseg = 0;
-  long long init_lo = 0, init_hi = n - 1;
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 16, 0);
-  if (n > SMALL) {
-    EXH_CHECK_HIP(hipMemcpyAsync(cur_lo, &init_lo, 8, hipMemcpyHostToDevice, c->stream));
-    EXH_CHECK_HIP(hipMemcpyAsync(cur_hi, &init_hi, 8, hipMemcpyHostToDevice, c->stream));
-    nseg = 1;
-  } else {
-    EXH_CHECK_HIP(hipMemcpyAsync(slo, &init_lo, 8, hipMemcpyHostToDevice, c->stream));
-    EXH_CHECK_HIP(hipMemcpyAsync(shi, &init_hi, 8, hipMemcpyHostToDevice, c->stream));
-    nsmall_host = 1;
+  long long level_elems = 0;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 64, 0);
+  {
+    long long init[2] = {0, n - 1};
+    long long* dlo = n > SMALL ? cur_lo : slo;
+    long long* dhi = n > SMALL ? cur_hi : shi;
+    EXH_CHECK_HIP(hipMemcpyAsync(dlo, &init[0], 8, hipMemcpyHostToDevice, c->stream));
+    EXH_CHECK_HIP(hipMemcpyAsync(dhi, &init[1], 8, hipMemcpyHostToDevice, c->stream));
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (n > SMALL) {
+      nseg = 1;
+      level_elems = n;
+    } else {
+      nsmall_host = 1;
+    }
   }
-  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-  int* seg_need = (int*)(aux + o_need);
-  int* nxa = (int*)(aux + o_nxa);
-  int* nxb = (int*)(aux + o_nxb);
-  int* flags = cnt + 2;  // [2] any segment needs doubling, [3] doubling not converged
+  bool pending_err = false;
   while (nseg > 0) {
     c->stat_sort_levels++;
+    // tiles of this level: at most ceil(elements / TILE) + segments; the
+    // tile kernels' blocks beyond the exact count return at once
+    const long long ntile_ub = (level_elems + TILE - 1) / TILE + nseg;
+    EXH_REQUIRE(ntile_ub <= maxtiles, EXH_ERR_INTERNAL, "peak sort: tile bound exceeded");
     hipLaunchKernelGGL(k_seg_ntiles, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, cur_lo,
                        cur_hi, nseg, ntiles, (long long*)j0);
     EXH_LAUNCH_CHECK();
     exclusive_scan<long long>(c, ntiles, toff, nseg);
-    long long ntile_host = 0;
-    EXH_CHECK_HIP(hipMemcpyAsync(&ntile_host, toff + nseg, 8, hipMemcpyDeviceToHost, c->stream));
-    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    hipLaunchKernelGGL(k_tile_count<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
-                       keys, cur_lo, cur_hi, toff, nseg, tle, j0);
+    hipLaunchKernelGGL(k_tile_count<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, keys,
+                       cur_lo, cur_hi, toff, nseg, tle, j0);
     EXH_LAUNCH_CHECK();
-    exclusive_scan<long long>(c, tle, tpre, ntile_host);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 1, 0);  // next count
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, flags, 2, 0);
+    exclusive_scan<long long>(c, tle, tpre, ntile_ub);
+    // reset the per-level counters (not cnt[1]: the small segments collect
+    // over all levels) and the round flags
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 1, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt + 2, 6, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, round_open, 48, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, seg_need,
                        nseg, 0);
     hipLaunchKernelGGL(k_seg_final<K>, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, keys,
                        vals, kB, vB, cur_lo, cur_hi, toff, tpre, j0, nseg, info, arot,
-                       (long long)SMALL, nxt_lo, nxt_hi, cnt, slo, shi, cnt + 1);
+                       (long long)SMALL, nxt_lo, nxt_hi, cnt, slo, shi, cnt + 1, next_elems);
     EXH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tile_scatter<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_tile_scatter<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
                        keys, vals, kB, vB, toff, tpre, nseg, info, arot, s0);
     EXH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tile_chase<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
-                       keys, vals, kB, vB, toff, nseg, info, arot, s0, seg_need, flags);
+    hipLaunchKernelGGL(k_tile_chase<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, keys,
+                       vals, kB, vB, toff, nseg, info, arot, s0, seg_need, cnt + 2, rmax, chase_cap());
     EXH_LAUNCH_CHECK();
-    EXH_CHECK_HIP(hipMemcpyAsync(hc, cnt, 12, hipMemcpyDeviceToHost, c->stream));
+    EXH_CHECK_HIP(hipMemcpyAsync(hc, cnt, 32, hipMemcpyDeviceToHost, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
     if (hc[2]) {
-      // extreme pivots: finish their chases by pointer doubling
+      // long chains: finish them by pointer doubling; after k rounds an
+      // entry covers 2^k steps, and no chain needs more than rm more steps
       c->stat_sort_doubling++;
-      hipLaunchKernelGGL(k_double_init, dim3((unsigned)ntile_host), dim3(256), 0, c->stream, toff,
-                         nseg, info, seg_need, arot, nxa);
+      const unsigned long long rm = *(unsigned long long*)(hc + 4);
+      int rounds = 1;
+      while ((1ull << (rounds - 1)) < rm && rounds < 62) rounds++;
+      EXH_REQUIRE(rounds <= 40, EXH_ERR_INTERNAL, "peak sort: segment too long for doubling");
+      hipLaunchKernelGGL(k_double_init, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, toff,
+                         nseg, info, seg_need, cnt + 2, arot, nxt);
       EXH_LAUNCH_CHECK();
-      for (int round = 0; round < 40; round++) {
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, flags + 1, 1, 0);
-        hipLaunchKernelGGL(k_double_round, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
-                           toff, nseg, info, seg_need, nxa, nxb, flags + 1);
+      for (int k = 0; k < rounds; k++) {
+        hipLaunchKernelGGL(k_double_round, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
+                           toff, nseg, info, seg_need, cnt + 2, nxt, round_open, k);
         EXH_LAUNCH_CHECK();
-        std::swap(nxa, nxb);
-        EXH_CHECK_HIP(hipMemcpyAsync(hc + 3, flags + 1, 4, hipMemcpyDeviceToHost, c->stream));
-        EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-        if (!hc[3]) break;
-        EXH_REQUIRE(round < 39, EXH_ERR_INTERNAL, "peak sort: pointer doubling did not converge");
       }
-      hipLaunchKernelGGL(k_tile_chase2<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
-                         keys, vals, kB, vB, toff, nseg, info, seg_need, nxa, s0);
+      hipLaunchKernelGGL(k_tile_chase2<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
+                         keys, vals, kB, vB, toff, nseg, info, seg_need, cnt + 2, nxt, s0, cnt + 8);
       EXH_LAUNCH_CHECK();
+      pending_err = true;
     }
-    hipLaunchKernelGGL(k_tile_copyback<K>, dim3((unsigned)ntile_host), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_tile_copyback<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
                        keys, vals, kB, vB, toff, nseg, info);
     EXH_LAUNCH_CHECK();
     nseg = hc[0];
     nsmall_host = hc[1];
+    level_elems = *(long long*)(hc + 6);
     EXH_REQUIRE(nseg <= maxseg && nsmall_host <= maxsmall, EXH_ERR_INTERNAL,
                 "peak sort: segment lists overflow (%d, %d)", nseg, nsmall_host);
     std::swap(cur_lo, nxt_lo);
     std::swap(cur_hi, nxt_hi);
   }
+  if (pending_err) {
+    EXH_CHECK_HIP(hipMemcpyAsync(hc + 8, cnt + 8, 4, hipMemcpyDeviceToHost, c->stream));
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    EXH_REQUIRE(!hc[8], EXH_ERR_INTERNAL, "peak sort: pointer doubling did not converge");
+  }
   if (nsmall_host > 0) {
-    hipLaunchKernelGGL((k_small_sort<K, SMALL>), dim3((unsigned)nsmall_host), dim3(64), 0,
+    hipLaunchKernelGGL((k_small_sort<K, SMALL>), dim3((unsigned)nsmall_host), dim3(64 * SMALL_WAVES), 0,
                        c->stream, keys, vals, slo, shi);
     EXH_LAUNCH_CHECK();
   }

@@ -4,6 +4,9 @@
 //                shifted by -Lb/2 for interlacing) with device-scope atomics
 //                (gridmodule.c:9-53, 146-270); float atomics in FP32, double
 //                atomics in FP64
+//  k_deposit_tiles  the same weights for particles in Lagrangian lattice
+//                order, accumulated per lattice tile in LDS (double) and
+//                flushed with one coalesced atomic per touched cell
 //  k_normalise   rho / rho_bar - 1 for unweighted grids (gridmodule.c:197-202)
 //  k_power_bins  one pass over the half-complex modes of all tracer grids:
 //                interlacing combine, window deconvolution and Normk
@@ -178,8 +181,11 @@ __global__ void k_deposit(const T* pos, const T* vel, const T* mass, const int* 
 // The block finds that box (cells relative to the tile centre, unwrapped
 // across the periodic boundary), accumulates the window weights there with
 // LDS atomics, and adds each touched cell to the grid with one coalesced
-// global atomic; per particle that is ~2 global atomics instead of 8
-// (CIC).  Tiles whose box exceeds the LDS capacity deposit directly.
+// global atomic (about 3 per particle for CIC at nd = nL instead of 8
+// conflicting ones).  Tiles whose box exceeds the LDS deposit directly.
+// The boxes accumulate in double for either grid precision: LDS float
+// atomic adds (ds_add_f32) run at a small fraction of the rate of
+// ds_add_f64 on gfx950, and the double sums round once when flushed.
 // Weights are computed exactly as in dep_ngp/dep_cic/dep_tsc.
 // ---------------------------------------------------------------------------
 __device__ inline int rel_cell(int c, int qc, int nd) {
@@ -269,10 +275,10 @@ static constexpr int TILE_PPT = TILE_L * TILE_L * TILE_L / TILE_THREADS;  // par
 
 template <typename T, int WIN>
 __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
-    const T* pos, const T* vel, const T* mass, int nL, int nd, T L, T Lb, int direction, T rsd,
-    int nsub, int cap, T* grid, unsigned long long* counts) {
+    const T* pos, const T* vel, const T* mass, int nL, int nd, int W, T L, T Lb, int direction,
+    T rsd, int nsub, int cap, T* grid, unsigned long long* counts) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  T* box = (T*)lds_raw;
+  double* box = (double*)lds_raw;
   __shared__ int red[2 * 3 * (TILE_THREADS / 64)];
   __shared__ unsigned long long nvalid;
   constexpr int NW = WIN == 1 ? 1 : (WIN == 2 ? 2 : 3);
@@ -280,11 +286,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
   const T Ld = L / (T)nd;
   const int nt = nL / TILE_L;
   const int tz = blockIdx.x % nt, ty = (blockIdx.x / nt) % nt, tx = blockIdx.x / (nt * nt);
-  // tile centre in grid cells (Lagrangian lattice mapped onto the grid)
-  const double scale = (double)nd / (double)nL;
-  const int qc[3] = {(int)((tx * TILE_L + TILE_L / 2) * scale) % nd,
-                     (int)((ty * TILE_L + TILE_L / 2) * scale) % nd,
-                     (int)((tz * TILE_L + TILE_L / 2) * scale) % nd};
+  // tile centre in grid cells: the middle of the grid tile of the same index
+  const int qc[3] = {tx * W + W / 2, ty * W + W / 2, tz * W + W / 2};
   if (threadIdx.x == 0) nvalid = 0;
   // particles of this thread: lanes along k (coalesced rows)
   T xs[TILE_PPT][3];
@@ -292,13 +295,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
   bool ok[TILE_PPT];
   unsigned long long myvalid = 0;
   for (int p = 0; p < TILE_PPT; p++) {
-    int lin = threadIdx.x + p * TILE_THREADS;  // 0..4095
+    int lin = threadIdx.x + p * TILE_THREADS;
     int c = lin % TILE_L, b = (lin / TILE_L) % TILE_L, a = lin / (TILE_L * TILE_L);
     size_t j = ((size_t)(tx * TILE_L + a) * nL + (ty * TILE_L + b)) * nL + (tz * TILE_L + c);
     T x[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
     if (direction >= 0) x[direction] = wrap_while<T>(x[direction] + vel[3 * j + direction] * rsd, L);
     ok[p] = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
-    for (int a2 = 0; a2 < 3; a2++) xs[p][a2] = x[a2];
+    for (int q = 0; q < 3; q++) xs[p][q] = x[q];
     ms[p] = mass ? mass[j] : (T)1;
     myvalid += ok[p];
   }
@@ -309,49 +312,50 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
     int vmin[3] = {1 << 30, 1 << 30, 1 << 30}, vmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
     for (int p = 0; p < TILE_PPT; p++) {
       if (!ok[p]) continue;
-      for (int a = 0; a < 3; a++) {
-        T pa = wrap_while<T>(xs[p][a] + sh, L);
+      for (int q = 0; q < 3; q++) {
+        T pa = wrap_while<T>(xs[p][q] + sh, L);
         int cell[3];
         T w[3];
         int n = axis_stencil<T, WIN>(pa, Ld, nd, cell, w);
-        for (int q = 0; q < n; q++) {
-          int r = rel_cell(cell[q], qc[a], nd);
-          vmin[a] = min(vmin[a], r);
-          vmax[a] = max(vmax[a], r);
+        for (int u = 0; u < n; u++) {
+          int r = rel_cell(cell[u], qc[q], nd);
+          vmin[q] = min(vmin[q], r);
+          vmax[q] = max(vmax[q], r);
         }
       }
     }
     block_minmax<3>(vmin, vmax, red);
+    if (vmax[0] < vmin[0]) continue;  // no valid particle
     const int e0 = vmax[0] - vmin[0] + 1, e1 = vmax[1] - vmin[1] + 1, e2 = vmax[2] - vmin[2] + 1;
-    const bool fits = vmax[0] >= vmin[0] && (long long)e0 * e1 * e2 <= cap;
+    const bool fits = (long long)e0 * e1 * e2 <= cap;
     if (fits) {
       const int vol = e0 * e1 * e2;
-      for (int v = threadIdx.x; v < vol; v += TILE_THREADS) box[v] = (T)0;
+      for (int v = threadIdx.x; v < vol; v += TILE_THREADS) box[v] = 0.0;
       __syncthreads();
     }
     for (int p = 0; p < TILE_PPT; p++) {
       if (!ok[p]) continue;
       int cell[3][3];
       T w[3][3];
-      for (int a = 0; a < 3; a++) {
-        T pa = wrap_while<T>(xs[p][a] + sh, L);
-        axis_stencil<T, WIN>(pa, Ld, nd, cell[a], w[a]);
+      for (int q = 0; q < 3; q++) {
+        T pa = wrap_while<T>(xs[p][q] + sh, L);
+        axis_stencil<T, WIN>(pa, Ld, nd, cell[q], w[q]);
       }
-      for (int a = 0; a < NW; a++) {
+      for (int u = 0; u < NW; u++) {
         for (int b = 0; b < NW; b++) {
-          T wab = ms[p] * w[0][a] * w[1][b];
+          T wab = ms[p] * w[0][u] * w[1][b];
           for (int c = 0; c < NW; c++) {
             T val = wab * w[2][c];
-            const int lx = rel_cell(cell[0][a], qc[0], nd) - vmin[0];
+            const int lx = rel_cell(cell[0][u], qc[0], nd) - vmin[0];
             const int ly = rel_cell(cell[1][b], qc[1], nd) - vmin[1];
             const int lz = rel_cell(cell[2][c], qc[2], nd) - vmin[2];
             // the box bounds were taken over these same stencils; the
             // range test only guards the LDS index
             if (fits && (unsigned)lx < (unsigned)e0 && (unsigned)ly < (unsigned)e1 &&
                 (unsigned)lz < (unsigned)e2) {
-              atomicAdd(&box[(lx * e1 + ly) * e2 + lz], val);
+              atomicAdd(&box[(lx * e1 + ly) * e2 + lz], (double)val);
             } else {
-              atomic_add_t(g + ((size_t)cell[0][a] * nd + cell[1][b]) * nd + cell[2][c], val);
+              atomic_add_t(g + ((size_t)cell[0][u] * nd + cell[1][b]) * nd + cell[2][c], val);
             }
           }
         }
@@ -361,12 +365,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
       __syncthreads();
       const int vol = e0 * e1 * e2;
       for (int v = threadIdx.x; v < vol; v += TILE_THREADS) {
-        T val = box[v];
-        if (val == (T)0) continue;
-        int lz = v % e2, ly = (v / e2) % e1, lx = v / (e1 * e2);
-        int cx = wrap_cell(qc[0] + vmin[0] + lx, nd), cy = wrap_cell(qc[1] + vmin[1] + ly, nd),
-            cz = wrap_cell(qc[2] + vmin[2] + lz, nd);
-        atomic_add_t(g + ((size_t)cx * nd + cy) * nd + cz, val);
+        const double val = box[v];
+        if (val == 0.0) continue;
+        const int lz = v % e2, ly = (v / e2) % e1, lx = v / (e1 * e2);
+        const int cx = wrap_cell(qc[0] + vmin[0] + lx, nd), cy = wrap_cell(qc[1] + vmin[1] + ly, nd),
+                  cz = wrap_cell(qc[2] + vmin[2] + lz, nd);
+        atomic_add_t(g + ((size_t)cx * nd + cy) * nd + cz, (T)val);
       }
       __syncthreads();
     }
@@ -447,23 +451,28 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   }
   unsigned gsz = grid_for(np, 256);
   if (gsz > 65536 * 4) gsz = 65536 * 4;
-  // particles on a lattice whose side is a multiple of the tile side are
-  // deposited tile by tile through LDS (correct for any order; fast when the
-  // array is in lattice order, as LPT particles are)
+  // particles on a lattice whose side is a multiple of the tile side, with
+  // a grid side divisible by the number of tiles per axis, are deposited
+  // tile by tile through LDS (correct for any order; fast when the array is
+  // in lattice order, as LPT particles are)
   const int nL = lattice_side(np);
-  const bool tiles = type == nullptr && nL >= TILE_L && nL % TILE_L == 0 && deposit_lds_bytes() > 0;
+  const int nt = nL / TILE_L;
+  const bool tiles = type == nullptr && nL >= TILE_L && nL % TILE_L == 0 && nd % nt == 0 &&
+                     deposit_lds_bytes() > 0;
   {
     TimedScope ts(c, window == 2 ? "cic_deposit" : (window == 1 ? "ngp_deposit" : "tsc_deposit"));
     if (tiles) {
       const int lds = deposit_lds_bytes();
-      const int cap = lds / (int)sizeof(T);
-      const unsigned nblk = (unsigned)((nL / TILE_L) * (nL / TILE_L) * (nL / TILE_L));
-#define EXH_DEPT(W)                                                                            \
+      const int cap = lds / (int)sizeof(double);
+      const int W = nd / nt;
+      const unsigned nblk = (unsigned)nt * nt * nt;
+#define EXH_DEPT(WN)                                                                           \
   do {                                                                                         \
-    EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_deposit_tiles<T, W>,                      \
+    EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_deposit_tiles<T, WN>,                     \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds));       \
-    hipLaunchKernelGGL((k_deposit_tiles<T, W>), dim3(nblk), dim3(TILE_THREADS), lds, c->stream, \
-                       pos, vel, mass, nL, nd, Lf, Lb, direction, rsd, nsub, cap, grid, counts); \
+    hipLaunchKernelGGL((k_deposit_tiles<T, WN>), dim3(nblk), dim3(TILE_THREADS), lds,          \
+                       c->stream, pos, vel, mass, nL, nd, W, Lf, Lb, direction, rsd, nsub, cap, \
+                       grid, counts);                                                          \
   } while (0)
       if (window == 1) EXH_DEPT(1);
       if (window == 2) EXH_DEPT(2);

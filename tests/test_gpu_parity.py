@@ -124,19 +124,21 @@ def test_grid_compute(prec, window, il):
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("window", [1, 2, 5])
 @pytest.mark.parametrize("il", [0, 1])
-@pytest.mark.parametrize("nd", [32, 64])
-def test_grid_compute_lattice_tiles(prec, window, il, nd):
-    """Particles in lattice order (np = 32^3) take the LDS-tiled deposit.
-    nd = 32: tile boxes fit in LDS, including a tile whose box wraps the
-    whole periodic grid (an outlier at the antipode); nd = 64: the 16-particle
-    tiles span 32+ cells a side and take the direct path.  Weighted and
-    redshift-space variants run through the same tiles."""
+@pytest.mark.parametrize("nL,nd", [(32, 32), (32, 64), (64, 64), (64, 128)])
+def test_grid_compute_lattice_tiles(prec, window, il, nL, nd):
+    """Particles in lattice order (np = nL^3) take the tiled deposit: LDS
+    boxes per 16^3 lattice tile, gathered per grid tile.  (32, 32): boxes
+    fit, including one that wraps the whole periodic grid (an outlier at the
+    antipode); (32, 64): 16-particle tiles span 32+ cells and fall back to
+    atomics; (64, *): four tiles per axis, so the 3x3x3 cover test decides,
+    and the outliers' tiles fall back.  Weighted and redshift-space variants
+    run through the same tiles."""
     set_precision(prec)
     R = np.float32 if prec == 4 else np.float64
-    nL, L = 32, 128.0
-    rng = np.random.default_rng(13 + window + 2 * il + nd)
+    L = 128.0
+    rng = np.random.default_rng(13 + window + 2 * il + nd + nL)
     q = (np.indices((nL, nL, nL)).reshape(3, -1).T + 0.5) * (L / nL)
-    pos = q + rng.normal(0, 2 * L / nd, q.shape)
+    pos = q + rng.normal(0, 1.5 * L / nd, q.shape)
     pos[0] += L / 2  # antipodal outlier in the first tile
     pos[5 * nL * nL + 3 * nL + 7] -= (L / 3, 0.0, L / 4)
     pos = np.mod(pos, L).astype(R)
