@@ -105,6 +105,22 @@ def _rc(precision):
     return C.c_float if precision == 4 else C.c_double
 
 
+def spline_eval(x, y, xq):
+    """Natural cubic spline through (x, y) evaluated at xq (the oracle's
+    restatement of GSL's cspline, density_grid.c:62)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    xq = np.ascontiguousarray(xq, dtype=np.float64)
+    out = np.zeros(len(xq), dtype=np.float64)
+    rc = lib().ora_spline_eval_many_f64(_ptr(x), _ptr(y), C.c_int(len(x)), _ptr(xq),
+                                        _ptr(out), C.c_int(len(xq)))
+    if rc < 0:
+        raise RuntimeError("oracle spline failed")
+    if rc == 1:
+        raise ValueError("spline evaluated outside its knots")
+    return out
+
+
 # --------------------------------------------------------------------------
 # field
 # --------------------------------------------------------------------------

@@ -1,0 +1,199 @@
+"""Pins of the oracle's mass assignment (gridmodule.c:9-53, 146-270) and
+power spectrum (spectrum_h.c:30-167, powermodule.c:4-108) on the CPU.
+
+Mass assignment is compared with the textbook kernels evaluated at every
+cell centre (NGP: the containing cell; CIC: prod(1 - |d|); TSC: 3/4 - d^2
+or (3/2 - |d|)^2 / 2), with mass conservation checked by hypothesis.  The
+power spectrum is compared with a single Fourier mode, whose power,
+window, interlacing and multipole factors are known in closed form, and
+with mode counts enumerated by numpy.
+"""
+
+import numpy as np
+import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as st
+
+from oracle import pipeline as O
+
+
+def _kernel(d, window):
+    a = np.abs(d)
+    if window == 1:
+        return ((d >= -0.5) & (d < 0.5)).astype(float)
+    if window == 2:
+        return np.clip(1.0 - a, 0.0, None)
+    return np.where(a <= 0.5, 0.75 - d * d, np.where(a < 1.5, 0.5 * (1.5 - a) ** 2, 0.0))
+
+
+def _brute_grid(pos, nd, L, window, mass=None):
+    """Sum of the kernel over every cell centre, with periodic distances
+    in cell units."""
+    Ld = L / nd
+    g = np.zeros((nd, nd, nd))
+    c = np.arange(nd) + 0.5
+    for n, x in enumerate(pos):
+        w = []
+        for a in range(3):
+            d = x[a] / Ld - c
+            d = (d + nd / 2) % nd - nd / 2
+            w.append(_kernel(d, window))
+        m = 1.0 if mass is None else mass[n]
+        g += m * w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
+    return g
+
+
+@pytest.mark.parametrize("window", [1, 2, 5])
+def test_deposit_weights_match_kernels(window):
+    nd, L = 12, 48.0
+    rng = np.random.default_rng(window)
+    pos = rng.uniform(0, L, (40, 3))
+    pos[0] = [0.01, L - 0.02, 47.999]  # near the periodic edges
+    mass = rng.uniform(0.5, 2.0, 40)
+    g = O.grid_compute(pos, nd, L, window=window, mass=mass)[0, 0]
+    np.testing.assert_allclose(g, _brute_grid(pos, nd, L, window, mass), atol=1e-12)
+
+
+def test_single_particle_at_a_cell_centre():
+    """All the weight goes to that cell; the unweighted grid is normalised
+    to rho / rho_bar - 1 (gridmodule.c:197-202)."""
+    nd, L = 8, 32.0
+    Ld = L / nd
+    pos = np.array([[(2 + 0.5) * Ld, (5 + 0.5) * Ld, (7 + 0.5) * Ld]])
+    for window in (1, 2, 5):
+        g = O.grid_compute(pos, nd, L, window=window)[0, 0]
+        if window == 5:  # TSC spreads to the neighbours even at the centre
+            continue
+        expect = np.full((nd, nd, nd), -1.0)
+        expect[2, 5, 7] = nd**3 - 1.0
+        np.testing.assert_allclose(g, expect, atol=1e-12)
+
+
+@pytest.mark.parametrize("window", [1, 2, 5])
+def test_interlaced_grid_is_the_half_cell_shifted_deposit(window):
+    """The second grid holds the particles moved by -Lb/2 on every axis
+    (gridmodule.c:233-262)."""
+    nd, L = 10, 40.0
+    pos = np.random.default_rng(7).uniform(0, L, (30, 3))
+    g = O.grid_compute(pos, nd, L, window=window, interlacing=True, mass=np.ones(30))
+    np.testing.assert_allclose(g[0, 0], _brute_grid(pos, nd, L, window), atol=1e-12)
+    shifted = np.mod(pos - 0.5 * L / nd, L)
+    np.testing.assert_allclose(g[0, 1], _brute_grid(shifted, nd, L, window), atol=1e-12)
+
+
+def test_redshift_space_shift():
+    """direction d moves x_d by v_d (1+z)^3 / H(z), H = 100 sqrt(Om (1+z)^3
+    + 1 - Om) (gridmodule.c:167-169)."""
+    nd, L, z, Om = 10, 40.0, 0.5, 0.3
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(0, L, (25, 3))
+    vel = rng.normal(0, 200, (25, 3))
+    H = 100 * np.sqrt(Om * (1 + z) ** 3 + 1 - Om)
+    moved = pos.copy()
+    moved[:, 1] = np.mod(moved[:, 1] + vel[:, 1] * (1 + z) ** 3 / H, L)
+    g = O.grid_compute(pos, nd, L, window=2, mass=np.ones(25), direction=1, vel=vel, Om0=Om,
+                       z=z)[0, 0]
+    np.testing.assert_allclose(g, _brute_grid(moved, nd, L, 2), atol=1e-10)
+
+
+@settings(max_examples=25, deadline=None)
+@given(n=st.integers(1, 60), seed=st.integers(0, 2**31 - 1),
+       window=st.sampled_from([1, 2, 5]))
+def test_mass_is_conserved(n, seed, window):
+    nd, L = 8, 16.0
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0, L, (n, 3))
+    mass = rng.uniform(0.1, 3.0, n)
+    g = O.grid_compute(pos, nd, L, window=window, mass=mass)
+    assert abs(g.sum() - mass.sum()) < 1e-10 * mass.sum()
+    # unweighted: the normalised grid averages to zero
+    g = O.grid_compute(pos, nd, L, window=window)
+    assert abs(g.mean()) < 1e-12
+
+
+# ------------------------------------------------------------------- P(k)
+def _mode_table(nd, L, k_min, k_max, Nk):
+    """All half-complex modes kept by the reference's mask
+    (powermodule.c:50-51), with |k| and float64 bin index."""
+    kn = 2 * np.pi / L
+    i, j, k = np.meshgrid(np.arange(nd), np.arange(nd), np.arange(nd // 2 + 1), indexing="ij")
+    plane = (k == 0) | (k == nd // 2)
+    drop = plane & (((i > nd // 2) & (j > nd // 2)) | (i == 0) | (i == nd // 2) | (j == 0) |
+                    (j == nd // 2) | ((i > nd // 2) & (j < nd // 2)))
+    kx = np.where(2 * i < nd, i, i - nd) * kn
+    ky = np.where(2 * j < nd, j, j - nd) * kn
+    kz = np.where(2 * k < nd, k, k - nd) * kn
+    kmod = np.sqrt(kx * kx + ky * ky + kz * kz)
+    dk = (k_max - k_min) / Nk
+    b = np.floor((kmod - k_min) / dk).astype(int)
+    keep = ~drop & (b >= 0) & (b < Nk)
+    return kmod[keep], b[keep]
+
+
+@pytest.mark.parametrize("nd,Nk", [(16, 8), (16, 40), (12, 5)])
+def test_mode_counts_and_mean_k(nd, Nk):
+    L = 4.0 * nd
+    kmin, kmax = 2 * np.pi / L, np.pi / L * nd
+    kmod, b = _mode_table(nd, L, kmin, kmax, Nk)
+    grid = np.random.default_rng(1).normal(size=(1, 1, nd, nd, nd))
+    out = O.power_compute(grid, 1, nd, L, window=0, Nk=Nk, k_min=kmin, k_max=kmax)
+    np.testing.assert_array_equal(out["Nk"], np.bincount(b, minlength=Nk))
+    kmean = np.bincount(b, weights=kmod, minlength=Nk) / np.maximum(out["Nk"], 1)
+    np.testing.assert_allclose(out["k64"], kmean, rtol=1e-13)
+
+
+def test_total_mode_count_closed_form():
+    """Over all |k|: nd^2 (nd/2 - 1) interior modes plus 2 (nd/2 - 1)
+    (nd - 2) in the kz in {0, nd/2} planes survive the mask."""
+    nd, L = 16, 64.0
+    grid = np.zeros((1, 1, nd, nd, nd))
+    out = O.power_compute(grid, 1, nd, L, window=0, Nk=1, k_min=0.0,
+                          k_max=1.01 * np.sqrt(3) * np.pi * nd / L)
+    assert out["Nk"][0] == nd * nd * (nd // 2 - 1) + 2 * (nd // 2 - 1) * (nd - 2)
+
+
+@pytest.mark.parametrize("window", [0, 1, 2, 5])
+@pytest.mark.parametrize("il", [False, True])
+def test_single_z_mode_power(window, il):
+    """grid = A cos(2 pi m z / nd): the only non-zero kept mode is (0, 0, m)
+    with |delta_k|^2 = A^2 L^3 / 4 after Normk = (L/nd^2)^(3/2); it is
+    divided by W^2 (W = sinc(pi m/nd)^p, p = 1, 2, 3 for NGP, CIC, TSC,
+    spectrum_h.c:58-59), multiplied by cos^2(theta/2), theta = pi m / nd,
+    when an identical interlaced copy is combined (spectrum_h.c:130-131),
+    and averaged over the bin's modes; the l = 2 multipole with the z
+    direction has P_2(1) = 1 and the factor 5."""
+    nd, L, Nk, m, A = 16, 64.0, 8, 3, 0.6
+    kmin, kmax = 2 * np.pi / L, np.pi / L * nd
+    z = np.indices((nd, nd, nd))[2]
+    g = A * np.cos(2 * np.pi * m * z / nd)
+    grid = np.stack([g, g])[None] if il else g[None, None]
+    out = O.power_compute(grid, 1, nd, L, window=window, interlacing=il, Nk=Nk, k_min=kmin,
+                          k_max=kmax, l_max=2, direction=2)
+    kmod, b = _mode_table(nd, L, kmin, kmax, Nk)
+    counts = np.bincount(b, minlength=Nk)
+    bm = int(np.floor((2 * np.pi * m / L - kmin) / ((kmax - kmin) / Nk)))
+    u = np.pi * m / nd
+    W = 1.0 if window == 0 else (np.sin(u) / u) ** {1: 1, 2: 2, 5: 3}[window]
+    P = A**2 * L**3 / 4 / W**2 / counts[bm]
+    if il:
+        P *= np.cos(u / 2) ** 2
+    expect0 = np.zeros(Nk)
+    expect0[bm] = P
+    np.testing.assert_allclose(out["Pk64"][0, 0], expect0, rtol=1e-10, atol=1e-12 * P)
+    np.testing.assert_allclose(out["Pk64"][0, 1], 5 * expect0, rtol=1e-10, atol=1e-12 * P)
+    # along x, mu = 0 and P_2(0) = -1/2
+    out = O.power_compute(grid, 1, nd, L, window=window, interlacing=il, Nk=Nk, k_min=kmin,
+                          k_max=kmax, l_max=2, direction=0)
+    np.testing.assert_allclose(out["Pk64"][0, 1], -2.5 * expect0, rtol=1e-10, atol=1e-12 * P)
+
+
+def test_cross_spectrum_of_two_tracers():
+    """Pairs (l, m <= l) in order (0,0), (1,0), (1,1): the cross power of a
+    grid with twice itself is twice the auto power (powermodule.c:66-72)."""
+    nd, L = 16, 64.0
+    g = np.random.default_rng(3).normal(size=(nd, nd, nd))
+    grid = np.stack([g[None], 2 * g[None]])
+    out = O.power_compute(grid, 2, nd, L, window=2, Nk=6)
+    P = out["Pk64"][:, 0]
+    np.testing.assert_allclose(P[1], 2 * P[0], rtol=1e-12)
+    np.testing.assert_allclose(P[2], 4 * P[0], rtol=1e-12)
