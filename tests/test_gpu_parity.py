@@ -107,10 +107,13 @@ def test_lpt(bbks, prec, do2):
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("window", [1, 2, 5])
 @pytest.mark.parametrize("il", [0, 1])
-def test_grid_compute(prec, window, il):
+@pytest.mark.parametrize("nd", [32, 24, 20])
+def test_grid_compute(prec, window, il, nd):
+    """Particles in random order (the direct atomic scatter) on grids of
+    several sizes."""
     set_precision(prec)
     rng = np.random.default_rng(2)
-    nd, L = 32, 128.0
+    L = 128.0
     R = np.float32 if prec == 4 else np.float64
     pos = (rng.random((20000, 3)) * L).astype(R)
     ref = O.grid_compute(pos, nd, L, window=window, interlacing=bool(il), precision=prec)
@@ -126,13 +129,12 @@ def test_grid_compute(prec, window, il):
 @pytest.mark.parametrize("il", [0, 1])
 @pytest.mark.parametrize("nL,nd", [(32, 32), (32, 64), (64, 64), (64, 128)])
 def test_grid_compute_lattice_tiles(prec, window, il, nL, nd):
-    """Particles in lattice order (np = nL^3) take the tiled deposit: LDS
-    boxes per 16^3 lattice tile, gathered per grid tile.  (32, 32): boxes
-    fit, including one that wraps the whole periodic grid (an outlier at the
-    antipode); (32, 64): 16-particle tiles span 32+ cells and fall back to
-    atomics; (64, *): four tiles per axis, so the 3x3x3 cover test decides,
-    and the outliers' tiles fall back.  Weighted and redshift-space variants
-    run through the same tiles."""
+    """Particles in Lagrangian lattice order (np = nL^3, as LPT writes
+    them) take the LDS-tiled deposit.  (32, 32): boxes fit, including one
+    that wraps the whole periodic grid (an outlier at the antipode);
+    (32, 64): 16-particle tiles span 32+ cells a side and deposit directly;
+    (64, *): four lattice tiles per axis, outlier tiles deposit directly.
+    Weighted and redshift-space variants run through the same tiles."""
     set_precision(prec)
     R = np.float32 if prec == 4 else np.float64
     L = 128.0
