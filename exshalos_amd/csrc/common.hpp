@@ -72,6 +72,7 @@ enum Slot {
   BUF_PART,     // P(k) block partials
   BUF_TAB,      // P(k) spline knots
   BUF_SHELL,    // shell offset table of the halo finder
+  BUF_EFF,      // halo growth: delta with flagged cells set to -Mtot
   BUF_NSLOTS
 };
 

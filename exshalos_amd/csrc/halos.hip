@@ -192,6 +192,7 @@ template <typename T>
 struct GrowArgs {
   int n0, n1, n2;
   const T* delta;
+  T* eff;          // delta, or -Mtot where a halo (or a rejected sphere) has flagged the cell
   int* flag;
   unsigned long long* claim;
   ShellTab sh;
@@ -245,13 +246,19 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
     den = den_tmp;
     count = count_tmp;
     den_tmp = den * ((double)count);
-    for (int o = a.sh.start[grows_tmp]; o < a.sh.start[grows_tmp + 1]; o++) {
-      size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
-      if (a.flag[id] != FLAG_FREE)
-        den_tmp += (double)a.negMtot;
-      else
-        den_tmp += (double)a.delta[id];
-      count_tmp++;
+    // the shell's cells in cube order; loads issued in batches, summed in
+    // order (flagged cells hold -Mtot in eff, as the reference adds)
+    const int o0 = a.sh.start[grows_tmp], o1 = a.sh.start[grows_tmp + 1];
+    for (int ob = o0; ob < o1; ob += 8) {
+      T v[8];
+      const int nb = min(8, o1 - ob);
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < nb) v[e] = a.eff[cell_off(ci, cj, ck, a.sh.offs[ob + e], a.n0, a.n1, a.n2)];
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < nb) den_tmp += (double)v[e];
+      count_tmp += nb;
     }
     den_tmp = den_tmp / ((double)count_tmp);
   }
@@ -271,9 +278,17 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
     den_tmp = den;
     count_tmp = count;
     den = den * ((double)count);
-    for (int o = a.sh.start[grows]; o < a.sh.start[grows + 1]; o++) {
-      den -= (double)a.delta[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)];
-      count--;
+    const int o0 = a.sh.start[grows], o1 = a.sh.start[grows + 1];
+    for (int ob = o0; ob < o1; ob += 8) {
+      T v[8];
+      const int nb = min(8, o1 - ob);
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < nb) v[e] = a.delta[cell_off(ci, cj, ck, a.sh.offs[ob + e], a.n0, a.n1, a.n2)];
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < nb) den -= (double)v[e];
+      count -= nb;
     }
     if (count > 0) den = den / ((double)count);
     if (count < count_tmp) grows_tmp = grows;
@@ -303,14 +318,22 @@ __global__ void k_commit(GrowArgs<T> a, const int* pend, int npend, unsigned lon
   decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
   // safe iff no denser pending candidate claims a cell of the read ball
   const unsigned lim = 0xffffffffu - (unsigned)rank;
-  for (int o = 0; o < a.sh.start[G + 1]; o++) {
-    size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
-    unsigned long long v = a.claim[id];
-    if ((v & 0xffffffff00000000ull) == tag_hi && (unsigned)(v & 0xffffffffu) > lim) {
-      int p = atomicAdd(nnext, 1);
-      next[p] = rank;
-      return;
-    }
+  const int oend = a.sh.start[G + 1];
+  bool blocked = false;
+  for (int ob = 0; ob < oend && !blocked; ob += 8) {
+    unsigned long long v[8];
+    const int nb = min(8, oend - ob);
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      v[e] = e < nb ? a.claim[cell_off(ci, cj, ck, a.sh.offs[ob + e], a.n0, a.n1, a.n2)] : 0ull;
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      blocked |= (v[e] & 0xffffffff00000000ull) == tag_hi && (unsigned)(v[e] & 0xffffffffu) > lim;
+  }
+  if (blocked) {
+    int p = atomicAdd(nnext, 1);
+    next[p] = rank;
+    return;
   }
   int count = a.cnt[rank];
   if (count == 0) {
@@ -325,8 +348,11 @@ __global__ void k_commit(GrowArgs<T> a, const int* pend, int npend, unsigned lon
   }
   bool keep = count >= a.Nmin;
   int v = keep ? (rank << 1) : FLAG_REJECTED;
-  for (int o = 0; o < a.sh.start[R2]; o++)
-    a.flag[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)] = v;
+  for (int o = 0; o < a.sh.start[R2]; o++) {
+    const size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
+    a.flag[id] = v;
+    a.eff[id] = a.negMtot;
+  }
   if (keep) a.flag[cell] = (rank << 1) | 1;
   a.kept[rank] = keep ? 1 : 0;
 }
@@ -623,6 +649,9 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   int* inv = (int*)(cand + o_inv);
   long long* pb = (long long*)(cand + o_pb);
   int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
+  T* eff = (T*)ctx_buf(c, BUF_EFF, (size_t)ng * sizeof(T));
+  EXH_CHECK_HIP(hipMemcpyAsync(eff, delta, (size_t)ng * sizeof(T), hipMemcpyDeviceToDevice,
+                               c->stream));
   unsigned long long* claim = (unsigned long long*)ctx_buf(c, BUF_CLAIM, (size_t)ng * 8);
   S.flag = flag;
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ng, 256 * 8)), dim3(256), 0, c->stream, flag, ng,
@@ -637,6 +666,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   ga.n1 = n1;
   ga.n2 = n2;
   ga.delta = delta;
+  ga.eff = eff;
   ga.flag = flag;
   ga.claim = claim;
   ga.sh = S.sh;
