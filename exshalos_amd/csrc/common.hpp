@@ -141,7 +141,7 @@ struct exh_ctx {
   size_t pinned_cap = 0;
   // halo finder statistics of the last run
   long long stat_peaks = 0, stat_candidates = 0, stat_rounds = 0, stat_sort_levels = 0,
-            stat_sort_doubling = 0;
+            stat_sort_doubling = 0, stat_sort_async_open = 0;
 };
 
 namespace exh {

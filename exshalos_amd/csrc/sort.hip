@@ -363,6 +363,56 @@ __global__ void k_double_round(const long long* tile_off, int nseg, const SegInf
   if (__syncthreads_or(open) && threadIdx.x == 0) round_open[k] = 1;
 }
 
+// Asynchronous pointer jumping over the same entries: every thread keeps
+// replacing its entries' pointers by the pointer they point to until each
+// leaves the stream prefix (>= r).  Pointers only ever move forward along
+// their chain and every stored value is a valid chain position, so reading
+// an entry before or after another thread advanced it is equally correct;
+// nothing waits for another thread.  Loads and stores are agent-scope
+// atomics so that advances made on other XCDs become visible.  Entries
+// still open after `guard` sweeps set *open (the host then finishes with
+// the synchronous rounds of k_double_round).
+__global__ __launch_bounds__(256) void k_double_async(const long long* tile_off, int nseg,
+                                                      const SegInfo* info, const int* seg_need,
+                                                      const int* need_any, int* nxt, int guard,
+                                                      int* open) {
+  if (!*need_any) return;
+  const long long t = blockIdx.x;
+  if (t >= tile_off[nseg]) return;
+  const int sg = find_seg(tile_off, nseg, t);
+  if (!seg_need[sg]) return;
+  const SegInfo g = info[sg];
+  const long long q0 = (t - g.first_tile) * TILE;
+  const bool last = t == tile_off[sg + 1] - 1;
+  const long long q1 = last ? g.r : min(q0 + TILE, g.r);
+  int* X = nxt + g.lo;
+  const int r = (int)g.r;
+  int v[PER_THREAD];
+  for (int e = 0; e < PER_THREAD; e++) {
+    const long long q = q0 + threadIdx.x + (long long)e * 256;
+    v[e] = q < q1 ? X[q] : r;
+  }
+  bool any = true;
+  for (int it = 0; it < guard && any; it++) {
+    int w[PER_THREAD];
+#pragma unroll
+    for (int e = 0; e < PER_THREAD; e++)
+      w[e] = v[e] < r ? __hip_atomic_load(&X[v[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      : v[e];
+    any = false;
+#pragma unroll
+    for (int e = 0; e < PER_THREAD; e++) {
+      if (v[e] < r) {
+        const long long q = q0 + threadIdx.x + (long long)e * 256;
+        v[e] = w[e];
+        __hip_atomic_store(&X[q], w[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        any |= w[e] < r;
+      }
+    }
+  }
+  if (any) atomicExch(open, 1);
+}
+
 template <typename K>
 __global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* valsB,
                               const long long* tile_off, int nseg, const SegInfo* info,
@@ -654,6 +704,7 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
   constexpr int SMALL = SmallCfg<K>::N;
   c->stat_sort_levels = 0;
   c->stat_sort_doubling = 0;
+  c->stat_sort_async_open = 0;
   if (n < 2) return;
   // scratch layout in BUF_SORT_AUX
   const long long maxseg = n / (SMALL + 1) * 2 + 4;
@@ -693,7 +744,8 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
   // level state read back once per level: [0] large children, [1] small
   // segments so far, [2] some segment needs doubling, [4..5] most steps an
   // unfinished chain still needs, [6..7] elements in large children; [8] doubling error (kept
-  // over all levels); [16..63] per-round flags of the doubling
+  // over all levels); [9] chains left open by the asynchronous doubling;
+  // [16..63] per-round flags of the doubling
   int* hc = (int*)ctx_pinned(c, 64);
   int* seg_need = (int*)(aux + o_need);
   int* nxt = (int*)(aux + o_nxa);
@@ -763,11 +815,21 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
       EXH_REQUIRE(rounds <= 40, EXH_ERR_INTERNAL, "peak sort: segment too long for doubling");
       hipLaunchKernelGGL(k_double_init, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, toff,
                          nseg, info, seg_need, cnt + 2, arot, nxt);
+      hipLaunchKernelGGL(k_double_async, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, toff,
+                         nseg, info, seg_need, cnt + 2, nxt, 64, cnt + 9);
       EXH_LAUNCH_CHECK();
-      for (int k = 0; k < rounds; k++) {
-        hipLaunchKernelGGL(k_double_round, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
-                           toff, nseg, info, seg_need, cnt + 2, nxt, round_open, k);
-        EXH_LAUNCH_CHECK();
+      EXH_CHECK_HIP(hipMemcpyAsync(hc + 9, cnt + 9, 4, hipMemcpyDeviceToHost, c->stream));
+      EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+      if (hc[9]) {
+        // some chain is still open: the synchronous rounds finish every
+        // chain within the bound
+        c->stat_sort_async_open++;
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt + 9, 1, 0);
+        for (int k = 0; k < rounds; k++) {
+          hipLaunchKernelGGL(k_double_round, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
+                             toff, nseg, info, seg_need, cnt + 2, nxt, round_open, k);
+          EXH_LAUNCH_CHECK();
+        }
       }
       hipLaunchKernelGGL(k_tile_chase2<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
                          keys, vals, kB, vB, toff, nseg, info, seg_need, cnt + 2, nxt, s0, cnt + 8);
