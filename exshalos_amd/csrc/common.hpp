@@ -64,6 +64,8 @@ enum Slot {
   BUF_SORT_K,   // sort ping-pong keys
   BUF_SORT_V,   // sort ping-pong values
   BUF_SORT_AUX, // sort scratch
+  BUF_SORT_K2,  // second sort work buffer (keys)
+  BUF_SORT_V2,  // second sort work buffer (values)
   BUF_CAND,     // halo candidate state
   BUF_HALO,     // halo outputs
   BUF_MISC,     // small scratch (counters, tables)

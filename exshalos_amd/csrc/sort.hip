@@ -44,19 +44,66 @@ struct SegInfo {
 
 // ---------------------------------------------------------------------------
 // level kernels
+//
+// Keys and values move between two work buffers level by level (src ->
+// dst).  Elements whose place is final (pivots, children of one element)
+// are also written to the output array; segments of 2..SMALL elements are
+// queued with the buffer that holds them and finished by k_small_sort.
+// Children of more than SMALL elements become the next level's segments:
+// each reserves its list slot and its range of tiles with one packed 64-bit
+// atomic (slot count in the high half, tile count in the low half), so the
+// tile offsets grow with the slot index and no scan is needed.
 // ---------------------------------------------------------------------------
-__global__ void k_seg_ntiles(const long long* seg_lo, const long long* seg_hi, int nseg,
-                             long long* ntiles, long long* j0) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg) return;
-  long long len = seg_hi[s] - seg_lo[s];  // elements lo..hi-1
-  ntiles[s] = (len + TILE - 1) / TILE;
-  j0[s] = seg_hi[s];  // "no > element" until found
-}
-
 __global__ void k_fill_int(int* p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+struct LevelState {
+  unsigned long long pk;          // next level: (segments << 32) | tiles
+  unsigned long long next_elems;  // elements in the next level's segments
+  unsigned long long rmax;        // most steps an unfinished chain still needs
+  unsigned long long nsmall;      // small segments queued so far (all levels)
+  unsigned long long need_any;    // some segment needs pointer doubling
+  unsigned long long async_open;  // chains left open by k_double_async
+  unsigned long long err;         // doubling did not converge (all levels)
+  unsigned long long pad;
+};
+
+struct LevelBufs {
+  const long long* lo;  // current segments
+  const long long* hi;
+  const long long* toff;  // first tile of each segment (ascending)
+  unsigned long long* j0;  // first ">" element of each segment
+  int nseg;
+  long long ntiles;
+  long long* nlo;  // next level's segments
+  long long* nhi;
+  long long* ntoff;
+  unsigned long long* nj0;
+  long long* slo;  // small segments and the buffer holding each
+  long long* shi;
+  int* sbuf;
+  int dst_id;  // 1 or 2: buffer id of dst for the small list
+  long long small_max;
+  SegInfo* info;
+  int* seg_need;
+  LevelState* st;
+};
+
+__device__ inline long long seg_end_tile(const LevelBufs& b, int sg) {
+  return sg + 1 < b.nseg ? b.toff[sg + 1] : b.ntiles;
+}
+
+__global__ void k_sort_init(long long* lo, long long* hi, long long* toff, unsigned long long* j0,
+                            LevelState* st, long long n) {
+  if (threadIdx.x == 0) {
+    lo[0] = 0;
+    hi[0] = n - 1;
+    toff[0] = 0;
+    j0[0] = (unsigned long long)(n - 1);
+    *st = LevelState{};
+  }
 }
 
 __device__ inline int find_seg(const long long* tile_off, int nseg, long long t) {
@@ -79,22 +126,25 @@ __device__ inline long long wave_min_ll(long long v) {
 
 // Per tile: number of "<=" elements and the first ">" element.  Lanes
 // read consecutive elements; one atomic per block for the segment's j0.
+// Block 0 also clears the per-level counters (read by the host after the
+// previous level).
 template <typename K>
-__global__ __launch_bounds__(256) void k_tile_count(const K* keys, const long long* seg_lo,
-                                                    const long long* seg_hi,
-                                                    const long long* tile_off, int nseg,
-                                                    long long* tile_le, unsigned long long* j0) {
+__global__ __launch_bounds__(256) void k_tile_count(const K* keys, LevelBufs b, long long* tile_le) {
   __shared__ int shc[4];
   __shared__ long long shj[4];
   const long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) {  // the grid is an upper bound of the tile count
-    if (threadIdx.x == 0) tile_le[t] = 0;
-    return;
+  if (t == 0 && threadIdx.x == 0) {
+    LevelState* st = b.st;
+    st->pk = 0;
+    st->next_elems = 0;
+    st->rmax = 0;
+    st->need_any = 0;
+    st->async_open = 0;
   }
-  const int s = find_seg(tile_off, nseg, t);
-  const long long lo = seg_lo[s], hi = seg_hi[s];
+  const int s = find_seg(b.toff, b.nseg, t);
+  const long long lo = b.lo[s], hi = b.hi[s];
   const K pivot = keys[hi];
-  const long long start = lo + (t - tile_off[s]) * TILE;
+  const long long start = lo + (t - b.toff[s]) * TILE;
   int cnt = 0;
   long long first_gt = hi;
   for (int q = 0; q < PER_THREAD; q++) {
@@ -119,70 +169,72 @@ __global__ __launch_bounds__(256) void k_tile_count(const K* keys, const long lo
     const long long fg = min(min(shj[0], shj[1]), min(shj[2], shj[3]));
     // most tiles come after the segment's first ">": a plain read of the
     // running minimum skips their atomic on this one address
-    if (fg < hi && (unsigned long long)fg < __atomic_load_n(&j0[s], __ATOMIC_RELAXED))
-      atomicMin(&j0[s], (unsigned long long)fg);
+    if (fg < hi && (unsigned long long)fg < __atomic_load_n(&b.j0[s], __ATOMIC_RELAXED))
+      atomicMin(&b.j0[s], (unsigned long long)fg);
   }
 }
 
-template <typename K>
-__global__ void k_seg_final(const K* keysA, const int* valsA, K* keysB, int* valsB,
-                            const long long* seg_lo, const long long* seg_hi,
-                            const long long* tile_off, const long long* tile_pre,
-                            const unsigned long long* j0s, int nseg, SegInfo* info,
-                            int* arot, long long small_max, long long* next_lo,
-                            long long* next_hi, int* n_next, long long* small_lo,
-                            long long* small_hi, int* n_small, unsigned long long* next_elems) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg) return;
-  long long lo = seg_lo[s], hi = seg_hi[s];
-  long long ft = tile_off[s], lt = tile_off[s + 1];
-  long long c = tile_pre[lt] - tile_pre[ft];
-  long long m = (hi - lo) - c;
-  long long j0 = (long long)j0s[s];
-  long long r = m > 0 ? (c - (j0 - lo)) + 1 : 0;
-  SegInfo in;
-  in.lo = lo;
-  in.hi = hi;
-  in.c = c;
-  in.r = r;
-  in.j0 = j0;
-  in.first_tile = ft;
-  info[s] = in;
-  keysB[lo + c] = keysA[hi];
-  valsB[lo + c] = valsA[hi];
-  if (m > 0) arot[lo + r - 1] = (int)m;
-  // children
-  long long cl[2] = {lo, lo + c + 1}, ch[2] = {lo + c - 1, hi};
-  for (int q = 0; q < 2; q++) {
-    long long len = ch[q] - cl[q] + 1;
-    if (len > small_max) {
-      int p = atomicAdd(n_next, 1);
-      next_lo[p] = cl[q];
-      next_hi[p] = ch[q];
-      atomicAdd(next_elems, (unsigned long long)len);
-    } else if (len >= 2) {
-      int p = atomicAdd(n_small, 1);
-      small_lo[p] = cl[q];
-      small_hi[p] = ch[q];
-    }
+__device__ inline SegInfo seg_info(const LevelBufs& b, const long long* tile_pre, int s) {
+  SegInfo g;
+  g.lo = b.lo[s];
+  g.hi = b.hi[s];
+  g.first_tile = b.toff[s];
+  const long long lt = seg_end_tile(b, s);
+  g.c = tile_pre[lt] - tile_pre[g.first_tile];
+  const long long m = (g.hi - g.lo) - g.c;
+  g.j0 = (long long)b.j0[s];
+  g.r = m > 0 ? (g.c - (g.j0 - g.lo)) + 1 : 0;
+  return g;
+}
+
+// Queue child [cl, ch] (held in buffer dst_id) as a next-level or small
+// segment.
+__device__ inline void push_child(const LevelBufs& b, long long cl, long long ch) {
+  const long long len = ch - cl + 1;
+  if (len > b.small_max) {
+    const unsigned long long nt = (unsigned long long)((len - 1 + TILE - 1) / TILE);
+    const unsigned long long old = atomicAdd(&b.st->pk, (1ull << 32) | nt);
+    const int p = (int)(old >> 32);
+    b.nlo[p] = cl;
+    b.nhi[p] = ch;
+    b.ntoff[p] = (long long)(old & 0xffffffffull);
+    b.nj0[p] = (unsigned long long)ch;  // "no > element" until found
+    atomicAdd(&b.st->next_elems, (unsigned long long)len);
+  } else if (len >= 2) {
+    const long long p = (long long)atomicAdd(&b.st->nsmall, 1ull);
+    b.slo[p] = cl;
+    b.shi[p] = ch;
+    b.sbuf[p] = b.dst_id;
   }
 }
 
-// Place the "<=" elements at lo + rank, record the rotation table A and
-// each ">" element's first stream index.  Wave w of a tile takes its
-// elements [w * 512, (w + 1) * 512) in eight coalesced rows of 64; ranks
-// come from ballots, the wave offsets from LDS.
+// Place the "<=" elements at lo + rank (dst; also the output when the left
+// child is a single element), record the rotation table A and each ">"
+// element's first stream index.  Wave w of a tile takes its elements
+// [w * 512, (w + 1) * 512) in eight coalesced rows of 64; ranks come from
+// ballots, the wave offsets from LDS.  The segment's first tile also
+// publishes its SegInfo, places the pivot in the output, closes A and
+// queues the two children.
 template <typename K>
 __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int* valsA, K* keysB,
-                                                      int* valsB, const long long* tile_off,
-                                                      const long long* tile_pre, int nseg,
-                                                      const SegInfo* info, int* arot, int* s0) {
+                                                      int* valsB, K* keysF, int* valsF,
+                                                      LevelBufs b, const long long* tile_pre,
+                                                      int* arot, int* s0) {
   __shared__ int shw[4];
   const long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) return;
-  const int sg = find_seg(tile_off, nseg, t);
-  const SegInfo g = info[sg];
+  const int sg = find_seg(b.toff, b.nseg, t);
+  const SegInfo g = seg_info(b, tile_pre, sg);
   const K pivot = keysA[g.hi];
+  const long long m = (g.hi - g.lo) - g.c;
+  if (t == g.first_tile && threadIdx.x == 0) {
+    b.info[sg] = g;
+    b.seg_need[sg] = 0;
+    keysF[g.lo + g.c] = pivot;
+    valsF[g.lo + g.c] = valsA[g.hi];
+    if (m > 0) arot[g.lo + g.r - 1] = (int)m;
+    push_child(b, g.lo, g.lo + g.c - 1);
+    push_child(b, g.lo + g.c + 1, g.hi);
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long long base = g.lo + (t - g.first_tile) * TILE + (long long)w * (TILE / 4);
   K kv[PER_THREAD];
@@ -201,13 +253,19 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int*
   for (int v = 0; v < w; v++) run += shw[v];
   const unsigned long long below = lanemask_lt();
   const long long jlead = g.j0 - g.lo;  // "<=" elements before the first ">"
+  const bool single_left = g.c == 1;
   for (int q = 0; q < PER_THREAD; q++) {
     const long long j = base + q * 64 + lane;
     if (j < g.hi) {
       const long long rank = run + __popcll(bal[q] & below);
       if ((bal[q] >> lane) & 1ull) {
+        const int vv = valsA[j];
         keysB[g.lo + rank] = kv[q];
-        valsB[g.lo + rank] = valsA[j];
+        valsB[g.lo + rank] = vv;
+        if (single_left) {
+          keysF[g.lo] = kv[q];
+          valsF[g.lo] = vv;
+        }
         if (g.r > 0 && j > g.j0) arot[g.lo + rank - jlead] = (int)((j - g.lo) - rank);
       } else {
         const long long gt_before = (j - g.lo) - rank;
@@ -222,24 +280,32 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int*
 // its eight elements' chains in lock-step, so their loads are in flight
 // together.  Elements that finish are placed and their s0 set to S0_DONE;
 // the others keep their progress in s0, and the block flags its segment
-// for pointer doubling (one write per block, not per element).
+// for pointer doubling (one write per block, not per element).  The block
+// also initialises the doubling pointers jmp[q] = q + A(q) of its tile's
+// stream indices.
 static constexpr int S0_DONE = 0x7fffffff;
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_tile_chase(const K* keysA, const int* valsA, K* keysB,
-                                                    int* valsB, const long long* tile_off,
-                                                    int nseg, const SegInfo* info,
-                                                    const int* arot, int* s0, int* seg_need,
-                                                    int* need_any, unsigned long long* rmax,
-                                                    int cap) {
+                                                    int* valsB, K* keysF, int* valsF,
+                                                    LevelBufs b, const int* arot, int* s0,
+                                                    int* jmp, int cap) {
   const long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) return;
-  const int sg = find_seg(tile_off, nseg, t);
-  const SegInfo g = info[sg];
+  const int sg = find_seg(b.toff, b.nseg, t);
+  const SegInfo g = b.info[sg];
   const K pivot = keysA[g.hi];
   const long long start = g.lo + (t - g.first_tile) * TILE;
   const int r = (int)g.r;
   const int* A = arot + g.lo;
+  const bool single_right = (g.hi - g.lo) - g.c == 1;
+  // doubling pointers of this tile's stream indices [q0, q1)
+  {
+    const long long q0 = (t - g.first_tile) * TILE;
+    const bool last = t == seg_end_tile(b, sg) - 1;
+    const long long q1 = last ? g.r : min(q0 + TILE, g.r);
+    for (long long q = q0 + threadIdx.x; q < q1; q += 256)
+      jmp[g.lo + q] = (int)q + max(A[q], 1);  // >= r: left the stream prefix
+  }
   K kv[PER_THREAD];
   int s[PER_THREAD];
   bool act[PER_THREAD];
@@ -277,8 +343,13 @@ __global__ __launch_bounds__(256) void k_tile_chase(const K* keysA, const int* v
     }
     s0[j] = S0_DONE;
     const long long dst = g.lo + g.c + 1 + (s[q] - r);
+    const int vv = valsA[j];
     keysB[dst] = kv[q];
-    valsB[dst] = valsA[j];
+    valsB[dst] = vv;
+    if (single_right) {
+      keysF[dst] = kv[q];
+      valsF[dst] = vv;
+    }
   }
   __shared__ unsigned long long sh_rem[4];
   for (int o = 32; o > 0; o >>= 1) rem = max(rem, (unsigned long long)__shfl_xor(rem, o, 64));
@@ -287,9 +358,9 @@ __global__ __launch_bounds__(256) void k_tile_chase(const K* keysA, const int* v
   if (threadIdx.x == 0) {
     rem = max(max(sh_rem[0], sh_rem[1]), max(sh_rem[2], sh_rem[3]));
     if (rem > 0) {
-      seg_need[sg] = 1;
-      if (!__atomic_load_n(need_any, __ATOMIC_RELAXED)) atomicExch(need_any, 1);
-      if (__atomic_load_n(rmax, __ATOMIC_RELAXED) < rem) atomicMax(rmax, rem);
+      b.seg_need[sg] = 1;
+      if (!__atomic_load_n(&b.st->need_any, __ATOMIC_RELAXED)) atomicExch(&b.st->need_any, 1ull);
+      if (__atomic_load_n(&b.st->rmax, __ATOMIC_RELAXED) < rem) atomicMax(&b.st->rmax, rem);
     }
   }
 }
@@ -304,88 +375,25 @@ static int chase_cap() {
   return v;
 }
 
-// Pointer doubling on nxt(q) = q + A(q) over the stream indices of flagged
-// segments: after k rounds one entry covers at least 2^k chase steps.
-// Values are stored relative to the segment start.  Each entry is either a
-// stream index below r on the chain from q or the chain's first index >= r,
-// so updating the array in place is exact: a round that reads an entry
-// another block has already advanced only jumps further along the same
-// chain, never past its exit.  A round whose predecessor left no entry
-// below r (round_open[k-1] == 0) returns at once, so the host can launch a
-// fixed number of rounds without waiting for the device.
-__global__ void k_double_init(const long long* tile_off, int nseg, const SegInfo* info,
-                              const int* seg_need, const int* need_any, const int* arot,
-                              int* nxt) {
-  if (!*need_any) return;
-  long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) return;
-  int sg = find_seg(tile_off, nseg, t);
-  if (!seg_need[sg]) return;
-  SegInfo g = info[sg];
-  long long q0 = (t - g.first_tile) * TILE;
-  bool last = t == tile_off[sg + 1] - 1;
-  long long q1 = last ? g.r : min(q0 + TILE, g.r);
-  for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
-    nxt[g.lo + q] = (int)q + max(arot[g.lo + q], 1);  // >= r: left the stream prefix
-  }
-}
-
-__global__ void k_double_round(const long long* tile_off, int nseg, const SegInfo* info,
-                               const int* seg_need, const int* need_any, int* nxt,
-                               int* round_open, int k) {
-  if (!*need_any || (k > 0 && !round_open[k - 1])) return;
-  long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) return;
-  int sg = find_seg(tile_off, nseg, t);
-  if (!seg_need[sg]) return;
-  SegInfo g = info[sg];
-  long long q0 = (t - g.first_tile) * TILE;
-  bool last = t == tile_off[sg + 1] - 1;
-  long long q1 = last ? g.r : min(q0 + TILE, g.r);
-  // all loads of a thread's entries are issued before its stores
-  int* X = nxt + g.lo;
-  const int r = (int)g.r;
-  int v[PER_THREAD];
-  for (int e = 0; e < PER_THREAD; e++) {
-    const long long q = q0 + threadIdx.x + (long long)e * 256;
-    v[e] = q < q1 ? X[q] : r;
-  }
-  int w[PER_THREAD];
-  for (int e = 0; e < PER_THREAD; e++) w[e] = v[e] < r ? X[v[e]] : v[e];
-  bool open = false;
-  for (int e = 0; e < PER_THREAD; e++) {
-    const long long q = q0 + threadIdx.x + (long long)e * 256;
-    if (v[e] < r) {
-      X[q] = w[e];
-      open |= w[e] < r;
-    }
-  }
-  if (__syncthreads_or(open) && threadIdx.x == 0) round_open[k] = 1;
-}
-
-// Asynchronous pointer jumping over the same entries: every thread keeps
-// replacing its entries' pointers by the pointer they point to until each
-// leaves the stream prefix (>= r).  Pointers only ever move forward along
-// their chain and every stored value is a valid chain position, so reading
-// an entry before or after another thread advanced it is equally correct;
-// nothing waits for another thread.  Loads and stores are agent-scope
-// atomics so that advances made on other XCDs become visible.  Entries
-// still open after `guard` sweeps set *open (the host then finishes with
-// the synchronous rounds of k_double_round).
-__global__ __launch_bounds__(256) void k_double_async(const long long* tile_off, int nseg,
-                                                      const SegInfo* info, const int* seg_need,
-                                                      const int* need_any, int* nxt, int guard,
-                                                      int* open) {
-  if (!*need_any) return;
+// Asynchronous pointer jumping over the stream indices of flagged
+// segments: every thread keeps replacing its entries' pointers jmp[q] by
+// the pointer they point to until each leaves the stream prefix (>= r).
+// Each entry is a stream index below r on the chain from q or the chain's
+// first index >= r; pointers only ever move forward along their chain, so
+// reading an entry before or after another thread advanced it is equally
+// correct, and nothing waits for another thread.  Loads and stores are
+// agent-scope atomics so that advances made on other XCDs become visible.
+// Entries still open after `guard` sweeps set async_open; the host then
+// finishes with the synchronous rounds of k_double_round.
+__global__ __launch_bounds__(256) void k_double_async(LevelBufs b, int* jmp, int guard) {
   const long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) return;
-  const int sg = find_seg(tile_off, nseg, t);
-  if (!seg_need[sg]) return;
-  const SegInfo g = info[sg];
+  const int sg = find_seg(b.toff, b.nseg, t);
+  if (!b.seg_need[sg]) return;
+  const SegInfo g = b.info[sg];
   const long long q0 = (t - g.first_tile) * TILE;
-  const bool last = t == tile_off[sg + 1] - 1;
+  const bool last = t == seg_end_tile(b, sg) - 1;
   const long long q1 = last ? g.r : min(q0 + TILE, g.r);
-  int* X = nxt + g.lo;
+  int* X = jmp + g.lo;
   const int r = (int)g.r;
   int v[PER_THREAD];
   for (int e = 0; e < PER_THREAD; e++) {
@@ -410,60 +418,69 @@ __global__ __launch_bounds__(256) void k_double_async(const long long* tile_off,
       }
     }
   }
-  if (any) atomicExch(open, 1);
+  if (any) atomicExch(&b.st->async_open, 1ull);
 }
 
+// Synchronous doubling round k (in place, same invariant as above); a
+// round whose predecessor left no entry open returns at once.
+__global__ __launch_bounds__(256) void k_double_round(LevelBufs b, int* jmp, int* round_open, int k) {
+  if (k > 0 && !round_open[k - 1]) return;
+  const long long t = blockIdx.x;
+  const int sg = find_seg(b.toff, b.nseg, t);
+  if (!b.seg_need[sg]) return;
+  const SegInfo g = b.info[sg];
+  const long long q0 = (t - g.first_tile) * TILE;
+  const bool last = t == seg_end_tile(b, sg) - 1;
+  const long long q1 = last ? g.r : min(q0 + TILE, g.r);
+  int* X = jmp + g.lo;
+  const int r = (int)g.r;
+  int v[PER_THREAD];
+  for (int e = 0; e < PER_THREAD; e++) {
+    const long long q = q0 + threadIdx.x + (long long)e * 256;
+    v[e] = q < q1 ? X[q] : r;
+  }
+  int w[PER_THREAD];
+  for (int e = 0; e < PER_THREAD; e++) w[e] = v[e] < r ? X[v[e]] : v[e];
+  bool open = false;
+  for (int e = 0; e < PER_THREAD; e++) {
+    const long long q = q0 + threadIdx.x + (long long)e * 256;
+    if (v[e] < r) {
+      X[q] = w[e];
+      open |= w[e] < r;
+    }
+  }
+  if (__syncthreads_or(open) && threadIdx.x == 0) round_open[k] = 1;
+}
+
+// Elements whose capped chase did not finish: follow the converged
+// pointers to their slot.
 template <typename K>
-__global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* valsB,
-                              const long long* tile_off, int nseg, const SegInfo* info,
-                              const int* seg_need, const int* need_any, const int* nxt,
-                              const int* s0, int* err) {
-  if (!*need_any) return;
-  long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) return;
-  int sg = find_seg(tile_off, nseg, t);
-  if (!seg_need[sg]) return;
-  SegInfo g = info[sg];
+__global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* valsB, K* keysF,
+                              int* valsF, LevelBufs b, const int* jmp, const int* s0) {
+  const long long t = blockIdx.x;
+  const int sg = find_seg(b.toff, b.nseg, t);
+  if (!b.seg_need[sg]) return;
+  const SegInfo g = b.info[sg];
   const K pivot = keysA[g.hi];
-  long long start = g.lo + (t - g.first_tile) * TILE;
+  const bool single_right = (g.hi - g.lo) - g.c == 1;
+  const long long start = g.lo + (t - g.first_tile) * TILE;
   for (int q = 0; q < PER_THREAD; q++) {
-    long long j = start + (long long)q * 256 + threadIdx.x;
+    const long long j = start + (long long)q * 256 + threadIdx.x;
     if (j >= g.hi) continue;
     if (keysA[j] <= pivot) continue;
     long long s = s0[j];
     if (s >= g.r) continue;  // placed by the capped chase
-    for (int guard = 0; s < g.r && guard < 64; guard++) s = nxt[g.lo + s];
+    for (int guard = 0; s < g.r && guard < 64; guard++) s = jmp[g.lo + s];
     if (s < g.r) {  // doubling rounds too few: report, write nothing
-      *err = 1;
+      atomicExch(&b.st->err, 1ull);
       continue;
     }
-    long long dst = g.lo + g.c + 1 + (s - g.r);
+    const long long dst = g.lo + g.c + 1 + (s - g.r);
     keysB[dst] = keysA[j];
     valsB[dst] = valsA[j];
-  }
-}
-
-template <typename K>
-__global__ void k_tile_copyback(K* keysA, int* valsA, const K* keysB, const int* valsB,
-                                const long long* tile_off, int nseg, const SegInfo* info) {
-  long long t = blockIdx.x;
-  if (t >= tile_off[nseg]) return;
-  int sg = find_seg(tile_off, nseg, t);
-  SegInfo g = info[sg];
-  long long start = g.lo + (t - g.first_tile) * TILE;
-  for (int q = 0; q < PER_THREAD; q++) {
-    long long j = start + (long long)q * 256 + threadIdx.x;
-    if (j > g.hi) continue;
-    if (j == g.hi && t != tile_off[sg + 1] - 1) continue;  // hi copied by the last tile
-    keysA[j] = keysB[j];
-    valsA[j] = valsB[j];
-  }
-  // the last tile also covers hi when the tile ends before it
-  if (t == tile_off[sg + 1] - 1 && threadIdx.x == 0) {
-    long long tile_end = start + TILE;
-    if (tile_end <= g.hi) {
-      keysA[g.hi] = keysB[g.hi];
-      valsA[g.hi] = valsB[g.hi];
+    if (single_right) {
+      keysF[dst] = keysA[j];
+      valsF[dst] = valsA[j];
     }
   }
 }
@@ -538,9 +555,9 @@ __device__ inline void lane_quicksort(K* sk, int* sv, int lo, int hi) {
 static constexpr int SMALL_WAVES = 4;
 
 template <typename K, int SMALL>
-__global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(K* keys, int* vals,
-                                                                 const long long* seg_lo,
-                                                                 const long long* seg_hi) {
+__global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
+    K* keys, int* vals, const K* keys1, const int* vals1, const K* keys2, const int* vals2,
+    const long long* seg_lo, const long long* seg_hi, const int* seg_buf) {
   static_assert(SMALL <= 32767, "LDS indices are 16-bit");
   constexpr int NT = 64 * SMALL_WAVES;
   __shared__ K sk[SMALL], tk[SMALL];
@@ -554,9 +571,12 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(K* keys, int* v
   const unsigned long long below = lanemask_lt();
   const long long base = seg_lo[blockIdx.x];
   const int n = (int)(seg_hi[blockIdx.x] - base + 1);
+  const int buf = seg_buf[blockIdx.x];
+  const K* srck = buf == 0 ? keys : (buf == 1 ? keys1 : keys2);
+  const int* srcv = buf == 0 ? vals : (buf == 1 ? vals1 : vals2);
   for (int i = tid; i < n; i += NT) {
-    sk[i] = keys[base + i];
-    sv[i] = vals[base + i];
+    sk[i] = srck[base + i];
+    sv[i] = srcv[base + i];
   }
   if (tid == 0) {
     ntiny = 0;
@@ -716,146 +736,142 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
     bytes += (b + 255) & ~(size_t)255;
     return o;
   };
-  size_t o_klo = take(maxseg * 8), o_khi = take(maxseg * 8), o_nlo = take(maxseg * 8),
-         o_nhi = take(maxseg * 8), o_ntiles = take((maxseg + 1) * 8),
-         o_toff = take((maxseg + 2) * 8), o_j0 = take(maxseg * 8),
-         o_tle = take((maxtiles + 1) * 8), o_tpre = take((maxtiles + 2) * 8),
-         o_info = take(maxseg * sizeof(SegInfo)), o_slo = take(maxsmall * 8),
-         o_shi = take(maxsmall * 8), o_cnt = take(64 * 4), o_arot = take((size_t)n * 4),
-         o_s0 = take((size_t)n * 4), o_need = take(maxseg * 4 + 64), o_nxa = take((size_t)n * 4 + 64);
+  size_t o_lo[2], o_hi[2], o_toff[2], o_j0[2];
+  for (int q = 0; q < 2; q++) {
+    o_lo[q] = take(maxseg * 8);
+    o_hi[q] = take(maxseg * 8);
+    o_toff[q] = take((maxseg + 1) * 8);
+    o_j0[q] = take(maxseg * 8);
+  }
+  const size_t o_tle = take((maxtiles + 1) * 8), o_tpre = take((maxtiles + 2) * 8),
+               o_info = take(maxseg * sizeof(SegInfo)), o_slo = take(maxsmall * 8),
+               o_shi = take(maxsmall * 8), o_sbuf = take(maxsmall * 4),
+               o_st = take(sizeof(LevelState)), o_round = take(64 * 4),
+               o_arot = take((size_t)n * 4), o_s0 = take((size_t)n * 4),
+               o_need = take(maxseg * 4 + 64), o_jmp = take((size_t)n * 4 + 64);
   char* aux = (char*)ctx_buf(c, BUF_SORT_AUX, bytes);
+  // work buffers: 1 = B, 2 = C; the output (0) is `keys`/`vals`
   K* kB = (K*)ctx_buf(c, BUF_SORT_K, (size_t)n * sizeof(K));
   int* vB = (int*)ctx_buf(c, BUF_SORT_V, (size_t)n * sizeof(int));
-  long long* cur_lo = (long long*)(aux + o_klo);
-  long long* cur_hi = (long long*)(aux + o_khi);
-  long long* nxt_lo = (long long*)(aux + o_nlo);
-  long long* nxt_hi = (long long*)(aux + o_nhi);
-  long long* ntiles = (long long*)(aux + o_ntiles);
-  long long* toff = (long long*)(aux + o_toff);
-  unsigned long long* j0 = (unsigned long long*)(aux + o_j0);
+  K* kC = (K*)ctx_buf(c, BUF_SORT_K2, (size_t)n * sizeof(K));
+  int* vC = (int*)ctx_buf(c, BUF_SORT_V2, (size_t)n * sizeof(int));
   long long* tle = (long long*)(aux + o_tle);
   long long* tpre = (long long*)(aux + o_tpre);
-  SegInfo* info = (SegInfo*)(aux + o_info);
-  long long* slo = (long long*)(aux + o_slo);
-  long long* shi = (long long*)(aux + o_shi);
-  int* cnt = (int*)(aux + o_cnt);  // [0] next count, [1] small count
+  LevelState* st = (LevelState*)(aux + o_st);
+  int* round_open = (int*)(aux + o_round);
   int* arot = (int*)(aux + o_arot);
   int* s0 = (int*)(aux + o_s0);
-  // level state read back once per level: [0] large children, [1] small
-  // segments so far, [2] some segment needs doubling, [4..5] most steps an
-  // unfinished chain still needs, [6..7] elements in large children; [8] doubling error (kept
-  // over all levels); [9] chains left open by the asynchronous doubling;
-  // [16..63] per-round flags of the doubling
-  int* hc = (int*)ctx_pinned(c, 64);
-  int* seg_need = (int*)(aux + o_need);
-  int* nxt = (int*)(aux + o_nxa);
-  int* round_open = cnt + 16;
-  unsigned long long* rmax = (unsigned long long*)(cnt + 4);
-  unsigned long long* next_elems = (unsigned long long*)(cnt + 6);
+  int* jmp = (int*)(aux + o_jmp);
+  LevelState* hs = (LevelState*)ctx_pinned(c, sizeof(LevelState));
 
-  int nsmall_host = 0;
-  int nseg = 0;
-  long long level_elems = 0;
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 64, 0);
-  {
+  LevelBufs b{};
+  b.slo = (long long*)(aux + o_slo);
+  b.shi = (long long*)(aux + o_shi);
+  b.sbuf = (int*)(aux + o_sbuf);
+  b.small_max = SMALL;
+  b.info = (SegInfo*)(aux + o_info);
+  b.seg_need = (int*)(aux + o_need);
+  b.st = st;
+  int cur = 0;
+  long long nsmall_host = 0;
+  if (n <= SMALL) {
     long long init[2] = {0, n - 1};
-    long long* dlo = n > SMALL ? cur_lo : slo;
-    long long* dhi = n > SMALL ? cur_hi : shi;
-    EXH_CHECK_HIP(hipMemcpyAsync(dlo, &init[0], 8, hipMemcpyHostToDevice, c->stream));
-    EXH_CHECK_HIP(hipMemcpyAsync(dhi, &init[1], 8, hipMemcpyHostToDevice, c->stream));
+    int zero = 0;
+    EXH_CHECK_HIP(hipMemcpyAsync(b.slo, &init[0], 8, hipMemcpyHostToDevice, c->stream));
+    EXH_CHECK_HIP(hipMemcpyAsync(b.shi, &init[1], 8, hipMemcpyHostToDevice, c->stream));
+    EXH_CHECK_HIP(hipMemcpyAsync(b.sbuf, &zero, 4, hipMemcpyHostToDevice, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    if (n > SMALL) {
-      nseg = 1;
-      level_elems = n;
-    } else {
-      nsmall_host = 1;
-    }
+    nsmall_host = 1;
+  } else {
+    // level 0 reads a copy, so pivots can be written to the output at once
+    EXH_CHECK_HIP(hipMemcpyAsync(kC, keys, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    EXH_CHECK_HIP(hipMemcpyAsync(vC, vals, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, c->stream,
+                       (long long*)(aux + o_lo[0]), (long long*)(aux + o_hi[0]),
+                       (long long*)(aux + o_toff[0]), (unsigned long long*)(aux + o_j0[0]), st, n);
+    EXH_LAUNCH_CHECK();
+    b.nseg = 1;
+    b.ntiles = (n - 1 + TILE - 1) / TILE;
   }
-  bool pending_err = false;
-  while (nseg > 0) {
+  // src/dst work buffers of the current level (ids 2 and 1 at level 0)
+  int src_id = 2;
+  while (b.nseg > 0) {
     c->stat_sort_levels++;
-    // tiles of this level: at most ceil(elements / TILE) + segments; the
-    // tile kernels' blocks beyond the exact count return at once
-    const long long ntile_ub = (level_elems + TILE - 1) / TILE + nseg;
-    EXH_REQUIRE(ntile_ub <= maxtiles, EXH_ERR_INTERNAL, "peak sort: tile bound exceeded");
-    hipLaunchKernelGGL(k_seg_ntiles, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, cur_lo,
-                       cur_hi, nseg, ntiles, (long long*)j0);
+    const int dst_id = 3 - src_id;
+    K* ks = src_id == 1 ? kB : kC;
+    int* vs = src_id == 1 ? vB : vC;
+    K* kd = dst_id == 1 ? kB : kC;
+    int* vd = dst_id == 1 ? vB : vC;
+    b.lo = (const long long*)(aux + o_lo[cur]);
+    b.hi = (const long long*)(aux + o_hi[cur]);
+    b.toff = (const long long*)(aux + o_toff[cur]);
+    b.j0 = (unsigned long long*)(aux + o_j0[cur]);
+    b.nlo = (long long*)(aux + o_lo[1 - cur]);
+    b.nhi = (long long*)(aux + o_hi[1 - cur]);
+    b.ntoff = (long long*)(aux + o_toff[1 - cur]);
+    b.nj0 = (unsigned long long*)(aux + o_j0[1 - cur]);
+    b.dst_id = dst_id;
+    EXH_REQUIRE(b.ntiles <= maxtiles && b.ntiles < (1ll << 31), EXH_ERR_INTERNAL,
+                "peak sort: tile count exceeded");
+    const unsigned nt = (unsigned)b.ntiles;
+    hipLaunchKernelGGL(k_tile_count<K>, dim3(nt), dim3(256), 0, c->stream, ks, b, tle);
     EXH_LAUNCH_CHECK();
-    exclusive_scan<long long>(c, ntiles, toff, nseg);
-    hipLaunchKernelGGL(k_tile_count<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, keys,
-                       cur_lo, cur_hi, toff, nseg, tle, j0);
+    exclusive_scan<long long>(c, tle, tpre, b.ntiles);
+    hipLaunchKernelGGL(k_tile_scatter<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd,
+                       keys, vals, b, tpre, arot, s0);
     EXH_LAUNCH_CHECK();
-    exclusive_scan<long long>(c, tle, tpre, ntile_ub);
-    // reset the per-level counters (not cnt[1]: the small segments collect
-    // over all levels) and the round flags
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt, 1, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt + 2, 6, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, round_open, 48, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, seg_need,
-                       nseg, 0);
-    hipLaunchKernelGGL(k_seg_final<K>, dim3(grid_for(nseg, 256)), dim3(256), 0, c->stream, keys,
-                       vals, kB, vB, cur_lo, cur_hi, toff, tpre, j0, nseg, info, arot,
-                       (long long)SMALL, nxt_lo, nxt_hi, cnt, slo, shi, cnt + 1, next_elems);
+    hipLaunchKernelGGL(k_tile_chase<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd, keys,
+                       vals, b, arot, s0, jmp, chase_cap());
     EXH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tile_scatter<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
-                       keys, vals, kB, vB, toff, tpre, nseg, info, arot, s0);
-    EXH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tile_chase<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, keys,
-                       vals, kB, vB, toff, nseg, info, arot, s0, seg_need, cnt + 2, rmax, chase_cap());
-    EXH_LAUNCH_CHECK();
-    EXH_CHECK_HIP(hipMemcpyAsync(hc, cnt, 32, hipMemcpyDeviceToHost, c->stream));
+    EXH_CHECK_HIP(hipMemcpyAsync(hs, st, sizeof(LevelState), hipMemcpyDeviceToHost, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    if (hc[2]) {
-      // long chains: finish them by pointer doubling; after k rounds an
-      // entry covers 2^k steps, and no chain needs more than rm more steps
+    if (hs->need_any) {
+      // long chains: finish them by pointer doubling (asynchronous; the
+      // synchronous rounds, bounded by the steps left, only if needed)
       c->stat_sort_doubling++;
-      const unsigned long long rm = *(unsigned long long*)(hc + 4);
-      int rounds = 1;
-      while ((1ull << (rounds - 1)) < rm && rounds < 62) rounds++;
-      EXH_REQUIRE(rounds <= 40, EXH_ERR_INTERNAL, "peak sort: segment too long for doubling");
-      hipLaunchKernelGGL(k_double_init, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, toff,
-                         nseg, info, seg_need, cnt + 2, arot, nxt);
-      hipLaunchKernelGGL(k_double_async, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream, toff,
-                         nseg, info, seg_need, cnt + 2, nxt, 64, cnt + 9);
+      hipLaunchKernelGGL(k_double_async, dim3(nt), dim3(256), 0, c->stream, b, jmp, 64);
       EXH_LAUNCH_CHECK();
-      EXH_CHECK_HIP(hipMemcpyAsync(hc + 9, cnt + 9, 4, hipMemcpyDeviceToHost, c->stream));
+      EXH_CHECK_HIP(hipMemcpyAsync(&hs->async_open, &st->async_open, 8, hipMemcpyDeviceToHost,
+                                   c->stream));
       EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-      if (hc[9]) {
-        // some chain is still open: the synchronous rounds finish every
-        // chain within the bound
+      if (hs->async_open) {
         c->stat_sort_async_open++;
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, cnt + 9, 1, 0);
-        for (int k = 0; k < rounds; k++) {
-          hipLaunchKernelGGL(k_double_round, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
-                             toff, nseg, info, seg_need, cnt + 2, nxt, round_open, k);
-          EXH_LAUNCH_CHECK();
-        }
+        const unsigned long long rm = hs->rmax;
+        int rounds = 1;
+        while ((1ull << (rounds - 1)) < rm && rounds < 62) rounds++;
+        EXH_REQUIRE(rounds <= 40, EXH_ERR_INTERNAL, "peak sort: segment too long for doubling");
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, round_open, 64, 0);
+        for (int k = 0; k < rounds; k++)
+          hipLaunchKernelGGL(k_double_round, dim3(nt), dim3(256), 0, c->stream, b, jmp,
+                             round_open, k);
+        EXH_LAUNCH_CHECK();
       }
-      hipLaunchKernelGGL(k_tile_chase2<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
-                         keys, vals, kB, vB, toff, nseg, info, seg_need, cnt + 2, nxt, s0, cnt + 8);
+      hipLaunchKernelGGL(k_tile_chase2<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd,
+                         keys, vals, b, jmp, s0);
       EXH_LAUNCH_CHECK();
-      pending_err = true;
     }
-    hipLaunchKernelGGL(k_tile_copyback<K>, dim3((unsigned)ntile_ub), dim3(256), 0, c->stream,
-                       keys, vals, kB, vB, toff, nseg, info);
-    EXH_LAUNCH_CHECK();
-    nseg = hc[0];
-    nsmall_host = hc[1];
-    level_elems = *(long long*)(hc + 6);
-    EXH_REQUIRE(nseg <= maxseg && nsmall_host <= maxsmall, EXH_ERR_INTERNAL,
-                "peak sort: segment lists overflow (%d, %d)", nseg, nsmall_host);
-    std::swap(cur_lo, nxt_lo);
-    std::swap(cur_hi, nxt_hi);
-  }
-  if (pending_err) {
-    EXH_CHECK_HIP(hipMemcpyAsync(hc + 8, cnt + 8, 4, hipMemcpyDeviceToHost, c->stream));
-    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    EXH_REQUIRE(!hc[8], EXH_ERR_INTERNAL, "peak sort: pointer doubling did not converge");
+    const int nseg_next = (int)(hs->pk >> 32);
+    const long long ntiles_next = (long long)(hs->pk & 0xffffffffull);
+    nsmall_host = (long long)hs->nsmall;
+    EXH_REQUIRE(nseg_next <= maxseg && nsmall_host <= maxsmall, EXH_ERR_INTERNAL,
+                "peak sort: segment lists overflow (%d, %lld)", nseg_next, nsmall_host);
+    b.nseg = nseg_next;
+    b.ntiles = ntiles_next;
+    cur = 1 - cur;
+    src_id = dst_id;
   }
   if (nsmall_host > 0) {
-    hipLaunchKernelGGL((k_small_sort<K, SMALL>), dim3((unsigned)nsmall_host), dim3(64 * SMALL_WAVES), 0,
-                       c->stream, keys, vals, slo, shi);
+    hipLaunchKernelGGL((k_small_sort<K, SMALL>), dim3((unsigned)nsmall_host),
+                       dim3(64 * SMALL_WAVES), 0, c->stream, keys, vals, kB, vB, kC, vC, b.slo,
+                       b.shi, b.sbuf);
     EXH_LAUNCH_CHECK();
+  }
+  if (c->stat_sort_doubling > 0) {
+    EXH_CHECK_HIP(hipMemcpyAsync(&hs->err, &st->err, 8, hipMemcpyDeviceToHost, c->stream));
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    EXH_REQUIRE(!hs->err, EXH_ERR_INTERNAL, "peak sort: pointer doubling did not converge");
   }
 }
 
