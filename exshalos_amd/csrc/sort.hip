@@ -488,7 +488,8 @@ __global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* v
 // ---------------------------------------------------------------------------
 // small segments: one wavefront each, LDS resident
 // ---------------------------------------------------------------------------
-static constexpr int TINY = 64;
+static constexpr int TINY = 32;      // sorted one per lane
+static constexpr int WAVE_MAX = 256; // partitioned by one wave
 
 // Scalar Lomuto quicksort of sk/sv[lo..hi] (LDS) by one lane: the
 // reference's partition verbatim.  The smaller side is sorted next and the
@@ -545,13 +546,84 @@ __device__ inline void lane_quicksort(K* sk, int* sv, int lo, int hi) {
   }
 }
 
+// Ordering of LDS accesses between the lanes of one wavefront.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Lomuto partition of sk/sv[lo..hi] (LDS) by one wavefront with the
+// closed-form rule of the file comment; temporaries tk/tv and the rotation
+// table at arot[lo + q].  Returns c, the number of "<=" elements.
+template <typename K>
+__device__ inline int wave_partition(K* sk, int* sv, K* tk, int* tv, short* arot, int lo, int hi,
+                                     int lane, unsigned long long below) {
+  const K pivot = sk[hi];
+  int c = 0, j0 = hi;
+  for (int b = lo; b < hi; b += 64) {
+    const int j = b + lane;
+    const bool in = j < hi;
+    const bool le = in && sk[j] <= pivot;
+    const unsigned long long mle = __ballot(le), mgt = __ballot(in && !le);
+    if (j0 == hi && mgt) j0 = b + __ffsll((long long)mgt) - 1;
+    c += __popcll(mle);
+  }
+  const int m = (hi - lo) - c;
+  const int r = m > 0 ? (c - (j0 - lo)) + 1 : 0;
+  const int jlead = j0 - lo;
+  int run = 0;
+  for (int b = lo; b < hi; b += 64) {
+    const int j = b + lane;
+    const bool le = j < hi && sk[j] <= pivot;
+    const unsigned long long mle = __ballot(le);
+    if (le) {
+      const int rank = run + __popcll(mle & below);
+      tk[lo + rank] = sk[j];
+      tv[lo + rank] = sv[j];
+      if (r > 0 && j > j0) arot[lo + rank - jlead] = (short)((j - lo) - rank);
+    }
+    run += __popcll(mle);
+  }
+  if (lane == 0) {
+    tk[lo + c] = sk[hi];
+    tv[lo + c] = sv[hi];
+    if (m > 0) arot[lo + r - 1] = (short)m;
+  }
+  wave_sync();
+  run = 0;
+  for (int b = lo; b < hi; b += 64) {
+    const int j = b + lane;
+    const bool in = j < hi;
+    const bool le = in && sk[j] <= pivot;
+    const unsigned long long mle = __ballot(le);
+    if (in && !le) {
+      const int rank = run + __popcll(mle & below);
+      int s = ((j - lo) - rank) + (rank - jlead);
+      for (int guard = 0; s < r && guard <= r; guard++) s += max((int)arot[lo + s], 1);
+      tk[lo + c + 1 + (s - r)] = sk[j];
+      tv[lo + c + 1 + (s - r)] = sv[j];
+    }
+    run += __popcll(mle);
+  }
+  wave_sync();
+  for (int i = lo + lane; i <= hi; i += 64) {
+    sk[i] = tk[i];
+    sv[i] = tv[i];
+  }
+  wave_sync();
+  return c;
+}
+
 // One workgroup of SMALL_WAVES wavefronts per segment of 2..SMALL
-// elements, resident in LDS.  Sub-segments longer than TINY are partitioned
-// by the whole workgroup with the closed-form Lomuto rule (see the file
-// comment): each wave takes a contiguous chunk, ranks come from ballots and
-// the chunk offsets from LDS.  Shorter sub-segments are queued and then
-// sorted one per lane by lane_quicksort.  Several waves share one segment's
-// LDS so that the LDS per CU, not per wave, bounds the occupancy.
+// elements, resident in LDS.  Sub-segments longer than WAVE_MAX are
+// partitioned by the whole workgroup (each wave takes a contiguous chunk;
+// ranks from ballots, chunk offsets from LDS); those of TINY+1..WAVE_MAX
+// go to a queue from which each wave takes sub-segments and partitions
+// them alone (no workgroup barriers); those of 2..TINY are sorted one per
+// lane by lane_quicksort.  All use the closed-form Lomuto rule of the file
+// comment, with the rotation table indexed from the sub-segment start, so
+// that waves working on disjoint sub-segments never share LDS.
 static constexpr int SMALL_WAVES = 4;
 
 template <typename K, int SMALL>
@@ -560,13 +632,16 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
     const long long* seg_lo, const long long* seg_hi, const int* seg_buf) {
   static_assert(SMALL <= 32767, "LDS indices are 16-bit");
   constexpr int NT = 64 * SMALL_WAVES;
+  constexpr int WQ = SMALL / (TINY + 1) + 2;
   __shared__ K sk[SMALL], tk[SMALL];
   __shared__ int sv[SMALL], tv[SMALL];
   __shared__ short arot[SMALL];
   __shared__ short tl_lo[SMALL / 2 + 1], tl_hi[SMALL / 2 + 1];
+  __shared__ short wq_lo[WQ], wq_hi[WQ];
   __shared__ short st_lo[64], st_hi[64];
+  __shared__ short ws_lo[SMALL_WAVES][16], ws_hi[SMALL_WAVES][16];
   __shared__ int wc[SMALL_WAVES], wj[SMALL_WAVES];
-  __shared__ int ntiny, top_sh;
+  __shared__ int ntiny, nwq, wq_next, top_sh;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const unsigned long long below = lanemask_lt();
   const long long base = seg_lo[blockIdx.x];
@@ -580,11 +655,17 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
   }
   if (tid == 0) {
     ntiny = 0;
+    nwq = 0;
+    wq_next = 0;
     top_sh = 0;
-    if (n > TINY) {
+    if (n > WAVE_MAX) {
       st_lo[0] = 0;
       st_hi[0] = (short)(n - 1);
       top_sh = 1;
+    } else if (n > TINY) {
+      wq_lo[0] = 0;
+      wq_hi[0] = (short)(n - 1);
+      nwq = 1;
     } else {
       tl_lo[0] = 0;
       tl_hi[0] = (short)(n - 1);
@@ -592,6 +673,7 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
     }
   }
   __syncthreads();
+  // ---- workgroup phase: sub-segments longer than WAVE_MAX
   for (;;) {
     const int top = top_sh;
     if (top == 0) break;
@@ -600,7 +682,6 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
     const int L = hi - lo;
     const int Q = ((L + SMALL_WAVES - 1) / SMALL_WAVES + 63) & ~63;
     const int cb = lo + w * Q, ce = min(cb + Q, hi);
-    // pass 1: per-chunk count of "<=" and first ">"
     int cnt = 0, j0w = hi;
     for (int b = cb; b < ce; b += 64) {
       const int j = b + lane;
@@ -624,7 +705,6 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
     const int m = L - c;
     const int r = m > 0 ? (c - (j0 - lo)) + 1 : 0;
     const int jlead = j0 - lo;
-    // pass 2: place "<=" elements, record the rotation table
     int run = wbase;
     for (int b = cb; b < ce; b += 64) {
       const int j = b + lane;
@@ -634,17 +714,16 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
         const int rank = run + __popcll(mle & below);
         tk[lo + rank] = sk[j];
         tv[lo + rank] = sv[j];
-        if (r > 0 && j > j0) arot[rank - jlead] = (short)((j - lo) - rank);
+        if (r > 0 && j > j0) arot[lo + rank - jlead] = (short)((j - lo) - rank);
       }
       run += __popcll(mle);
     }
     if (tid == 0) {
       tk[lo + c] = sk[hi];
       tv[lo + c] = sv[hi];
-      if (m > 0) arot[r - 1] = (short)m;
+      if (m > 0) arot[lo + r - 1] = (short)m;
     }
     __syncthreads();
-    // pass 3: chase the queue from each ">" element's first stream index
     run = wbase;
     for (int b = cb; b < ce; b += 64) {
       const int j = b + lane;
@@ -654,7 +733,7 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
       if (in && !le) {
         const int rank = run + __popcll(mle & below);
         int s = ((j - lo) - rank) + (rank - jlead);
-        for (int guard = 0; s < r && guard <= r; guard++) s += max((int)arot[s], 1);
+        for (int guard = 0; s < r && guard <= r; guard++) s += max((int)arot[lo + s], 1);
         tk[lo + c + 1 + (s - r)] = sk[j];
         tv[lo + c + 1 + (s - r)] = sv[j];
       }
@@ -666,7 +745,8 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
       sv[i] = tv[i];
     }
     // children: long ones on the stack (larger first, so the smaller is
-    // partitioned next), short ones to the per-lane list
+    // partitioned next), medium ones to the wave queue, short ones to the
+    // per-lane list
     if (tid == 0) {
       int al = lo, ah = lo + c - 1, bl = lo + c + 1, bh = hi;
       if (ah - al > bh - bl) {
@@ -677,28 +757,71 @@ __global__ __launch_bounds__(64 * SMALL_WAVES) void k_small_sort(
         bh = y;
       }
       int tt = top - 1;
-      if (bh - bl + 1 > TINY) {
-        st_lo[tt] = (short)bl;
-        st_hi[tt] = (short)bh;
-        tt++;
-      } else if (bh > bl) {
-        tl_lo[ntiny] = (short)bl;
-        tl_hi[ntiny] = (short)bh;
-        ntiny++;
-      }
-      if (ah - al + 1 > TINY) {
-        st_lo[tt] = (short)al;
-        st_hi[tt] = (short)ah;
-        tt++;
-      } else if (ah > al) {
-        tl_lo[ntiny] = (short)al;
-        tl_hi[ntiny] = (short)ah;
-        ntiny++;
+      const int cl[2] = {bl, al}, ch[2] = {bh, ah};
+      for (int q = 0; q < 2; q++) {
+        const int len = ch[q] - cl[q] + 1;
+        if (len > WAVE_MAX) {
+          st_lo[tt] = (short)cl[q];
+          st_hi[tt] = (short)ch[q];
+          tt++;
+        } else if (len > TINY) {
+          wq_lo[nwq] = (short)cl[q];
+          wq_hi[nwq] = (short)ch[q];
+          nwq++;
+        } else if (len >= 2) {
+          tl_lo[ntiny] = (short)cl[q];
+          tl_hi[ntiny] = (short)ch[q];
+          ntiny++;
+        }
       }
       top_sh = tt;
     }
     __syncthreads();
   }
+  // ---- wave phase: sub-segments of TINY+1..WAVE_MAX, one wave each
+  for (;;) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(&wq_next, 1);
+    idx = __shfl(idx, 0, 64);
+    if (idx >= nwq) break;
+    int top = 1;
+    if (lane == 0) {
+      ws_lo[w][0] = wq_lo[idx];
+      ws_hi[w][0] = wq_hi[idx];
+    }
+    wave_sync();
+    while (top > 0) {
+      top--;
+      const int lo = ws_lo[w][top], hi = ws_hi[w][top];
+      wave_sync();
+      const int c = wave_partition<K>(sk, sv, tk, tv, arot, lo, hi, lane, below);
+      int al = lo, ah = lo + c - 1, bl = lo + c + 1, bh = hi;
+      if (ah - al > bh - bl) {
+        int x = al, y = ah;
+        al = bl;
+        ah = bh;
+        bl = x;
+        bh = y;
+      }
+      const int cl[2] = {bl, al}, ch[2] = {bh, ah};
+      for (int q = 0; q < 2; q++) {
+        const int len = ch[q] - cl[q] + 1;
+        if (len > TINY) {
+          if (lane == 0) {
+            ws_lo[w][top] = (short)cl[q];
+            ws_hi[w][top] = (short)ch[q];
+          }
+          top++;
+        } else if (len >= 2 && lane == 0) {
+          const int p = atomicAdd(&ntiny, 1);
+          tl_lo[p] = (short)cl[q];
+          tl_hi[p] = (short)ch[q];
+        }
+      }
+      wave_sync();
+    }
+  }
+  __syncthreads();
   for (int q = tid; q < ntiny; q += NT) lane_quicksort<K>(sk, sv, tl_lo[q], tl_hi[q]);
   __syncthreads();
   for (int i = tid; i < n; i += NT) {
