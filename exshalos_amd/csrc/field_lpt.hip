@@ -6,13 +6,14 @@
 //                  noise so that every mode is independent
 //  k_field         delta_k * scale for the c2r that gives delta(x)
 //  k_poisson_*     phi_k = delta_k / (-k^2) (lpt.c:12-44, 141-173)
-//  fd_grad1        S1 = -D6(phi) / (60 Lc) (lpt.c:56-93)
-//  lpt2_source     sum_{a<b} (S_aa S_bb - S_ab^2) (lpt.c:116-134)
-//  fd_grad2        S2 = -3/7 Omz^(-1/143) D6(phi2) / (60 Lc), S = S1 + S2,
-//                  velocities (lpt.c:182-223)
+//  k_grad_tiles    S1 = -D6(phi) / (60 Lc) (lpt.c:56-93); S2 = -3/7
+//                  Omz^(-1/143) D6(phi2) / (60 Lc) added to S, velocities
+//                  (lpt.c:182-223)
+//  k_source_tiles  sum_{a<b} (S_aa S_bb - S_ab^2) (lpt.c:116-134)
 //  to_positions    q + S with one periodic wrap (lpt.c:229-242)
 //
-// All kernels are HBM-bound streaming kernels: one thread per mode/cell.
+// Mode kernels stream over the half-complex grid; the stencil kernels
+// march planes through LDS.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -23,33 +24,51 @@ struct FieldArgs {
   int n0, n1, n2, nz2;
   T kl0, kl1, kl2;
   T k_smooth;
-  const double* sx;  // spline knots, values, second derivatives (device)
+  const double* sx;  // spline knots, values and per-interval coefficients (device)
   const double* sy;
-  const double* sm;
+  const double* sb;
+  const double* sc;
+  const double* sd;
   int nk;
+  int stage;  // copy the spline into LDS (5 nk doubles)
   const T* noise;
   int fixed;
   T phase;
 };
 
-__device__ inline double spline_dev(const double* x, const double* y, const double* m, int n,
-                                    double xv) {
-  if (xv < x[0]) xv = x[0];
-  if (xv > x[n - 1]) xv = x[n - 1];
-  int lo = 0, hi = n - 1;
+// Natural cubic spline evaluated from per-interval coefficients
+// b, c, d computed on the host with the same expressions as the oracle's
+// spline_eval (y + t (b + t (c + t d))), so the values are identical.
+struct SplineView {
+  const double *x, *y, *b, *c, *d;
+  int n;
+};
+
+__device__ inline double spline_eval_v(const SplineView& s, double xv) {
+  if (xv < s.x[0]) xv = s.x[0];
+  if (xv > s.x[s.n - 1]) xv = s.x[s.n - 1];
+  int lo = 0, hi = s.n - 1;
   while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (x[mid] > xv)
+    const int mid = (lo + hi) >> 1;
+    if (s.x[mid] > xv)
       hi = mid;
     else
       lo = mid;
   }
-  double h = x[hi] - x[lo];
-  double t = xv - x[lo];
-  double b = (y[hi] - y[lo]) / h - h * (2.0 * m[lo] + m[hi]) / 6.0;
-  double c = m[lo] / 2.0;
-  double d = (m[hi] - m[lo]) / (6.0 * h);
-  return y[lo] + t * (b + t * (c + t * d));
+  const double t = xv - s.x[lo];
+  return s.y[lo] + t * (s.b[lo] + t * (s.c[lo] + t * s.d[lo]));
+}
+
+// The spline tables in LDS (when a.stage) or in global memory.
+template <typename T>
+__device__ inline SplineView stage_spline(const FieldArgs<T>& a, double* lds) {
+  SplineView v{a.sx, a.sy, a.sb, a.sc, a.sd, a.nk};
+  if (!a.stage) return v;
+  const double* src[5] = {a.sx, a.sy, a.sb, a.sc, a.sd};
+  for (int q = 0; q < 5; q++)
+    for (int e = threadIdx.x; e < a.nk; e += blockDim.x) lds[q * a.nk + e] = src[q][e];
+  __syncthreads();
+  return SplineView{lds, lds + a.nk, lds + 2 * a.nk, lds + 3 * a.nk, lds + 4 * a.nk, a.nk};
 }
 
 template <typename T>
@@ -66,7 +85,8 @@ __device__ inline double sqrt_t<double>(double v) {
 // Generated (non-copied) mode at (i, j, k): A(|k|) * noise, with the
 // reference's float/double placement (density_grid.c:152-198).
 template <typename T>
-__device__ inline void generated_mode(const FieldArgs<T>& a, int i, int j, int k, T& re, T& im) {
+__device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j,
+                                      int k, T& re, T& im) {
   T kx = (i < a.n0 / 2) ? (T)i * a.kl0 : (T)(i - a.n0) * a.kl0;
   T ky = (j < a.n1 / 2) ? (T)j * a.kl1 : (T)(j - a.n1) * a.kl1;
   T kz = (T)k * a.kl2;
@@ -74,7 +94,7 @@ __device__ inline void generated_mode(const FieldArgs<T>& a, int i, int j, int k
   T kmod = sqrt_t<T>(kx * kx + ky * ky + kz * kz);
   T A = 0;
   if (kmod <= a.k_smooth && kmod > (T)0)
-    A = (T)sqrt(spline_dev(a.sx, a.sy, a.sm, a.nk, (double)kmod) / 2.0);
+    A = (T)sqrt(spline_eval_v(sp, (double)kmod) / 2.0);
   size_t id = ((size_t)i * a.n1 + j) * (size_t)a.nz2 + k;
   T g0 = a.noise[2 * id], g1 = a.noise[2 * id + 1];
   bool selfconj = (i == 0 || i == a.n0 / 2) && (j == 0 || j == a.n1 / 2) &&
@@ -99,7 +119,8 @@ __device__ inline void generated_mode(const FieldArgs<T>& a, int i, int j, int k
 // Mode (i, j, k) of the field including the Hermitian copy rule of the
 // kz = 0 and kz = nd/2 planes (density_grid.c:109-150).
 template <typename T>
-__device__ inline void field_mode(const FieldArgs<T>& a, int i, int j, int k, T& re, T& im) {
+__device__ inline void field_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j, int k,
+                                  T& re, T& im) {
   int pi = i, pj = j;
   bool cj = false;
   if ((k == 0 || k == a.n2 / 2) && (i > a.n0 / 2 || j > a.n1 / 2)) {
@@ -119,24 +140,28 @@ __device__ inline void field_mode(const FieldArgs<T>& a, int i, int j, int k, T&
       cj = true;
     }
   }
-  generated_mode(a, pi, pj, k, re, im);
+  generated_mode(a, sp, pi, pj, k, re, im);
   if (cj) im = -im;
 }
 
+// Mode kernels walk the (i, j) rows of the half-complex grid (grid-stride
+// over rows, threads along k), so the index arithmetic is per row.
 template <typename T>
-__global__ void k_field(FieldArgs<T> a, T scale, typename cplx<T>::type* out,
-                        typename cplx<T>::type* raw) {
-  size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
-  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
-       id += (size_t)gridDim.x * blockDim.x) {
-    int k = (int)(id % a.nz2);
-    size_t r = id / a.nz2;
-    int j = (int)(r % a.n1);
-    int i = (int)(r / a.n1);
-    T re, im;
-    field_mode(a, i, j, k, re, im);
-    if (raw) raw[id] = {re, im};
-    out[id] = {re * scale, im * scale};
+__global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale,
+                                               typename cplx<T>::type* out,
+                                               typename cplx<T>::type* raw) {
+  extern __shared__ double spl[];
+  const SplineView sp = stage_spline(a, spl);
+  const long long rows = (long long)a.n0 * a.n1;
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int i = (int)(row / a.n1), j = (int)(row % a.n1);
+    for (int k = threadIdx.x; k < a.nz2; k += blockDim.x) {
+      const size_t id = (size_t)row * a.nz2 + k;
+      T re, im;
+      field_mode(a, sp, i, j, k, re, im);
+      if (raw) raw[id] = {re, im};
+      out[id] = {re * scale, im * scale};
+    }
   }
 }
 
@@ -160,38 +185,69 @@ __device__ inline bool lpt_fact(int n0, int n1, int n2, T kl0, T kl1, T kl2, T k
 // phi_k = delta_k / (-k^2) * scale with delta_k taken from the white-noise
 // modes (no stored copy of delta_k is needed).
 template <typename T>
-__global__ void k_poisson_modes(FieldArgs<T> a, T k_smooth, T scale,
-                                typename cplx<T>::type* out) {
-  size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
-  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
-       id += (size_t)gridDim.x * blockDim.x) {
-    int k = (int)(id % a.nz2);
-    size_t r = id / a.nz2;
-    int j = (int)(r % a.n1);
-    int i = (int)(r / a.n1);
-    T fact;
-    if (lpt_fact<T>(a.n0, a.n1, a.n2, a.kl0, a.kl1, a.kl2, k_smooth, i, j, k, fact)) {
-      T re, im;
-      field_mode(a, i, j, k, re, im);
-      out[id] = {re / fact * scale, im / fact * scale};
-    } else {
-      out[id] = {(T)0, (T)0};
+__global__ __launch_bounds__(256) void k_poisson_modes(FieldArgs<T> a, T k_smooth, T scale,
+                                                       typename cplx<T>::type* out) {
+  extern __shared__ double spl[];
+  const SplineView sp = stage_spline(a, spl);
+  const long long rows = (long long)a.n0 * a.n1;
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int i = (int)(row / a.n1), j = (int)(row % a.n1);
+    for (int k = threadIdx.x; k < a.nz2; k += blockDim.x) {
+      const size_t id = (size_t)row * a.nz2 + k;
+      T fact;
+      if (lpt_fact<T>(a.n0, a.n1, a.n2, a.kl0, a.kl1, a.kl2, k_smooth, i, j, k, fact)) {
+        T re, im;
+        field_mode(a, sp, i, j, k, re, im);
+        out[id] = {re / fact * scale, im / fact * scale};
+      } else {
+        out[id] = {(T)0, (T)0};
+      }
     }
   }
 }
 
-// In-place phi_k = c_k * pre / (-k^2) * scale for a given half-complex grid.
+// Division by a fixed divisor 1 <= d < 2^31 through a 64-bit reciprocal
+// m = ceil(2^64 / d): for 0 <= x < 2^31 the high 64 bits of x m are
+// exactly floor(x / d) (the error x (m - 2^64/d) / 2^64 < 2^-33 cannot
+// carry a fraction of at most (d-1)/d across an integer).
+struct FastDiv {
+  unsigned d;
+  unsigned long long m;
+};
+
+static FastDiv fast_div(unsigned d) {
+  FastDiv f;
+  f.d = d;
+  f.m = d > 1 ? ~0ull / d + 1 : 0;
+  return f;
+}
+
+__device__ inline unsigned fdiv(unsigned x, const FastDiv& f) {
+  return f.d == 1 ? x : (unsigned)__umul64hi((unsigned long long)x, f.m);
+}
+
+// In-place phi_k = c_k * pre / (-k^2) * scale for a given half-complex
+// grid (flat index over modes; 32-bit index arithmetic below 2^31 modes).
 template <typename T>
-__global__ void k_poisson_grid(int n0, int n1, int n2, T kl0, T kl1, T kl2, T k_smooth, T pre,
-                               T scale, typename cplx<T>::type* ck) {
-  int nz2 = n2 / 2 + 1;
-  size_t nh = (size_t)n0 * n1 * nz2;
+__global__ __launch_bounds__(256) void k_poisson_grid(int n0, int n1, int n2, T kl0, T kl1, T kl2,
+                                                      T k_smooth, T pre, T scale, FastDiv dz,
+                                                      FastDiv dy, typename cplx<T>::type* ck) {
+  const int nz2 = n2 / 2 + 1;
+  const size_t nh = (size_t)n0 * n1 * nz2;
   for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
        id += (size_t)gridDim.x * blockDim.x) {
-    int k = (int)(id % nz2);
-    size_t r = id / nz2;
-    int j = (int)(r % n1);
-    int i = (int)(r / n1);
+    int i, j, k;
+    if (nh < (1ull << 31)) {
+      const unsigned r = fdiv((unsigned)id, dz);
+      k = (int)((unsigned)id - r * (unsigned)nz2);
+      i = (int)fdiv(r, dy);
+      j = (int)(r - (unsigned)i * (unsigned)n1);
+    } else {
+      k = (int)(id % nz2);
+      const size_t r = id / nz2;
+      j = (int)(r % n1);
+      i = (int)(r / n1);
+    }
     T fact;
     if (lpt_fact<T>(n0, n1, n2, kl0, kl1, kl2, k_smooth, i, j, k, fact)) {
       auto v = ck[id];
@@ -202,80 +258,183 @@ __global__ void k_poisson_grid(int n0, int n1, int n2, T kl0, T kl1, T kl2, T k_
   }
 }
 
-// Sixth-order central difference D6 f = (f+3 - f-3) - 9 (f+2 - f-2)
-// + 45 (f+1 - f-1), periodic, at stride `st` along an axis of length n.
-template <typename T>
-__device__ inline T d6(const T* f, size_t base, int c, int n, size_t st, int comp, int ncomp) {
-  auto at = [&](int o) { return f[(base + (size_t)wrap1(c, o, n) * st) * ncomp + comp]; };
-  return (at(3) - at(-3)) - (T)9 * (at(2) - at(-2)) + (T)45 * (at(1) - at(-1));
+// ---------------------------------------------------------------------------
+// Stencil tiles.  A workgroup owns a ST_TJ x ST_TK tile of (j, k) and walks
+// i over a slab of ST_TI planes.  Plane i, with halos of 3 cells in j and
+// k, is staged in LDS for the y and z differences; the x difference uses a
+// register window of the thread's points over i-3..i+3, so each value is
+// read from HBM about once.  256 threads: 64 along k, 4 along j, each
+// thread owning the 4 rows j = ty + 4 q of its column.  Differences are
+// evaluated in double in the reference's order,
+//   1 f[+3] - 9 f[+2] + 45 f[+1] - 45 f[-1] + 9 f[-2] - 1 f[-3]
+// (lpt.c:64-68), and scaled by 1 / (60 Lc) in double.
+// ---------------------------------------------------------------------------
+static constexpr int ST_TJ = 16, ST_TK = 64, ST_TI = 32, ST_H = 3;
+static constexpr int ST_PJ = ST_TJ + 2 * ST_H, ST_PK = ST_TK + 2 * ST_H;
+static constexpr int ST_Q = ST_TJ / 4;
+
+__device__ inline double d6w(double m3, double m2, double m1, double p1, double p2, double p3) {
+  return 1.0 * p3 - 9.0 * p2 + 45.0 * p1 - 45.0 * m1 + 9.0 * m2 - 1.0 * m3;
 }
 
+struct TileGeom {
+  int n0, n1, n2, ntk, ntj, nti;
+};
+
+__device__ inline void tile_origin(const TileGeom& g, int& i0, int& j0, int& k0) {
+  const int b = blockIdx.x;
+  const int bk = b % g.ntk, bj = (b / g.ntk) % g.ntj, bi = b / (g.ntk * g.ntj);
+  i0 = bi * ST_TI;
+  j0 = bj * ST_TJ;
+  k0 = bk * ST_TK;
+}
+
+// Stage plane i of component `comp` (of ncomp) into pl[ST_PJ][ST_PK] with
+// periodic halos; cells outside a partial tile are staged too (harmless).
 template <typename T>
-__global__ void k_fd_grad1(int n0, int n1, int n2, const T* phi, T inv60Lc, T* S, T* V, T f1) {
-  size_t ng = (size_t)n0 * n1 * n2;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ng;
-       c += (size_t)gridDim.x * blockDim.x) {
-    int k = (int)(c % n2);
-    size_t r = c / n2;
-    int j = (int)(r % n1);
-    int i = (int)(r / n1);
-    T gx = d6(phi, c - (size_t)i * n1 * n2, i, n0, (size_t)n1 * n2, 0, 1);
-    T gy = d6(phi, c - (size_t)j * n2, j, n1, (size_t)n2, 0, 1);
-    T gz = d6(phi, c - (size_t)k, k, n2, 1, 0, 1);
-    T sx = -gx * inv60Lc, sy = -gy * inv60Lc, sz = -gz * inv60Lc;
-    S[3 * c] = sx;
-    S[3 * c + 1] = sy;
-    S[3 * c + 2] = sz;
-    if (V) {
-      V[3 * c] = f1 * sx;
-      V[3 * c + 1] = f1 * sy;
-      V[3 * c + 2] = f1 * sz;
+__device__ inline void stage_plane(const TileGeom& g, const T* f, int ncomp, int comp, int i,
+                                   int j0, int k0, T* pl) {
+  for (int v = threadIdx.x; v < ST_PJ * ST_PK; v += 256) {
+    const int pj = v / ST_PK, pk = v % ST_PK;
+    int j = j0 + pj - ST_H, k = k0 + pk - ST_H;
+    j = j < 0 ? j + g.n1 : (j >= g.n1 ? j - g.n1 : j);
+    k = k < 0 ? k + g.n2 : (k >= g.n2 ? k - g.n2 : k);
+    j = min(j, g.n1 - 1);  // partial tiles smaller than the halo
+    k = min(k, g.n2 - 1);
+    pl[v] = f[(((size_t)i * g.n1 + j) * g.n2 + k) * ncomp + comp];
+  }
+}
+
+__device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Gradient of a scalar field phi: MODE 1 writes S = -D phi / (60 Lc) and
+// V = f1 S (lpt.c:56-93); MODE 2 adds -3/7 Omz^(-1/143) D phi2 / (60 Lc) to
+// S and f2 times it to V (lpt.c:182-223; coef carries the prefactor).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, double inv60,
+                                                    double coef, T* S, T* V, T fv) {
+  __shared__ T pl[ST_PJ * ST_PK];
+  int i0, j0, k0;
+  tile_origin(g, i0, j0, k0);
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int k = k0 + tx;
+  const bool kin = k < g.n2;
+  const int kc = kin ? k : g.n2 - 1;
+  T win[ST_Q][7];
+  for (int q = 0; q < ST_Q; q++) {
+    const int j = min(j0 + ty + 4 * q, g.n1 - 1);
+    for (int d = 0; d < 6; d++) {
+      const int ii = wrapi(i0 - 3 + d, g.n0);
+      win[q][d] = phi[((size_t)ii * g.n1 + j) * g.n2 + kc];
     }
   }
-}
-
-template <typename T>
-__global__ void k_lpt2_source(int n0, int n1, int n2, const T* S, T inv60Lc, T* src) {
-  size_t ng = (size_t)n0 * n1 * n2;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ng;
-       c += (size_t)gridDim.x * blockDim.x) {
-    int k = (int)(c % n2);
-    size_t r = c / n2;
-    int j = (int)(r % n1);
-    int i = (int)(r / n1);
-    size_t bx = c - (size_t)i * n1 * n2, by = c - (size_t)j * n2, bz = c - (size_t)k;
-    size_t sxs = (size_t)n1 * n2, sys = (size_t)n2;
-    T sxx = d6(S, bx, i, n0, sxs, 0, 3) * inv60Lc;
-    T sxy = d6(S, by, j, n1, sys, 0, 3) * inv60Lc;
-    T sxz = d6(S, bz, k, n2, 1, 0, 3) * inv60Lc;
-    T syy = d6(S, by, j, n1, sys, 1, 3) * inv60Lc;
-    T syz = d6(S, bz, k, n2, 1, 1, 3) * inv60Lc;
-    T szz = d6(S, bz, k, n2, 1, 2, 3) * inv60Lc;
-    src[c] = sxx * syy + sxx * szz + syy * szz - sxy * sxy - sxz * sxz - syz * syz;
+  const int i1 = min(i0 + ST_TI, g.n0);
+  for (int i = i0; i < i1; i++) {
+    const int ip = wrapi(i + 3, g.n0);
+    for (int q = 0; q < ST_Q; q++) {
+      const int j = min(j0 + ty + 4 * q, g.n1 - 1);
+      win[q][6] = phi[((size_t)ip * g.n1 + j) * g.n2 + kc];
+    }
+    __syncthreads();  // previous plane's readers are done
+    stage_plane<T>(g, phi, 1, 0, i, j0, k0, pl);
+    __syncthreads();
+    for (int q = 0; q < ST_Q; q++) {
+      const int jl = ty + 4 * q, j = j0 + jl;
+      const T* w = win[q];
+      const double gx = d6w(w[0], w[1], w[2], w[4], w[5], w[6]);
+      const T* rowj = pl + (jl + ST_H) * ST_PK + tx + ST_H;
+      const double gy = d6w(rowj[-3 * ST_PK], rowj[-2 * ST_PK], rowj[-ST_PK], rowj[ST_PK],
+                            rowj[2 * ST_PK], rowj[3 * ST_PK]);
+      const double gz = d6w(rowj[-3], rowj[-2], rowj[-1], rowj[1], rowj[2], rowj[3]);
+      if (kin && j < g.n1) {
+        const size_t c = ((size_t)i * g.n1 + j) * g.n2 + k;
+        const double gg[3] = {gx, gy, gz};
+        for (int a = 0; a < 3; a++) {
+          if (MODE == 1) {
+            const T sv = (T)(-gg[a] * inv60);
+            S[3 * c + a] = sv;
+            if (V) V[3 * c + a] = fv * sv;
+          } else {
+            const T sv = (T)(coef * gg[a] * inv60);
+            S[3 * c + a] += sv;
+            if (V) V[3 * c + a] += fv * sv;
+          }
+        }
+      }
+    }
+    for (int q = 0; q < ST_Q; q++)
+      for (int d = 0; d < 6; d++) win[q][d] = win[q][d + 1];
   }
 }
 
+// Second-order source phi2 = Sxx Syy + Sxx Szz + Syy Szz - Sxy^2 - Sxz^2 -
+// Syz^2 with S_ab = D_b(S_a) / (60 Lc) (lpt.c:116-134): the three
+// products in the field's precision, the squares in double, as the
+// reference's expression (pow(x, 2.0) promotes) evaluates them.
 template <typename T>
-__global__ void k_fd_grad2(int n0, int n1, int n2, const T* phi2, T coef, T* S, T* V, T f2) {
-  size_t ng = (size_t)n0 * n1 * n2;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ng;
-       c += (size_t)gridDim.x * blockDim.x) {
-    int k = (int)(c % n2);
-    size_t r = c / n2;
-    int j = (int)(r % n1);
-    int i = (int)(r / n1);
-    T gx = d6(phi2, c - (size_t)i * n1 * n2, i, n0, (size_t)n1 * n2, 0, 1) * coef;
-    T gy = d6(phi2, c - (size_t)j * n2, j, n1, (size_t)n2, 0, 1) * coef;
-    T gz = d6(phi2, c - (size_t)k, k, n2, 1, 0, 1) * coef;
-    S[3 * c] += gx;
-    S[3 * c + 1] += gy;
-    S[3 * c + 2] += gz;
-    if (V) {
-      V[3 * c] += f2 * gx;
-      V[3 * c + 1] += f2 * gy;
-      V[3 * c + 2] += f2 * gz;
+__global__ __launch_bounds__(256) void k_source_tiles(TileGeom g, const T* S, double inv60,
+                                                      T* src) {
+  __shared__ T pl[3][ST_PJ * ST_PK];
+  int i0, j0, k0;
+  tile_origin(g, i0, j0, k0);
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int k = k0 + tx;
+  const bool kin = k < g.n2;
+  const int kc = kin ? k : g.n2 - 1;
+  T win[ST_Q][7];  // Sx over i-3..i+3
+  for (int q = 0; q < ST_Q; q++) {
+    const int j = min(j0 + ty + 4 * q, g.n1 - 1);
+    for (int d = 0; d < 6; d++) {
+      const int ii = wrapi(i0 - 3 + d, g.n0);
+      win[q][d] = S[(((size_t)ii * g.n1 + j) * g.n2 + kc) * 3];
     }
   }
+  const int i1 = min(i0 + ST_TI, g.n0);
+  for (int i = i0; i < i1; i++) {
+    const int ip = wrapi(i + 3, g.n0);
+    for (int q = 0; q < ST_Q; q++) {
+      const int j = min(j0 + ty + 4 * q, g.n1 - 1);
+      win[q][6] = S[(((size_t)ip * g.n1 + j) * g.n2 + kc) * 3];
+    }
+    __syncthreads();
+    for (int comp = 0; comp < 3; comp++) stage_plane<T>(g, S, 3, comp, i, j0, k0, pl[comp]);
+    __syncthreads();
+    for (int q = 0; q < ST_Q; q++) {
+      const int jl = ty + 4 * q, j = j0 + jl;
+      const T* w = win[q];
+      const int o = (jl + ST_H) * ST_PK + tx + ST_H;
+      const T* x = pl[0] + o;
+      const T* y = pl[1] + o;
+      const T* z = pl[2] + o;
+      const T sxx = (T)(d6w(w[0], w[1], w[2], w[4], w[5], w[6]) * inv60);
+      const T sxy = (T)(d6w(x[-3 * ST_PK], x[-2 * ST_PK], x[-ST_PK], x[ST_PK], x[2 * ST_PK],
+                            x[3 * ST_PK]) * inv60);
+      const T sxz = (T)(d6w(x[-3], x[-2], x[-1], x[1], x[2], x[3]) * inv60);
+      const T syy = (T)(d6w(y[-3 * ST_PK], y[-2 * ST_PK], y[-ST_PK], y[ST_PK], y[2 * ST_PK],
+                            y[3 * ST_PK]) * inv60);
+      const T syz = (T)(d6w(y[-3], y[-2], y[-1], y[1], y[2], y[3]) * inv60);
+      const T szz = (T)(d6w(z[-3], z[-2], z[-1], z[1], z[2], z[3]) * inv60);
+      if (kin && j < g.n1) {
+        const T t3 = sxx * syy + sxx * szz + syy * szz;
+        const double v = (double)t3 - (double)sxy * (double)sxy - (double)sxz * (double)sxz -
+                         (double)syz * (double)syz;
+        src[((size_t)i * g.n1 + j) * g.n2 + k] = (T)v;
+      }
+    }
+    for (int q = 0; q < ST_Q; q++)
+      for (int d = 0; d < 6; d++) win[q][d] = win[q][d + 1];
+  }
+}
+
+static TileGeom tile_geom(int n0, int n1, int n2) {
+  TileGeom g;
+  g.n0 = n0;
+  g.n1 = n1;
+  g.n2 = n2;
+  g.ntk = (n2 + ST_TK - 1) / ST_TK;
+  g.ntj = (n1 + ST_TJ - 1) / ST_TJ;
+  g.nti = (n0 + ST_TI - 1) / ST_TI;
+  return g;
 }
 
 template <typename T>
@@ -302,6 +461,16 @@ __global__ void k_to_positions(int n0, int n1, int n2, T Lc, T L0, T L1, T L2, T
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+// Row-walking mode kernels: grid and LDS of the staged spline.
+static unsigned row_grid(long long rows) {
+  return (unsigned)std::max(1LL, std::min(rows, 4096LL));
+}
+
+template <typename T>
+static size_t spline_lds(const FieldArgs<T>& a) {
+  return a.stage ? 5 * (size_t)a.nk * sizeof(double) : 0;
+}
+
 static unsigned stream_grid(size_t n) {
   size_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
@@ -326,20 +495,34 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
   a.phase = (T)s->phase;
   // spline of P(k) on knots rounded as the wrappers pass them (fft_real)
   int nk = s->Nk;
-  std::vector<double> tab(3 * (size_t)nk);
+  EXH_REQUIRE(nk >= 2, EXH_ERR_ARG, "P(k) needs at least two knots");
+  // [x | y | b | c | d]: interval coefficients of y + t (b + t (c + t d)),
+  // with the expressions of the oracle's spline_eval
+  std::vector<double> tab(5 * (size_t)nk, 0.0), m(nk);
+  double *x = tab.data(), *y = x + nk, *cb = y + nk, *cc = cb + nk, *cd = cc + nk;
   for (int i = 0; i < nk; i++) {
-    tab[i] = (double)(T)s->k[i];
-    tab[nk + i] = (double)(T)s->P[i];
+    x[i] = (double)(T)s->k[i];
+    y[i] = (double)(T)s->P[i];
   }
-  spline_second_derivs(tab.data(), tab.data() + nk, nk, tab.data() + 2 * nk);
+  spline_second_derivs(x, y, nk, m.data());
+  for (int lo = 0; lo + 1 < nk; lo++) {
+    const int hi = lo + 1;
+    const double h = x[hi] - x[lo];
+    cb[lo] = (y[hi] - y[lo]) / h - h * (2.0 * m[lo] + m[hi]) / 6.0;
+    cc[lo] = m[lo] / 2.0;
+    cd[lo] = (m[hi] - m[lo]) / (6.0 * h);
+  }
   double* d = (double*)ctx_buf(c, BUF_TAB, tab.size() * sizeof(double));
   EXH_CHECK_HIP(hipMemcpyAsync(d, tab.data(), tab.size() * sizeof(double),
                                hipMemcpyHostToDevice, c->stream));
   EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // tab is a host temporary
   a.sx = d;
   a.sy = d + nk;
-  a.sm = d + 2 * nk;
+  a.sb = d + 2 * nk;
+  a.sc = d + 3 * nk;
+  a.sd = d + 4 * nk;
   a.nk = nk;
+  a.stage = 5 * (size_t)nk * sizeof(double) <= 48 * 1024 ? 1 : 0;
   // the reference aborts (GSL EDOM) when |k| of the grid leaves the table
   double kmax = 0;
   for (int ax = 0; ax < 3; ax++) {
@@ -365,7 +548,8 @@ void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k
   C2* ck = (C2*)ctx_buf(c, BUF_CK0, nh * sizeof(C2));
   {
     TimedScope ts(c, "field_modes");
-    hipLaunchKernelGGL(k_field<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, a,
+    hipLaunchKernelGGL(k_field<T>, dim3(row_grid((long long)b->nd[0] * b->nd[1])), dim3(256),
+                       spline_lds(a), c->stream, a,
                        (T)b->Normx, ck, (C2*)deltak_raw);
     EXH_LAUNCH_CHECK();
   }
@@ -388,52 +572,55 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
     TimedScope ts(c, "lpt1_kspace");
     if (s->kind == EXH_SRC_MODES) {
       auto a = make_field_args<T>(c, b, s, 1e5);
-      hipLaunchKernelGGL(k_poisson_modes<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream,
+      hipLaunchKernelGGL(k_poisson_modes<T>, dim3(row_grid((long long)n0 * n1)), dim3(256),
+                         spline_lds(a), c->stream,
                          a, (T)k_smooth, Normx, ck);
     } else if (s->kind == EXH_SRC_DELTAK) {
       EXH_CHECK_HIP(hipMemcpyAsync(ck, s->data, nh * sizeof(C2), hipMemcpyDeviceToDevice,
                                    c->stream));
       hipLaunchKernelGGL(k_poisson_grid<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, n0,
-                         n1, n2, kl0, kl1, kl2, (T)k_smooth, (T)1, Normx, ck);
+                         n1, n2, kl0, kl1, kl2, (T)k_smooth, (T)1, Normx, fast_div(b->nz2), fast_div(n1), ck);
     } else {
       // Compute_Denk (density_grid.c:220-235): r2c then * Normk
       fft_r2c(c, n0, n1, n2, b->precision, 1, s->data, ck);
       hipLaunchKernelGGL(k_poisson_grid<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, n0,
-                         n1, n2, kl0, kl1, kl2, (T)k_smooth, Normk, Normx, ck);
+                         n1, n2, kl0, kl1, kl2, (T)k_smooth, Normk, Normx, fast_div(b->nz2), fast_div(n1), ck);
     }
     EXH_LAUNCH_CHECK();
   }
   fft_c2r(c, n0, n1, n2, b->precision, 1, ck, phi);
-  T inv60Lc = (T)(1.0 / (60.0 * b->Lc));
+  const double inv60 = 1.0 / (60.0 * (double)(T)b->Lc);
+  const TileGeom tg = tile_geom(n0, n1, n2);
+  const unsigned tblocks = (unsigned)(tg.ntk * tg.ntj * tg.nti);
   T f1 = (T)(std::pow((T)b->Omz, 0.5454) * (T)b->Hz / (1.0 + (T)b->redshift));
   {
     TimedScope ts(c, "lpt1_grad");
-    hipLaunchKernelGGL(k_fd_grad1<T>, dim3(stream_grid(ng)), dim3(256), 0, c->stream, n0, n1,
-                       n2, phi, inv60Lc, S, V, f1);
+    hipLaunchKernelGGL((k_grad_tiles<T, 1>), dim3(tblocks), dim3(256), 0, c->stream, tg, phi,
+                       inv60, 1.0, S, V, f1);
     EXH_LAUNCH_CHECK();
   }
   if (do_2lpt) {
     T* src = (T*)ctx_buf(c, BUF_R1, ng * sizeof(T));
     {
       TimedScope ts(c, "lpt2_source");
-      hipLaunchKernelGGL(k_lpt2_source<T>, dim3(stream_grid(ng)), dim3(256), 0, c->stream, n0,
-                         n1, n2, S, inv60Lc, src);
+      hipLaunchKernelGGL(k_source_tiles<T>, dim3(tblocks), dim3(256), 0, c->stream, tg, S, inv60,
+                         src);
       EXH_LAUNCH_CHECK();
     }
     fft_r2c(c, n0, n1, n2, b->precision, 1, src, ck);
     {
       TimedScope ts(c, "lpt2_kspace");
       hipLaunchKernelGGL(k_poisson_grid<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, n0,
-                         n1, n2, kl0, kl1, kl2, (T)k_smooth, Normk, Normx, ck);
+                         n1, n2, kl0, kl1, kl2, (T)k_smooth, Normk, Normx, fast_div(b->nz2), fast_div(n1), ck);
       EXH_LAUNCH_CHECK();
     }
     fft_c2r(c, n0, n1, n2, b->precision, 1, ck, src);
-    T coef = (T)(-3.0 / 7.0 * std::pow((T)b->Omz, -1.0 / 143) / (60.0 * b->Lc));
+    const double coef = -3.0 / 7.0 * std::pow((double)(T)b->Omz, -1.0 / 143);
     T f2 = (T)(2.0 * std::pow((T)b->Omz, 6.0 / 11.0) * (T)b->Hz / (1.0 + (T)b->redshift));
     {
       TimedScope ts(c, "lpt2_grad");
-      hipLaunchKernelGGL(k_fd_grad2<T>, dim3(stream_grid(ng)), dim3(256), 0, c->stream, n0, n1,
-                         n2, src, coef, S, V, f2);
+      hipLaunchKernelGGL((k_grad_tiles<T, 2>), dim3(tblocks), dim3(256), 0, c->stream, tg, src,
+                         inv60, coef, S, V, f2);
       EXH_LAUNCH_CHECK();
     }
   }
