@@ -74,6 +74,8 @@ enum Slot {
   BUF_PART,     // P(k) block partials
   BUF_TAB,      // P(k) spline knots
   BUF_SHELL,    // shell offset table of the halo finder
+  BUF_HALOTAB,  // halo finder tables (barrier, mass function, spheres, taus)
+  BUF_PTAB,     // power spectrum phase and window tables
   BUF_EFF,      // halo growth: delta with flagged cells set to -Mtot
   BUF_NSLOTS
 };
@@ -124,6 +126,13 @@ struct HaloState {
 
 }  // namespace exh
 
+// Byte key of a cached table: appends the raw bytes of n objects.
+template <typename U>
+inline void key_append(std::vector<unsigned char>& key, const U* p, size_t n) {
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
+  key.insert(key.end(), b, b + n * sizeof(U));
+}
+
 struct exh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -141,6 +150,13 @@ struct exh_ctx {
   // pinned host staging for small results
   void* pinned = nullptr;
   size_t pinned_cap = 0;
+  // host-built tables kept on the device while their inputs are unchanged
+  std::vector<unsigned char> halotab_key;  // halo tables in BUF_HALOTAB
+  int halotab_ncells = 0;
+  double halotab_bmin = 0;
+  std::vector<unsigned char> spline_key;  // P(k) spline in BUF_TAB
+  double spline_x0 = 0, spline_x1 = 0;    // its first and last knot
+  long long shell_noffs = 0;
   // halo finder statistics of the last run
   long long stat_peaks = 0, stat_candidates = 0, stat_rounds = 0, stat_sort_levels = 0,
             stat_sort_doubling = 0, stat_sort_async_open = 0;

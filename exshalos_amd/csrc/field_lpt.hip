@@ -493,29 +493,42 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
   a.noise = (const T*)s->data;
   a.fixed = s->fixed;
   a.phase = (T)s->phase;
-  // spline of P(k) on knots rounded as the wrappers pass them (fft_real)
+  // spline of P(k) on knots rounded as the wrappers pass them (fft_real),
+  // built on the host and kept on the device while k and P are unchanged
   int nk = s->Nk;
   EXH_REQUIRE(nk >= 2, EXH_ERR_ARG, "P(k) needs at least two knots");
-  // [x | y | b | c | d]: interval coefficients of y + t (b + t (c + t d)),
-  // with the expressions of the oracle's spline_eval
-  std::vector<double> tab(5 * (size_t)nk, 0.0), m(nk);
-  double *x = tab.data(), *y = x + nk, *cb = y + nk, *cc = cb + nk, *cd = cc + nk;
-  for (int i = 0; i < nk; i++) {
-    x[i] = (double)(T)s->k[i];
-    y[i] = (double)(T)s->P[i];
+  const size_t cap0 = c->cap[BUF_TAB];
+  double* d = (double*)ctx_buf(c, BUF_TAB, 5 * (size_t)nk * sizeof(double));
+  if (c->cap[BUF_TAB] != cap0) c->spline_key.clear();  // reallocated
+  std::vector<unsigned char> key;
+  const int kp[2] = {(int)sizeof(T), nk};
+  key_append(key, kp, 2);
+  key_append(key, s->k, nk);
+  key_append(key, s->P, nk);
+  if (key != c->spline_key) {
+    // [x | y | b | c | d]: interval coefficients of y + t (b + t (c + t d)),
+    // with the expressions of the oracle's spline_eval
+    std::vector<double> tab(5 * (size_t)nk, 0.0), m(nk);
+    double *x = tab.data(), *y = x + nk, *cb = y + nk, *cc = cb + nk, *cd = cc + nk;
+    for (int i = 0; i < nk; i++) {
+      x[i] = (double)(T)s->k[i];
+      y[i] = (double)(T)s->P[i];
+    }
+    spline_second_derivs(x, y, nk, m.data());
+    for (int lo = 0; lo + 1 < nk; lo++) {
+      const int hi = lo + 1;
+      const double h = x[hi] - x[lo];
+      cb[lo] = (y[hi] - y[lo]) / h - h * (2.0 * m[lo] + m[hi]) / 6.0;
+      cc[lo] = m[lo] / 2.0;
+      cd[lo] = (m[hi] - m[lo]) / (6.0 * h);
+    }
+    EXH_CHECK_HIP(hipMemcpyAsync(d, tab.data(), tab.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // tab is a host temporary
+    c->spline_key = key;
+    c->spline_x0 = x[0];
+    c->spline_x1 = x[nk - 1];
   }
-  spline_second_derivs(x, y, nk, m.data());
-  for (int lo = 0; lo + 1 < nk; lo++) {
-    const int hi = lo + 1;
-    const double h = x[hi] - x[lo];
-    cb[lo] = (y[hi] - y[lo]) / h - h * (2.0 * m[lo] + m[hi]) / 6.0;
-    cc[lo] = m[lo] / 2.0;
-    cd[lo] = (m[hi] - m[lo]) / (6.0 * h);
-  }
-  double* d = (double*)ctx_buf(c, BUF_TAB, tab.size() * sizeof(double));
-  EXH_CHECK_HIP(hipMemcpyAsync(d, tab.data(), tab.size() * sizeof(double),
-                               hipMemcpyHostToDevice, c->stream));
-  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // tab is a host temporary
   a.sx = d;
   a.sy = d + nk;
   a.sb = d + 2 * nk;
@@ -532,10 +545,11 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
   kmax = std::sqrt(kmax);
   double kmin = (double)std::min({(T)b->kl[0], (T)b->kl[1], (T)b->kl[2]});
   if (kmin > k_smooth) kmin = 0;
-  EXH_REQUIRE(tab[0] <= kmin * (1 + 1e-6) && std::min(kmax, k_smooth) <= tab[nk - 1] * (1 - 1e-7) + 1e-300,
+  const double x0 = c->spline_x0, x1 = c->spline_x1;
+  EXH_REQUIRE(x0 <= kmin * (1 + 1e-6) && std::min(kmax, k_smooth) <= x1 * (1 - 1e-7) + 1e-300,
               EXH_ERR_RANGE,
               "the box's |k| range [%g, %g] lies outside the tabulated P(k) [%g, %g]", kmin,
-              std::min(kmax, k_smooth), tab[0], tab[nk - 1]);
+              std::min(kmax, k_smooth), x0, x1);
   return a;
 }
 

@@ -525,10 +525,10 @@ __global__ void k_halo_posh(const int* hx, const int* hcount, const int* hr2, lo
 // it; returns the number of offsets.
 static long long ensure_shells(exh_ctx* c, int stab, int tmax) {
   HaloState& S = c->halo;
+  if (S.sh.stab == stab && S.sh_tmax == tmax) return c->shell_noffs;
   std::vector<int> start;
   std::vector<unsigned> packed;
   shell_offsets(stab, tmax, start, packed);
-  if (S.sh.stab == stab && S.sh_tmax == tmax) return (long long)packed.size();
   size_t bytes = (start.size() + packed.size()) * 4;
   int* d = (int*)ctx_buf(c, BUF_SHELL, bytes);
   EXH_CHECK_HIP(hipMemcpyAsync(d, start.data(), start.size() * 4, hipMemcpyHostToDevice, c->stream));
@@ -539,7 +539,23 @@ static long long ensure_shells(exh_ctx* c, int stab, int tmax) {
   S.sh.offs = (const unsigned*)(d + start.size());
   S.sh.stab = stab;
   S.sh_tmax = tmax;
-  return (long long)packed.size();
+  c->shell_noffs = (long long)packed.size();
+  return c->shell_noffs;
+}
+
+// Constant tables, built once per process.
+static const std::vector<int>& sphere_table() {
+  static const std::vector<int> t = cells_in_spheres(10000);
+  return t;
+}
+
+static const std::vector<unsigned>& taus_table() {
+  static const std::vector<unsigned> t = [] {
+    std::vector<unsigned> m(3 * 40 * 32);
+    taus_jump_tables(m.data());
+    return m;
+  }();
+  return t;
 }
 
 template <typename T>
@@ -554,41 +570,60 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   const int stab = std::min(S_TAB_MAX, 3 * tmax * tmax + 2);
   const long long noffs = ensure_shells(c, stab, tmax);
 
-  // tables (host): barrier per count, mass-function knots
+  // tables (built on the host, kept on the device while their inputs are
+  // unchanged): barrier per count, mass-function knots and their splines,
+  // sphere counts, taus jump matrices
   const int nBt = (int)noffs + 2;
-  std::vector<T> sig(std::max(1, (int)std::floor(1e16 / (T)b->Mcell)) + 1), Bt(nBt);
-  std::vector<double> Mk(o->Nk), Int(o->Nk);
-  int Ncells = halo_tables(b, o, sig.data(), Bt.data(), nBt, Mk.data(), Int.data());
-  EXH_REQUIRE(Ncells >= 1, Ncells < 0 ? EXH_ERR_RANGE : EXH_ERR_ARG,
-              "sigma(M) table: cell mass %g gives %d cells below M_max", (double)b->Mcell, Ncells);
-  std::vector<double> mI(o->Nk), mInv(o->Nk);
-  spline_second_derivs(Mk.data(), Int.data(), o->Nk, mI.data());
-  spline_second_derivs(Int.data(), Mk.data(), o->Nk, mInv.data());
-  std::vector<int> sph = cells_in_spheres(10000);
-  std::vector<unsigned> mats(3 * 40 * 32);
-  taus_jump_tables(mats.data());
-
-  // misc device tables in BUF_MISC
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t r = off;
-    off += (bytes + 255) & ~(size_t)255;
+  const std::vector<int>& sph = sphere_table();
+  const std::vector<unsigned>& mats = taus_table();
+  size_t toff = 0;
+  auto ttake = [&](size_t bytes) {
+    size_t r = toff;
+    toff += (bytes + 255) & ~(size_t)255;
     return r;
   };
-  size_t o_bt = take(nBt * sizeof(T)), o_mk = take(o->Nk * 8), o_int = take(o->Nk * 8),
-         o_mi = take(o->Nk * 8), o_minv = take(o->Nk * 8), o_sph = take(sph.size() * 4),
-         o_mats = take(mats.size() * 4), o_cnt = take(256);
-  char* misc = (char*)ctx_buf(c, BUF_MISC, off);
-  auto up = [&](size_t o2, const void* src, size_t bytes) {
-    EXH_CHECK_HIP(hipMemcpyAsync(misc + o2, src, bytes, hipMemcpyHostToDevice, c->stream));
-  };
-  up(o_bt, Bt.data(), nBt * sizeof(T));
-  up(o_mk, Mk.data(), o->Nk * 8);
-  up(o_int, Int.data(), o->Nk * 8);
-  up(o_mi, mI.data(), o->Nk * 8);
-  up(o_minv, mInv.data(), o->Nk * 8);
-  up(o_sph, sph.data(), sph.size() * 4);
-  up(o_mats, mats.data(), mats.size() * 4);
+  const size_t o_bt = ttake(nBt * sizeof(T)), o_mk = ttake(o->Nk * 8), o_int = ttake(o->Nk * 8),
+               o_mi = ttake(o->Nk * 8), o_minv = ttake(o->Nk * 8), o_sph = ttake(sph.size() * 4),
+               o_mats = ttake(mats.size() * 4);
+  std::vector<unsigned char> key;
+  const double oscal[3] = {o->a, o->beta, o->alpha};
+  const int sizes[2] = {o->Nk, nBt};
+  key_append(key, b, 1);
+  key_append(key, oscal, 3);
+  key_append(key, sizes, 2);
+  key_append(key, o->k, o->Nk);
+  key_append(key, o->P, o->Nk);
+  const size_t cap0 = c->cap[BUF_HALOTAB];
+  char* tabs = (char*)ctx_buf(c, BUF_HALOTAB, toff);
+  if (c->cap[BUF_HALOTAB] != cap0) c->halotab_key.clear();  // reallocated
+  if (key != c->halotab_key) {
+    std::vector<T> sig(std::max(1, (int)std::floor(1e16 / (T)b->Mcell)) + 1), Bt(nBt);
+    std::vector<double> Mk(o->Nk), Int(o->Nk);
+    int Ncells = halo_tables(b, o, sig.data(), Bt.data(), nBt, Mk.data(), Int.data());
+    EXH_REQUIRE(Ncells >= 1, Ncells < 0 ? EXH_ERR_RANGE : EXH_ERR_ARG,
+                "sigma(M) table: cell mass %g gives %d cells below M_max", (double)b->Mcell,
+                Ncells);
+    std::vector<double> mI(o->Nk), mInv(o->Nk);
+    spline_second_derivs(Mk.data(), Int.data(), o->Nk, mI.data());
+    spline_second_derivs(Int.data(), Mk.data(), o->Nk, mInv.data());
+    auto up = [&](size_t o2, const void* src, size_t bytes) {
+      EXH_CHECK_HIP(hipMemcpyAsync(tabs + o2, src, bytes, hipMemcpyHostToDevice, c->stream));
+    };
+    up(o_bt, Bt.data(), nBt * sizeof(T));
+    up(o_mk, Mk.data(), o->Nk * 8);
+    up(o_int, Int.data(), o->Nk * 8);
+    up(o_mi, mI.data(), o->Nk * 8);
+    up(o_minv, mInv.data(), o->Nk * 8);
+    up(o_sph, sph.data(), sph.size() * 4);
+    up(o_mats, mats.data(), mats.size() * 4);
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // the host tables are temporaries
+    c->halotab_key = key;
+    c->halotab_ncells = Ncells;
+    c->halotab_bmin = (double)Bt[Ncells - 1];
+  }
+  const int Ncells = c->halotab_ncells;
+  char* misc = (char*)ctx_buf(c, BUF_MISC, 256);
+  const size_t o_cnt = 0;
   long long* dcnt = (long long*)(misc + o_cnt);  // [0..7] scratch counters
   int* derr = (int*)(misc + o_cnt + 128);
   long long* hbuf = (long long*)ctx_pinned(c, 256);
@@ -620,7 +655,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   lomuto_sort<T>(c, pkey, pidx, np);
 
   // 3. candidates
-  const T Bmin = Bt[Ncells - 1];
+  const T Bmin = (T)c->halotab_bmin;
   hipLaunchKernelGGL(k_lower_bound<T>, dim3(1), dim3(1), 0, c->stream, pkey, np, Bmin, dcnt);
   EXH_LAUNCH_CHECK();
   EXH_CHECK_HIP(hipMemcpyAsync(hbuf, dcnt, 8, hipMemcpyDeviceToHost, c->stream));
@@ -670,7 +705,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   ga.flag = flag;
   ga.claim = claim;
   ga.sh = S.sh;
-  ga.Bt = (const T*)(misc + o_bt);
+  ga.Bt = (const T*)(tabs + o_bt);
   ga.nB = nBt;
   ga.Bmin = Bmin;
   ga.negMtot = -(T)b->Mtot;
@@ -764,12 +799,12 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     taus_seed_state(o->seed, st);
     double* u = (double*)(hal + o_u);
     hipLaunchKernelGGL(k_taus, dim3(grid_for((nh + 63) / 64, 256)), dim3(256), 0, c->stream,
-                       (const unsigned*)(misc + o_mats), st[0], st[1], st[2], nh, u);
+                       (const unsigned*)(tabs + o_mats), st[0], st[1], st[2], nh, u);
     EXH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_masses<T>, dim3(grid_for(nh, 256)), dim3(256), 0, c->stream, S.hcount, nh,
-                       (const int*)(misc + o_sph), (int)sph.size(), Ncells, (T)b->Mcell,
-                       (const double*)(misc + o_mk), (const double*)(misc + o_int),
-                       (const double*)(misc + o_mi), (const double*)(misc + o_minv), o->Nk, u,
+                       (const int*)(tabs + o_sph), (int)sph.size(), Ncells, (T)b->Mcell,
+                       (const double*)(tabs + o_mk), (const double*)(tabs + o_int),
+                       (const double*)(tabs + o_mi), (const double*)(tabs + o_minv), o->Nk, u,
                        (T*)S.Mh, derr);
     EXH_LAUNCH_CHECK();
     EXH_CHECK_HIP(hipMemcpyAsync(hbuf, derr, 4, hipMemcpyDeviceToHost, c->stream));

@@ -735,7 +735,7 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
     wz[k] = 1.0 / std::pow(sinc_t<T>((T)((double)kz * (double)Lb / 2.0)), pw);
   }
   size_t tb = (2 * (size_t)nd + 2 * nz) * sizeof(double2);
-  char* tab = (char*)ctx_buf(c, BUF_TAB, tb);
+  char* tab = (char*)ctx_buf(c, BUF_PTAB, tb);
   double2* dex = (double2*)tab;
   double2* dez = dex + nd;
   double* dwx = (double*)(dez + nz);

@@ -87,6 +87,25 @@ def test_find_halos_ties_and_barrier(bbks, prec):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+def test_cached_tables_follow_pk(bbks, prec):
+    """The P(k) spline and the halo tables are kept on the device between
+    calls; changing P(k), the box and back must rebuild them each time."""
+    k, P = bbks
+    set_precision(prec)
+    nd = 32
+    noise = _noise(nd, 5, prec)
+    for (Pi, Lc) in [(P, LC), (2.0 * P, LC), (2.0 * P, 5.0), (P, LC)]:
+        g = X.density_grid_compute(k, Pi, 1e5, nd, nd, nd, Lc, 1, 3, 0, 0.0, 1e5, 0, 1,
+                                   noise=noise)
+        delta, _, _ = O.field_from_modes(k, Pi, nd, Lc, noise, precision=prec)
+        assert _rel(g["grid"], delta) < (2e-6 if prec == 4 else 1e-12)
+        ref = O.find_halos(delta, k, Pi, Lc, seed=12345, precision=prec)
+        out = X.find_halos(delta, k, Pi, Lc, 0.31, 0.0, -1.0, 1, 1.0, 0.0, 0.0, 1, 0, 0)
+        np.testing.assert_array_equal(X.halo_catalogue()["count"], ref["count"])
+        np.testing.assert_array_equal(out["Mh"], ref["Mh"])
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("do2", [0, 1])
 def test_lpt(bbks, prec, do2):
     k, P = bbks
