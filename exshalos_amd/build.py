@@ -28,6 +28,7 @@ SOURCES = [
     "host_tables.cpp",
     "capi.cpp",
     "util.hip",
+    "fft.hip",
     "field_lpt.hip",
     "sort.hip",
     "halos.hip",

@@ -81,10 +81,12 @@ enum Slot {
 };
 
 struct FftKey {
-  int n0, n1, n2, prec, inverse, batch;
+  int n0, n1, n2, prec, inverse;
+  long long batch;
+  int dims;  // 3: full real transform; 1: along the contiguous axis only
   bool operator<(const FftKey& o) const {
-    return std::tie(n0, n1, n2, prec, inverse, batch) <
-           std::tie(o.n0, o.n1, o.n2, o.prec, o.inverse, o.batch);
+    return std::tie(n0, n1, n2, prec, inverse, batch, dims) <
+           std::tie(o.n0, o.n1, o.n2, o.prec, o.inverse, o.batch, o.dims);
   }
 };
 
@@ -140,6 +142,7 @@ struct exh_ctx {
   void* buf[exh::BUF_NSLOTS] = {};
   size_t cap[exh::BUF_NSLOTS] = {};
   std::map<exh::FftKey, exh::FftPlan> plans;
+  std::map<int, void*> fft_tw;  // twiddle tables of the strided FFT passes
   // per-kernel HIP-event timing (enabled by exh_ctx_set_timing)
   bool timing = false;
   std::vector<exh::Timing> events;
@@ -172,6 +175,11 @@ void* ctx_pinned(exh_ctx* c, size_t bytes);
 void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void* in, void* out);
 void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, void* out);
 void fft_release(exh_ctx* c);
+// strided complex passes of the 3D transforms (fft.hip)
+bool fft_axis_supported(int n);
+void fft_axis_pass(exh_ctx* c, int prec, bool inverse, int N, void* data, long long stride,
+                   long long ncol, long long line_stride, long long nlines);
+void fft_axis_release(exh_ctx* c);
 
 // event timing around a launch sequence
 struct TimedScope {

@@ -1,0 +1,331 @@
+// Strided complex FFT passes along the two outer axes of a half-complex grid.
+//
+// A 3D real transform of [n0][n1][n2] runs as three passes: a batched 1D real
+// transform along the contiguous axis (rocFFT), and one complex pass along
+// each strided axis (this file).  A pass sees the grid as
+//
+//     lines x [N points, `stride` elements apart] x [ncol contiguous columns]
+//
+// (axis 1: lines = n0, stride = ncol = n2/2+1; axis 0: lines = 1,
+// stride = ncol = n1 (n2/2+1)) and transforms every column in place.
+//
+// One workgroup owns BZ adjacent columns of one line: 128 bytes per point,
+// so every global access is a full 128-byte row segment.  The transform is a
+// Stockham autosort FFT (Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors", SC 2008) with radix-8 stages
+// and one leading radix-2 or radix-4 stage where log2 N is not a multiple of
+// three.  The first stage reads its operands straight from global memory and
+// the last stage writes straight back; only the intermediate stages go
+// through LDS, whose rows are padded by one element against bank conflicts.
+// Twiddles W_N^k are computed on the host in double and rounded once.
+//
+// Direction and scaling follow rocFFT: forward uses exp(-2 pi i k n / N),
+// inverse exp(+2 pi i k n / N), neither is normalised.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace exh {
+
+namespace {
+
+constexpr int FFT_THREADS = 256;
+
+template <typename T>
+struct FftCfg;
+template <>
+struct FftCfg<float> {
+  static constexpr int BZ = 16;  // columns per workgroup: 16 x 8 bytes
+  static constexpr int PAD = 1;
+};
+template <>
+struct FftCfg<double> {
+  static constexpr int BZ = 8;  // 8 x 16 bytes
+  static constexpr int PAD = 0;  // keeps two workgroups per CU at N = 512
+};
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+template <int N>
+constexpr int fft_nstages() {
+  return (ilog2(N) % 3 ? 1 : 0) + ilog2(N) / 3;
+}
+
+template <int N>
+constexpr int fft_radix(int s) {
+  return (ilog2(N) % 3 && s == 0) ? (1 << (ilog2(N) % 3)) : 8;
+}
+
+template <typename C2>
+__device__ __forceinline__ C2 cadd(C2 a, C2 b) {
+  C2 r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+
+template <typename C2>
+__device__ __forceinline__ C2 csub(C2 a, C2 b) {
+  C2 r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+
+// a * w (forward) or a * conj(w) (inverse)
+template <bool INV, typename C2>
+__device__ __forceinline__ C2 ctw(C2 a, C2 w) {
+  C2 r;
+  if (INV) {
+    r.x = a.x * w.x + a.y * w.y;
+    r.y = a.y * w.x - a.x * w.y;
+  } else {
+    r.x = a.x * w.x - a.y * w.y;
+    r.y = a.x * w.y + a.y * w.x;
+  }
+  return r;
+}
+
+// a * W_4: -i forward, +i inverse
+template <bool INV, typename C2>
+__device__ __forceinline__ C2 crot(C2 a) {
+  C2 r;
+  if (INV) {
+    r.x = -a.y;
+    r.y = a.x;
+  } else {
+    r.x = a.y;
+    r.y = -a.x;
+  }
+  return r;
+}
+
+// a * W_8: (1 -+ i) / sqrt 2
+template <bool INV, typename C2>
+__device__ __forceinline__ C2 cw8(C2 a) {
+  using T = decltype(a.x);
+  const T h = (T)0.70710678118654752440;
+  C2 r;
+  if (INV) {
+    r.x = (a.x - a.y) * h;
+    r.y = (a.x + a.y) * h;
+  } else {
+    r.x = (a.x + a.y) * h;
+    r.y = (a.y - a.x) * h;
+  }
+  return r;
+}
+
+template <int R, bool INV, typename C2>
+__device__ __forceinline__ void dft(C2* v) {
+  if constexpr (R == 2) {
+    const C2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const C2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const C2 t2 = cadd(v[1], v[3]), t3 = crot<INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  } else {
+    static_assert(R == 8, "radix 2, 4 or 8");
+    C2 e[4] = {v[0], v[2], v[4], v[6]};
+    C2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft<4, INV>(e);
+    dft<4, INV>(o);
+    o[1] = cw8<INV>(o[1]);
+    o[2] = crot<INV>(o[2]);
+    o[3] = crot<INV>(cw8<INV>(o[3]));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+}
+
+// One Stockham stage over the workgroup's BZ columns: butterfly j of a
+// column reads points j + r N/R, multiplies by W_{NS R}^{(j mod NS) r} and
+// writes points (j / NS) NS R + (j mod NS) + r NS.
+template <typename C2, int N, int R, int NS, int BZ, int LD, bool INV, bool FROM_GLOBAL,
+          bool TO_GLOBAL>
+__device__ __forceinline__ void fft_stage(C2* __restrict__ g, long long stride, bool ok,
+                                          C2* __restrict__ lds, const C2* __restrict__ tw,
+                                          int c, int grp) {
+  constexpr int G = FFT_THREADS / BZ;  // threads per column
+  constexpr int NBF = N / R;           // butterflies per column
+  constexpr int PER = (NBF + G - 1) / G;
+  C2 v[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int j = grp + p * G;
+    if (NBF % G == 0 || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int n = j + r * NBF;
+        if constexpr (FROM_GLOBAL) {
+          if (ok) {
+            v[p][r] = g[(long long)n * stride];
+          } else {
+            v[p][r].x = 0;
+            v[p][r].y = 0;
+          }
+        } else {
+          v[p][r] = lds[n * LD + c];
+        }
+      }
+    }
+  }
+  if constexpr (!FROM_GLOBAL) __syncthreads();  // every read before any write
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int j = grp + p * G;
+    if (NBF % G == 0 || j < NBF) {
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; r++) v[p][r] = ctw<INV>(v[p][r], tw[k * r * (N / (NS * R))]);
+      }
+      dft<R, INV>(v[p]);
+      const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int n = base + r * NS;
+        if constexpr (TO_GLOBAL) {
+          if (ok) g[(long long)n * stride] = v[p][r];
+        } else {
+          lds[n * LD + c] = v[p][r];
+        }
+      }
+    }
+  }
+  if constexpr (!TO_GLOBAL) __syncthreads();
+}
+
+template <typename C2, int N, int BZ, int LD, bool INV, int S, int NS>
+__device__ __forceinline__ void fft_stages(C2* __restrict__ g, long long stride, bool ok,
+                                           C2* __restrict__ lds, const C2* __restrict__ tw,
+                                           int c, int grp) {
+  constexpr int NST = fft_nstages<N>();
+  constexpr int R = fft_radix<N>(S);
+  fft_stage<C2, N, R, NS, BZ, LD, INV, S == 0, S == NST - 1>(g, stride, ok, lds, tw, c, grp);
+  if constexpr (S + 1 < NST) fft_stages<C2, N, BZ, LD, INV, S + 1, NS * R>(g, stride, ok, lds, tw, c, grp);
+}
+
+template <typename T, int N>
+constexpr size_t fft_lds_bytes() {
+  using C2 = typename cplx<T>::type;
+  return ((size_t)N * (FftCfg<T>::BZ + FftCfg<T>::PAD) + N) * sizeof(C2);
+}
+
+// Workgroup w runs on XCD w mod 8; each XCD gets a contiguous range of
+// (line, column block) pairs so that neighbouring blocks, which share
+// 128-byte lines when a row is not line-aligned, meet in one L2.
+template <typename T, int N, bool INV>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(typename cplx<T>::type* data,
+                                                          long long stride, long long ncol,
+                                                          long long line_stride, int nbx,
+                                                          long long nblocks,
+                                                          const typename cplx<T>::type* tw) {
+  using C2 = typename cplx<T>::type;
+  constexpr int BZ = FftCfg<T>::BZ;
+  constexpr int LD = BZ + FftCfg<T>::PAD;
+  extern __shared__ __align__(16) unsigned char fft_lds_raw[];
+  C2* lds = reinterpret_cast<C2*>(fft_lds_raw);
+  C2* tws = lds + N * LD;
+  const long long w = blockIdx.x;
+  const long long q = nblocks / 8, rem = nblocks % 8;
+  const long long xcd = w % 8;
+  const long long logical = xcd * q + (xcd < rem ? xcd : rem) + w / 8;
+  const long long line = logical / nbx;
+  const long long xb = logical - line * nbx;
+  const int t = threadIdx.x, c = t % BZ, grp = t / BZ;
+  for (int i = t; i < N; i += FFT_THREADS) tws[i] = tw[i];
+  __syncthreads();
+  const long long col = xb * BZ + c;
+  const bool ok = col < ncol;
+  C2* g = data + line * line_stride + (ok ? col : 0);
+  fft_stages<C2, N, BZ, LD, INV, 0, 1>(g, stride, ok, lds, tws, c, grp);
+}
+
+// W_N^k = exp(-2 pi i k / N), k < N, rounded from double; one table per
+// (N, precision), kept for the life of the context.
+template <typename T>
+const typename cplx<T>::type* twiddles(exh_ctx* c, int N) {
+  using C2 = typename cplx<T>::type;
+  const int key = N * 16 + (int)sizeof(T);
+  auto it = c->fft_tw.find(key);
+  if (it != c->fft_tw.end()) return (const C2*)it->second;
+  std::vector<C2> h(N);
+  for (int k = 0; k < N; k++) {
+    const double a = -2.0 * M_PI * (double)k / (double)N;
+    h[k].x = (T)std::cos(a);
+    h[k].y = (T)std::sin(a);
+  }
+  void* d = nullptr;
+  EXH_CHECK_HIP(hipMalloc(&d, N * sizeof(C2)));
+  c->fft_tw[key] = d;
+  EXH_CHECK_HIP(hipMemcpyAsync(d, h.data(), N * sizeof(C2), hipMemcpyHostToDevice, c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // h is a host temporary
+  return (const C2*)d;
+}
+
+template <typename T, int N, bool INV>
+void launch_axis(exh_ctx* c, void* data, long long stride, long long ncol, long long line_stride,
+                 long long nlines) {
+  constexpr int BZ = FftCfg<T>::BZ;
+  const int nbx = (int)((ncol + BZ - 1) / BZ);
+  const long long nblocks = (long long)nbx * nlines;
+  EXH_REQUIRE(nblocks < (1LL << 31), EXH_ERR_ARG, "FFT pass too large");
+  const size_t lds = fft_lds_bytes<T, N>();
+  auto kern = k_fft_axis<T, N, INV>;
+  EXH_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(FFT_THREADS), lds, c->stream,
+                     (typename cplx<T>::type*)data, stride, ncol, line_stride, nbx, nblocks,
+                     twiddles<T>(c, N));
+  EXH_LAUNCH_CHECK();
+}
+
+template <typename T, bool INV>
+void axis_dispatch(exh_ctx* c, int N, void* data, long long stride, long long ncol,
+                   long long line_stride, long long nlines) {
+  switch (N) {
+    case 16: return launch_axis<T, 16, INV>(c, data, stride, ncol, line_stride, nlines);
+    case 32: return launch_axis<T, 32, INV>(c, data, stride, ncol, line_stride, nlines);
+    case 64: return launch_axis<T, 64, INV>(c, data, stride, ncol, line_stride, nlines);
+    case 128: return launch_axis<T, 128, INV>(c, data, stride, ncol, line_stride, nlines);
+    case 256: return launch_axis<T, 256, INV>(c, data, stride, ncol, line_stride, nlines);
+    case 512: return launch_axis<T, 512, INV>(c, data, stride, ncol, line_stride, nlines);
+    case 1024: return launch_axis<T, 1024, INV>(c, data, stride, ncol, line_stride, nlines);
+    default: EXH_REQUIRE(false, EXH_ERR_ARG, "no strided FFT pass of length %d", N);
+  }
+}
+
+}  // namespace
+
+bool fft_axis_supported(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+
+void fft_axis_pass(exh_ctx* c, int prec, bool inverse, int N, void* data, long long stride,
+                   long long ncol, long long line_stride, long long nlines) {
+  if (prec == 4) {
+    if (inverse)
+      axis_dispatch<float, true>(c, N, data, stride, ncol, line_stride, nlines);
+    else
+      axis_dispatch<float, false>(c, N, data, stride, ncol, line_stride, nlines);
+  } else {
+    if (inverse)
+      axis_dispatch<double, true>(c, N, data, stride, ncol, line_stride, nlines);
+    else
+      axis_dispatch<double, false>(c, N, data, stride, ncol, line_stride, nlines);
+  }
+}
+
+void fft_axis_release(exh_ctx* c) {
+  for (auto& kv : c->fft_tw) (void)hipFree(kv.second);
+  c->fft_tw.clear();
+}
+
+}  // namespace exh

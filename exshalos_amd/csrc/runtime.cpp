@@ -6,6 +6,8 @@
 #include <cstring>
 #include <mutex>
 
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace exh {
@@ -78,9 +80,13 @@ static std::once_flag g_fft_once;
     }                                                                              \
   } while (0)
 
-static FftPlan& get_plan(exh_ctx* c, int n0, int n1, int n2, int prec, int inverse, int batch) {
+// dims 3: the [n0][n1][n2] real transform; dims 1: `batch` 1D real
+// transforms of length n2 (n0, n1 unused).
+static FftPlan& get_plan(exh_ctx* c, int n0, int n1, int n2, int prec, int inverse,
+                         long long batch, int dims) {
   std::call_once(g_fft_once, [] { rocfft_setup(); });
-  FftKey key{n0, n1, n2, prec, inverse, batch};
+  if (dims == 1) n0 = n1 = 0;
+  FftKey key{n0, n1, n2, prec, inverse, batch, dims};
   auto it = c->plans.find(key);
   if (it != c->plans.end()) return it->second;
   FftPlan p;
@@ -100,7 +106,8 @@ static FftPlan& get_plan(exh_ctx* c, int n0, int n1, int n2, int prec, int inver
   EXH_CHECK_FFT(rocfft_plan_create(
       &plan, rocfft_placement_notinplace,
       inverse ? rocfft_transform_type_real_inverse : rocfft_transform_type_real_forward,
-      prec == 4 ? rocfft_precision_single : rocfft_precision_double, 3, len, (size_t)batch,
+      prec == 4 ? rocfft_precision_single : rocfft_precision_double, (size_t)dims, len,
+      (size_t)batch,
       desc));
   rocfft_plan_description_destroy(desc);
   size_t ws = 0;
@@ -124,14 +131,38 @@ static void run_plan(exh_ctx* c, FftPlan& p, const void* in, void* out) {
   EXH_CHECK_FFT(rocfft_execute((rocfft_plan)p.plan, ib, ob, (rocfft_execution_info)p.info));
 }
 
+// The strided axes go through the LDS-staged passes of fft.hip when both
+// have a supported length (EXH_FFT_AXIS=0 keeps rocFFT's 3D plans).
+static bool use_axis_passes(int n0, int n1) {
+  static const bool on = [] {
+    const char* e = std::getenv("EXH_FFT_AXIS");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on && fft_axis_supported(n0) && fft_axis_supported(n1);
+}
+
 void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void* in, void* out) {
   TimedScope ts(c, "fft_r2c");
-  run_plan(c, get_plan(c, n0, n1, n2, prec, 0, batch), in, out);
+  if (!use_axis_passes(n0, n1)) {
+    run_plan(c, get_plan(c, n0, n1, n2, prec, 0, batch, 3), in, out);
+    return;
+  }
+  const long long nz2 = n2 / 2 + 1, plane = (long long)n1 * nz2;
+  run_plan(c, get_plan(c, n0, n1, n2, prec, 0, (long long)batch * n0 * n1, 1), in, out);
+  fft_axis_pass(c, prec, false, n1, out, nz2, nz2, plane, (long long)batch * n0);
+  fft_axis_pass(c, prec, false, n0, out, plane, plane, n0 * plane, batch);
 }
 
 void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, void* out) {
   TimedScope ts(c, "fft_c2r");
-  run_plan(c, get_plan(c, n0, n1, n2, prec, 1, batch), in, out);
+  if (!use_axis_passes(n0, n1)) {
+    run_plan(c, get_plan(c, n0, n1, n2, prec, 1, batch, 3), in, out);
+    return;
+  }
+  const long long nz2 = n2 / 2 + 1, plane = (long long)n1 * nz2;
+  fft_axis_pass(c, prec, true, n0, in, plane, plane, n0 * plane, batch);
+  fft_axis_pass(c, prec, true, n1, in, nz2, nz2, plane, (long long)batch * n0);
+  run_plan(c, get_plan(c, n0, n1, n2, prec, 1, (long long)batch * n0 * n1, 1), in, out);
 }
 
 void fft_release(exh_ctx* c) {
@@ -141,6 +172,7 @@ void fft_release(exh_ctx* c) {
     if (kv.second.work) (void)hipFree(kv.second.work);
   }
   c->plans.clear();
+  fft_axis_release(c);
 }
 
 }  // namespace exh
