@@ -60,6 +60,7 @@ def algorithmic_bytes(name: str, nd: int, npk: int, interlacing: int, prec: int)
         "field_modes": 2 * c * nh,                  # read noise, write delta_k
         "lpt1_kspace": 2 * c * nh,                  # read noise, write phi_k
         "lpt2_kspace": 2 * c * nh,                  # read/write in place
+        "lpt_kspace": 2 * c * nh,                   # Poisson kernel where not fused into the FFT
         "peaks": r * ng,                            # read delta (count pass)
         "peaks_write": r * ng,                      # read delta again, write peaks
         "lpt1_grad": r * ng + 3 * r * ng,           # read phi, write S1

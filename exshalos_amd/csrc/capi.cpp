@@ -301,12 +301,19 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
                           void* delta_out, void* S_out, void* V_out) {
   const size_t ng = (size_t)b->ng;
   T* delta = delta_out ? (T*)delta_out : (T*)ctx_buf(c, BUF_R1, ng * sizeof(T));
-  launch_field<T>(c, b, modes, 1e5, delta, nullptr);
+  // delta_k is kept for the LPT, which divides it by -k^2 as the reference
+  // does with its deltak array (box.c:17-44)
+  const size_t nhalf = (size_t)b->nd[0] * b->nd[1] * b->nz2;
+  void* deltak = ctx_buf(c, BUF_CK1, nhalf * 2 * sizeof(T));
+  launch_field<T>(c, b, modes, 1e5, delta, deltak);
   long long nh = run_find_halos<T>(c, b, delta, o);
   T* S = S_out ? (T*)S_out : (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
   T* V = out_vel ? (V_out ? (T*)V_out : (T*)ctx_buf(c, BUF_V, ng * 3 * sizeof(T))) : nullptr;
   // the field's BUF_R1 copy may be reused by the 2LPT source from here on
-  launch_lpt<T>(c, b, modes, k_smooth, do_2lpt, 0, S, V);
+  exh_field_src ks{};
+  ks.kind = EXH_SRC_DELTAK;
+  ks.data = deltak;
+  launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V);
   run_halo_posh<T>(c, b, S, V, 0);
   c->halo.have_posh = true;
   c->halo.have_velh = V != nullptr;

@@ -24,6 +24,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "kspace.hpp"
 
 namespace exh {
 
@@ -146,14 +147,52 @@ __device__ __forceinline__ void dft(C2* v) {
   }
 }
 
+// Operators applied to each point as the first stage loads it (point index
+// n along the axis; the column is bound once per thread).
+struct OpNone {
+  __device__ void bind(long long) {}
+  template <typename C2>
+  __device__ C2 operator()(C2 v, int) const {
+    return v;
+  }
+};
+
+// phi_k = c_k * pre / (-k^2) * scale on the pass along axis 0, whose columns
+// are the (j, k) pairs of a plane: the expression of the standalone Poisson
+// kernel, so the values are bit-identical.
+template <typename T>
+struct OpPoisson {
+  int n0, n1, n2, nz2;
+  T kl0, kl1, kl2, k_smooth, pre, scale;
+  int j, k;
+  __device__ void bind(long long col) {
+    j = (int)(col / nz2);
+    k = (int)(col - (long long)j * nz2);
+  }
+  template <typename C2>
+  __device__ C2 operator()(C2 v, int i) const {
+    T fact;
+    C2 r;
+    if (lpt_fact<T>(n0, n1, n2, kl0, kl1, kl2, k_smooth, i, j, k, fact)) {
+      r.x = v.x * pre / fact * scale;
+      r.y = v.y * pre / fact * scale;
+    } else {
+      r.x = 0;
+      r.y = 0;
+    }
+    return r;
+  }
+};
+
 // One Stockham stage over the workgroup's BZ columns: butterfly j of a
 // column reads points j + r N/R, multiplies by W_{NS R}^{(j mod NS) r} and
-// writes points (j / NS) NS R + (j mod NS) + r NS.
+// writes points (j / NS) NS R + (j mod NS) + r NS.  The first stage reads
+// `gs` (through the load operator), the last writes `gd`.
 template <typename C2, int N, int R, int NS, int BZ, int LD, bool INV, bool FROM_GLOBAL,
-          bool TO_GLOBAL>
-__device__ __forceinline__ void fft_stage(C2* __restrict__ g, long long stride, bool ok,
+          bool TO_GLOBAL, typename Op>
+__device__ __forceinline__ void fft_stage(const C2* gs, C2* gd, long long stride, bool ok,
                                           C2* __restrict__ lds, const C2* __restrict__ tw,
-                                          int c, int grp) {
+                                          int c, int grp, const Op& op) {
   constexpr int G = FFT_THREADS / BZ;  // threads per column
   constexpr int NBF = N / R;           // butterflies per column
   constexpr int PER = (NBF + G - 1) / G;
@@ -167,7 +206,7 @@ __device__ __forceinline__ void fft_stage(C2* __restrict__ g, long long stride, 
         const int n = j + r * NBF;
         if constexpr (FROM_GLOBAL) {
           if (ok) {
-            v[p][r] = g[(long long)n * stride];
+            v[p][r] = op(gs[(long long)n * stride], n);
           } else {
             v[p][r].x = 0;
             v[p][r].y = 0;
@@ -194,7 +233,7 @@ __device__ __forceinline__ void fft_stage(C2* __restrict__ g, long long stride, 
       for (int r = 0; r < R; r++) {
         const int n = base + r * NS;
         if constexpr (TO_GLOBAL) {
-          if (ok) g[(long long)n * stride] = v[p][r];
+          if (ok) gd[(long long)n * stride] = v[p][r];
         } else {
           lds[n * LD + c] = v[p][r];
         }
@@ -204,14 +243,16 @@ __device__ __forceinline__ void fft_stage(C2* __restrict__ g, long long stride, 
   if constexpr (!TO_GLOBAL) __syncthreads();
 }
 
-template <typename C2, int N, int BZ, int LD, bool INV, int S, int NS>
-__device__ __forceinline__ void fft_stages(C2* __restrict__ g, long long stride, bool ok,
+template <typename C2, int N, int BZ, int LD, bool INV, int S, int NS, typename Op>
+__device__ __forceinline__ void fft_stages(const C2* gs, C2* gd, long long stride, bool ok,
                                            C2* __restrict__ lds, const C2* __restrict__ tw,
-                                           int c, int grp) {
+                                           int c, int grp, const Op& op) {
   constexpr int NST = fft_nstages<N>();
   constexpr int R = fft_radix<N>(S);
-  fft_stage<C2, N, R, NS, BZ, LD, INV, S == 0, S == NST - 1>(g, stride, ok, lds, tw, c, grp);
-  if constexpr (S + 1 < NST) fft_stages<C2, N, BZ, LD, INV, S + 1, NS * R>(g, stride, ok, lds, tw, c, grp);
+  fft_stage<C2, N, R, NS, BZ, LD, INV, S == 0, S == NST - 1>(gs, gd, stride, ok, lds, tw, c, grp,
+                                                             op);
+  if constexpr (S + 1 < NST)
+    fft_stages<C2, N, BZ, LD, INV, S + 1, NS * R>(gs, gd, stride, ok, lds, tw, c, grp, op);
 }
 
 template <typename T, int N>
@@ -223,12 +264,14 @@ constexpr size_t fft_lds_bytes() {
 // Workgroup w runs on XCD w mod 8; each XCD gets a contiguous range of
 // (line, column block) pairs so that neighbouring blocks, which share
 // 128-byte lines when a row is not line-aligned, meet in one L2.
-template <typename T, int N, bool INV>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(typename cplx<T>::type* data,
+template <typename T, int N, bool INV, typename Op>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(const typename cplx<T>::type* src,
+                                                          typename cplx<T>::type* dst,
                                                           long long stride, long long ncol,
                                                           long long line_stride, int nbx,
                                                           long long nblocks,
-                                                          const typename cplx<T>::type* tw) {
+                                                          const typename cplx<T>::type* tw,
+                                                          Op op) {
   using C2 = typename cplx<T>::type;
   constexpr int BZ = FftCfg<T>::BZ;
   constexpr int LD = BZ + FftCfg<T>::PAD;
@@ -246,8 +289,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(typename cplx<T>::type
   __syncthreads();
   const long long col = xb * BZ + c;
   const bool ok = col < ncol;
-  C2* g = data + line * line_stride + (ok ? col : 0);
-  fft_stages<C2, N, BZ, LD, INV, 0, 1>(g, stride, ok, lds, tws, c, grp);
+  const long long off = line * line_stride + (ok ? col : 0);
+  op.bind(ok ? col : 0);
+  fft_stages<C2, N, BZ, LD, INV, 0, 1>(src + off, dst + off, stride, ok, lds, tws, c, grp, op);
 }
 
 // W_N^k = exp(-2 pi i k / N), k < N, rounded from double; one table per
@@ -272,36 +316,55 @@ const typename cplx<T>::type* twiddles(exh_ctx* c, int N) {
   return (const C2*)d;
 }
 
-template <typename T, int N, bool INV>
-void launch_axis(exh_ctx* c, void* data, long long stride, long long ncol, long long line_stride,
-                 long long nlines) {
+template <typename T, int N, bool INV, typename Op>
+void launch_axis(exh_ctx* c, const void* src, void* dst, long long stride, long long ncol,
+                 long long line_stride, long long nlines, const Op& op) {
   constexpr int BZ = FftCfg<T>::BZ;
   const int nbx = (int)((ncol + BZ - 1) / BZ);
   const long long nblocks = (long long)nbx * nlines;
   EXH_REQUIRE(nblocks < (1LL << 31), EXH_ERR_ARG, "FFT pass too large");
   const size_t lds = fft_lds_bytes<T, N>();
-  auto kern = k_fft_axis<T, N, INV>;
+  auto kern = k_fft_axis<T, N, INV, Op>;
   EXH_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
   hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(FFT_THREADS), lds, c->stream,
-                     (typename cplx<T>::type*)data, stride, ncol, line_stride, nbx, nblocks,
-                     twiddles<T>(c, N));
+                     (const typename cplx<T>::type*)src, (typename cplx<T>::type*)dst, stride,
+                     ncol, line_stride, nbx, nblocks, twiddles<T>(c, N), op);
   EXH_LAUNCH_CHECK();
 }
 
-template <typename T, bool INV>
-void axis_dispatch(exh_ctx* c, int N, void* data, long long stride, long long ncol,
-                   long long line_stride, long long nlines) {
+template <typename T, bool INV, typename Op>
+void axis_dispatch(exh_ctx* c, int N, const void* src, void* dst, long long stride,
+                   long long ncol, long long line_stride, long long nlines, const Op& op) {
   switch (N) {
-    case 16: return launch_axis<T, 16, INV>(c, data, stride, ncol, line_stride, nlines);
-    case 32: return launch_axis<T, 32, INV>(c, data, stride, ncol, line_stride, nlines);
-    case 64: return launch_axis<T, 64, INV>(c, data, stride, ncol, line_stride, nlines);
-    case 128: return launch_axis<T, 128, INV>(c, data, stride, ncol, line_stride, nlines);
-    case 256: return launch_axis<T, 256, INV>(c, data, stride, ncol, line_stride, nlines);
-    case 512: return launch_axis<T, 512, INV>(c, data, stride, ncol, line_stride, nlines);
-    case 1024: return launch_axis<T, 1024, INV>(c, data, stride, ncol, line_stride, nlines);
+#define EXH_AXIS_CASE(NN) \
+  case NN: return launch_axis<T, NN, INV>(c, src, dst, stride, ncol, line_stride, nlines, op);
+    EXH_AXIS_CASE(16)
+    EXH_AXIS_CASE(32)
+    EXH_AXIS_CASE(64)
+    EXH_AXIS_CASE(128)
+    EXH_AXIS_CASE(256)
+    EXH_AXIS_CASE(512)
+    EXH_AXIS_CASE(1024)
+#undef EXH_AXIS_CASE
     default: EXH_REQUIRE(false, EXH_ERR_ARG, "no strided FFT pass of length %d", N);
   }
+}
+
+template <typename T>
+OpPoisson<T> poisson_op(const PoissonParams& p) {
+  OpPoisson<T> o{};
+  o.n0 = p.n0;
+  o.n1 = p.n1;
+  o.n2 = p.n2;
+  o.nz2 = p.n2 / 2 + 1;
+  o.kl0 = (T)p.kl0;
+  o.kl1 = (T)p.kl1;
+  o.kl2 = (T)p.kl2;
+  o.k_smooth = (T)p.k_smooth;
+  o.pre = (T)p.pre;
+  o.scale = (T)p.scale;
+  return o;
 }
 
 }  // namespace
@@ -310,17 +373,29 @@ bool fft_axis_supported(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) ==
 
 void fft_axis_pass(exh_ctx* c, int prec, bool inverse, int N, void* data, long long stride,
                    long long ncol, long long line_stride, long long nlines) {
+  const OpNone op;
   if (prec == 4) {
     if (inverse)
-      axis_dispatch<float, true>(c, N, data, stride, ncol, line_stride, nlines);
+      axis_dispatch<float, true>(c, N, data, data, stride, ncol, line_stride, nlines, op);
     else
-      axis_dispatch<float, false>(c, N, data, stride, ncol, line_stride, nlines);
+      axis_dispatch<float, false>(c, N, data, data, stride, ncol, line_stride, nlines, op);
   } else {
     if (inverse)
-      axis_dispatch<double, true>(c, N, data, stride, ncol, line_stride, nlines);
+      axis_dispatch<double, true>(c, N, data, data, stride, ncol, line_stride, nlines, op);
     else
-      axis_dispatch<double, false>(c, N, data, stride, ncol, line_stride, nlines);
+      axis_dispatch<double, false>(c, N, data, data, stride, ncol, line_stride, nlines, op);
   }
+}
+
+void fft_axis0_inverse_poisson(exh_ctx* c, int prec, const void* src, void* dst,
+                               const PoissonParams& p) {
+  const long long plane = (long long)p.n1 * (p.n2 / 2 + 1);
+  if (prec == 4)
+    axis_dispatch<float, true>(c, p.n0, src, dst, plane, plane, (long long)p.n0 * plane, 1,
+                               poisson_op<float>(p));
+  else
+    axis_dispatch<double, true>(c, p.n0, src, dst, plane, plane, (long long)p.n0 * plane, 1,
+                                poisson_op<double>(p));
 }
 
 void fft_axis_release(exh_ctx* c) {

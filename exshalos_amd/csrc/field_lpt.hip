@@ -16,6 +16,7 @@
 // march planes through LDS.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "kspace.hpp"
 
 namespace exh {
 
@@ -163,23 +164,6 @@ __global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale,
       out[id] = {re * scale, im * scale};
     }
   }
-}
-
-// -k^2 as the LPT code evaluates it (lpt.c:14-31): pow() in double, the
-// modulus rounded to fft_real, then squared in double.
-template <typename T>
-__device__ inline bool lpt_fact(int n0, int n1, int n2, T kl0, T kl1, T kl2, T k_smooth, int i,
-                                int j, int k, T& fact) {
-  T kx = (2 * i < n0) ? (T)i * kl0 : (T)(i - n0) * kl0;
-  T ky = (2 * j < n1) ? (T)j * kl1 : (T)(j - n1) * kl1;
-  T kz = (T)k * kl2;
-  if (k == n2 / 2) kz = (T)(-(T)n2 / 2.0 * (double)kl2);
-  T kmod = (T)sqrt((double)kx * kx + (double)ky * ky + (double)kz * kz);
-  if ((kx != (T)0 || ky != (T)0 || kz != (T)0) && kmod <= k_smooth) {
-    fact = (T)(-((double)kmod * (double)kmod));
-    return true;
-  }
-  return false;
 }
 
 // phi_k = delta_k / (-k^2) * scale with delta_k taken from the white-noise
@@ -582,27 +566,41 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
   T* phi = (T*)ctx_buf(c, BUF_R0, ng * sizeof(T));
   T kl0 = (T)b->kl[0], kl1 = (T)b->kl[1], kl2 = (T)b->kl[2];
   T Normx = (T)b->Normx, Normk = (T)b->Normk;
-  {
-    TimedScope ts(c, "lpt1_kspace");
-    if (s->kind == EXH_SRC_MODES) {
+  // phi_k = c_k pre / (-k^2) Normx, then phi(x): the division is fused into
+  // the first inverse FFT pass where the strided passes cover the grid
+  PoissonParams pp{n0, n1, n2, (double)kl0, (double)kl1, (double)kl2, (double)(T)k_smooth,
+                   1.0, (double)Normx};
+  auto poisson_c2r = [&](const void* src, T* out) {
+    if (fft_c2r_poisson(c, b->precision, src, ck, out, pp)) return;
+    {
+      TimedScope ts(c, "lpt_kspace");
+      if (src != ck)
+        EXH_CHECK_HIP(hipMemcpyAsync(ck, src, nh * sizeof(C2), hipMemcpyDeviceToDevice,
+                                     c->stream));
+      hipLaunchKernelGGL(k_poisson_grid<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, n0,
+                         n1, n2, kl0, kl1, kl2, (T)k_smooth, (T)pp.pre, Normx, fast_div(b->nz2),
+                         fast_div(n1), ck);
+      EXH_LAUNCH_CHECK();
+    }
+    fft_c2r(c, n0, n1, n2, b->precision, 1, ck, out);
+  };
+  if (s->kind == EXH_SRC_MODES) {
+    {
+      TimedScope ts(c, "lpt1_kspace");
       auto a = make_field_args<T>(c, b, s, 1e5);
       hipLaunchKernelGGL(k_poisson_modes<T>, dim3(row_grid((long long)n0 * n1)), dim3(256),
-                         spline_lds(a), c->stream,
-                         a, (T)k_smooth, Normx, ck);
-    } else if (s->kind == EXH_SRC_DELTAK) {
-      EXH_CHECK_HIP(hipMemcpyAsync(ck, s->data, nh * sizeof(C2), hipMemcpyDeviceToDevice,
-                                   c->stream));
-      hipLaunchKernelGGL(k_poisson_grid<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, n0,
-                         n1, n2, kl0, kl1, kl2, (T)k_smooth, (T)1, Normx, fast_div(b->nz2), fast_div(n1), ck);
-    } else {
-      // Compute_Denk (density_grid.c:220-235): r2c then * Normk
-      fft_r2c(c, n0, n1, n2, b->precision, 1, s->data, ck);
-      hipLaunchKernelGGL(k_poisson_grid<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, n0,
-                         n1, n2, kl0, kl1, kl2, (T)k_smooth, Normk, Normx, fast_div(b->nz2), fast_div(n1), ck);
+                         spline_lds(a), c->stream, a, (T)k_smooth, Normx, ck);
+      EXH_LAUNCH_CHECK();
     }
-    EXH_LAUNCH_CHECK();
+    fft_c2r(c, n0, n1, n2, b->precision, 1, ck, phi);
+  } else if (s->kind == EXH_SRC_DELTAK) {
+    poisson_c2r(s->data, phi);  // c_k = delta_k as given (pre = 1)
+  } else {
+    // Compute_Denk (density_grid.c:220-235): r2c then * Normk
+    fft_r2c(c, n0, n1, n2, b->precision, 1, s->data, ck);
+    pp.pre = (double)Normk;
+    poisson_c2r(ck, phi);
   }
-  fft_c2r(c, n0, n1, n2, b->precision, 1, ck, phi);
   const double inv60 = 1.0 / (60.0 * (double)(T)b->Lc);
   const TileGeom tg = tile_geom(n0, n1, n2);
   const unsigned tblocks = (unsigned)(tg.ntk * tg.ntj * tg.nti);
@@ -622,13 +620,8 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
       EXH_LAUNCH_CHECK();
     }
     fft_r2c(c, n0, n1, n2, b->precision, 1, src, ck);
-    {
-      TimedScope ts(c, "lpt2_kspace");
-      hipLaunchKernelGGL(k_poisson_grid<T>, dim3(stream_grid(nh)), dim3(256), 0, c->stream, n0,
-                         n1, n2, kl0, kl1, kl2, (T)k_smooth, Normk, Normx, fast_div(b->nz2), fast_div(n1), ck);
-      EXH_LAUNCH_CHECK();
-    }
-    fft_c2r(c, n0, n1, n2, b->precision, 1, ck, src);
+    pp.pre = (double)Normk;
+    poisson_c2r(ck, src);
     const double coef = -3.0 / 7.0 * std::pow((double)(T)b->Omz, -1.0 / 143);
     T f2 = (T)(2.0 * std::pow((T)b->Omz, 6.0 / 11.0) * (T)b->Hz / (1.0 + (T)b->redshift));
     {

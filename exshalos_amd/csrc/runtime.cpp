@@ -9,6 +9,7 @@
 #include <cstdlib>
 
 #include "common.hpp"
+#include "kspace.hpp"
 
 namespace exh {
 
@@ -163,6 +164,17 @@ void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, 
   fft_axis_pass(c, prec, true, n0, in, plane, plane, n0 * plane, batch);
   fft_axis_pass(c, prec, true, n1, in, nz2, nz2, plane, (long long)batch * n0);
   run_plan(c, get_plan(c, n0, n1, n2, prec, 1, (long long)batch * n0 * n1, 1), in, out);
+}
+
+bool fft_c2r_poisson(exh_ctx* c, int prec, const void* src, void* work, void* out,
+                     const PoissonParams& p) {
+  if (!use_axis_passes(p.n0, p.n1)) return false;
+  TimedScope ts(c, "fft_c2r");
+  const long long nz2 = p.n2 / 2 + 1, plane = (long long)p.n1 * nz2;
+  fft_axis0_inverse_poisson(c, prec, src, work, p);
+  fft_axis_pass(c, prec, true, p.n1, work, nz2, nz2, plane, p.n0);
+  run_plan(c, get_plan(c, p.n0, p.n1, p.n2, prec, 1, (long long)p.n0 * p.n1, 1), work, out);
+  return true;
 }
 
 void fft_release(exh_ctx* c) {
