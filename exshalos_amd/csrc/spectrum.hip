@@ -195,22 +195,32 @@ __device__ inline int rel_cell(int c, int qc, int nd) {
   return d;
 }
 
-__device__ inline int wrap_cell(int c, int nd) {
-  c %= nd;
-  return c < 0 ? c + nd : c;
+// Cells handled here lie within (-nd/2, 3nd/2) (a stencil neighbour, or a
+// box cell a relative offset of less than nd/2 from a tile centre), so one
+// conditional step wraps them exactly.
+__device__ inline int wrap_cell(int c, int nd) { return c < 0 ? c + nd : (c >= nd ? c - nd : c); }
+
+// Scaled coordinate of a wrapped position, the one division of a window
+// evaluation: x / Ld for NGP and CIC, x / Ld - 0.5 in double for TSC.
+template <typename T, int WIN>
+__device__ inline auto scale_q(T x, T Ld) {
+  if constexpr (WIN == 5)
+    return (double)x / (double)Ld - 0.5;
+  else
+    return x / Ld;
 }
 
-// Cells and weights of one particle along one axis for window WIN.
-template <typename T, int WIN>
-__device__ inline int axis_stencil(T x, T Ld, int nd, int* cell, T* w) {
-  if (WIN == 1) {
-    int xt = (int)floor(x / Ld);
+// Cells and weights along one axis from the scaled coordinate, with the
+// expressions of dep_ngp/dep_cic/dep_tsc.
+template <typename T, int WIN, typename Q>
+__device__ inline int stencil_q(Q q, int nd, int* cell, T* w) {
+  if constexpr (WIN == 1) {
+    int xt = (int)floor(q);
     if (xt == nd) xt -= 1;
     cell[0] = xt;
     w[0] = (T)1;
     return 1;
-  } else if (WIN == 2) {
-    T q = x / Ld;
+  } else if constexpr (WIN == 2) {
     int xt = (int)floor(q);
     if (xt == nd) xt -= 1;
     T dx = (T)((double)q - ((double)xt + 0.5));
@@ -227,10 +237,10 @@ __device__ inline int axis_stencil(T x, T Ld, int nd, int* cell, T* w) {
     w[1] = dx;
     return 2;
   } else {
-    double u = (double)x / (double)Ld - 0.5;
-    double cf = floor(u + 0.5);
-    double d = u - cf;
-    int c = (int)cf;
+    const double u = q;
+    const double cf = floor(u + 0.5);
+    const double d = u - cf;
+    const int c = (int)cf;
     cell[0] = wrap_cell(c - 1, nd);
     cell[1] = wrap_cell(c, nd);
     cell[2] = wrap_cell(c + 1, nd);
@@ -239,6 +249,12 @@ __device__ inline int axis_stencil(T x, T Ld, int nd, int* cell, T* w) {
     w[2] = (T)(0.5 * (0.5 + d) * (0.5 + d));
     return 3;
   }
+}
+
+// Cells and weights of one particle along one axis for window WIN.
+template <typename T, int WIN>
+__device__ inline int axis_stencil(T x, T Ld, int nd, int* cell, T* w) {
+  return stencil_q<T, WIN>(scale_q<T, WIN>(x, Ld), nd, cell, w);
 }
 
 template <int N>
@@ -382,6 +398,157 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
   }
 }
 
+// Window deposit: the tile's particles go into a fixed LDS window of
+// E = W + 2 WIN_HALO cells per axis (W = grid cells per tile side), placed at
+// the tile's mean displacement.  The mean is taken over one sampled
+// particle per wave, so no pass over all particles precedes the deposit;
+// each particle's wrap and division are evaluated once per sub-grid.  The
+// rare stencil that leaves the window goes straight to the grid.  The
+// window is flushed row by row, lanes along z, one global atomic per
+// touched cell.
+static constexpr int WIN_THREADS = 1024;
+static constexpr int WIN_HALO = 4;
+static constexpr int WIN_PPT = TILE_L * TILE_L * TILE_L / WIN_THREADS;
+
+template <typename T, int WIN>
+__global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
+    const T* pos, const T* vel, const T* mass, int nL, int nd, int W, T L, T Lb, int direction,
+    T rsd, int nsub, T* grid, unsigned long long* counts) {
+  constexpr int NW = WIN == 1 ? 1 : (WIN == 2 ? 2 : 3);
+  constexpr int NWAVE = WIN_THREADS / 64;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* box = (double*)lds_raw;
+  __shared__ float samp[3][NWAVE];
+  __shared__ int wo_s[3];
+  __shared__ unsigned long long nvalid;
+  const size_t ng = (size_t)nd * nd * nd;
+  const T Ld = L / (T)nd;
+  const int nt = nL / TILE_L;
+  const int E = W + 2 * WIN_HALO, sx = E * E;
+  const int tz = blockIdx.x % nt, ty = (blockIdx.x / nt) % nt, tx = blockIdx.x / (nt * nt);
+  // tile centre in grid cells: the middle of the grid tile of the same index
+  const int qc[3] = {tx * W + W / 2, ty * W + W / 2, tz * W + W / 2};
+  if (threadIdx.x == 0) nvalid = 0;
+  T xs[WIN_PPT][3];
+  T ms[WIN_PPT];
+  bool ok[WIN_PPT];
+  unsigned long long myvalid = 0;
+#pragma unroll
+  for (int p = 0; p < WIN_PPT; p++) {
+    const int lin = threadIdx.x + p * WIN_THREADS;
+    const int c = lin % TILE_L, b = (lin / TILE_L) % TILE_L, a = lin / (TILE_L * TILE_L);
+    const size_t j = ((size_t)(tx * TILE_L + a) * nL + (ty * TILE_L + b)) * nL + (tz * TILE_L + c);
+    T x[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    if (direction >= 0) x[direction] = wrap_while<T>(x[direction] + vel[3 * j + direction] * rsd, L);
+    ok[p] = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
+#pragma unroll
+    for (int q = 0; q < 3; q++) xs[p][q] = x[q];
+    ms[p] = mass ? mass[j] : (T)1;
+    myvalid += ok[p];
+  }
+  // displacement (in cells) of each wave's first particle: its cell relative
+  // to the tile centre minus its lattice point's offset from the centre
+  if ((threadIdx.x & 63) == 0) {
+    const int lin = threadIdx.x;
+    const int lat[3] = {lin / (TILE_L * TILE_L), (lin / TILE_L) % TILE_L, lin % TILE_L};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      float r = 1e30f;
+      if (ok[0]) {
+        int cc = (int)floor(wrap_while<T>(xs[0][q], L) / Ld);
+        cc = cc < nd ? cc : nd - 1;
+        r = (float)rel_cell(cc, qc[q], nd) -
+            ((lat[q] + 0.5f) * (float)nd / (float)nL - 0.5f * (float)W);
+      }
+      samp[q][threadIdx.x >> 6] = r;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    float sum = 0.f;
+    int n = 0;
+    for (int w = 0; w < NWAVE; w++) {
+      const float v = samp[threadIdx.x][w];
+      if (v < 1e29f) {
+        sum += v;
+        n++;
+      }
+    }
+    const float mean = n ? sum / (float)n : 0.f;
+    wo_s[threadIdx.x] = (int)floorf(mean + 0.5f) - W / 2 - WIN_HALO;
+  }
+  __syncthreads();
+  const int wo[3] = {wo_s[0], wo_s[1], wo_s[2]};
+  const int vol = E * sx;
+  const T shift = (T)(-(double)Lb / 2.0);
+  for (int s = 0; s < nsub; s++) {
+    T* g = grid + (size_t)s * ng;
+    for (int v = threadIdx.x; v < vol; v += WIN_THREADS) box[v] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < WIN_PPT; p++) {
+      if (!ok[p]) continue;
+      int cell[3][3], off[3][3];
+      T w[3][3];
+      bool in = true;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const T pa = wrap_while<T>(s == 0 ? xs[p][q] : xs[p][q] + shift, L);
+        stencil_q<T, WIN>(scale_q<T, WIN>(pa, Ld), nd, cell[q], w[q]);
+        const int st = q == 0 ? sx : (q == 1 ? E : 1);
+#pragma unroll
+        for (int u = 0; u < NW; u++) {
+          const int l = rel_cell(cell[q][u], qc[q], nd) - wo[q];
+          in = in && (unsigned)l < (unsigned)E;
+          off[q][u] = l * st;
+        }
+      }
+      if (in) {
+#pragma unroll
+        for (int u = 0; u < NW; u++) {
+#pragma unroll
+          for (int b = 0; b < NW; b++) {
+            const T wab = ms[p] * w[0][u] * w[1][b];
+            const int o = off[0][u] + off[1][b];
+#pragma unroll
+            for (int c = 0; c < NW; c++) atomicAdd(&box[o + off[2][c]], (double)(wab * w[2][c]));
+          }
+        }
+      } else {
+        for (int u = 0; u < NW; u++)
+          for (int b = 0; b < NW; b++) {
+            const T wab = ms[p] * w[0][u] * w[1][b];
+            for (int c = 0; c < NW; c++)
+              atomic_add_t(g + ((size_t)cell[0][u] * nd + cell[1][b]) * nd + cell[2][c],
+                           wab * w[2][c]);
+          }
+      }
+    }
+    __syncthreads();
+    // flush: (y, z) rows over the threads, z fastest in 32-lane groups
+    const int lz0 = threadIdx.x & 31;
+    for (int ly = threadIdx.x >> 5; ly < E; ly += WIN_THREADS / 32) {
+      const int cy = wrap_cell(qc[1] + wo[1] + ly, nd);
+      for (int lz = lz0; lz < E; lz += 32) {
+        const int cz = wrap_cell(qc[2] + wo[2] + lz, nd);
+        const double* bp = box + ly * E + lz;
+        int cx = wrap_cell(qc[0] + wo[0], nd);
+        for (int lx = 0; lx < E; lx++) {
+          const double val = bp[lx * sx];
+          if (val != 0.0) atomic_add_t(g + ((size_t)cx * nd + cy) * nd + cz, (T)val);
+          cx = cx + 1 == nd ? 0 : cx + 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!mass && counts) {
+    atomicAdd(&nvalid, myvalid);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&counts[0], nvalid);
+  }
+}
+
 template <typename T>
 __global__ void k_zero(T* g, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -423,6 +590,16 @@ static int deposit_lds_bytes() {
   return v;
 }
 
+// The window deposit for lattice-ordered particles (EXH_DEPOSIT_WINDOW=0
+// keeps the extent-sized boxes of k_deposit_tiles).
+static bool deposit_window_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("EXH_DEPOSIT_WINDOW");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
@@ -461,10 +638,28 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
                      deposit_lds_bytes() > 0;
   {
     TimedScope ts(c, window == 2 ? "cic_deposit" : (window == 1 ? "ngp_deposit" : "tsc_deposit"));
-    if (tiles) {
+    const int W = tiles ? nd / nt : 0;
+    const int E = W + 2 * WIN_HALO;
+    const int win_lds = E * E * E * (int)sizeof(double);
+    // nd >= E keeps every window cell within (-nd, 2nd), where the flush's
+    // single-step wrap is exact
+    if (tiles && nd >= E && win_lds <= deposit_lds_bytes() && deposit_window_enabled()) {
+      const unsigned nblk = (unsigned)nt * nt * nt;
+#define EXH_DEPW(WN)                                                                             \
+  do {                                                                                           \
+    EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_deposit_window<T, WN>,                      \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, win_lds));     \
+    hipLaunchKernelGGL((k_deposit_window<T, WN>), dim3(nblk), dim3(WIN_THREADS), win_lds,        \
+                       c->stream, pos, vel, mass, nL, nd, W, Lf, Lb, direction, rsd, nsub, grid, \
+                       counts);                                                                  \
+  } while (0)
+      if (window == 1) EXH_DEPW(1);
+      if (window == 2) EXH_DEPW(2);
+      if (window == 5) EXH_DEPW(5);
+#undef EXH_DEPW
+    } else if (tiles) {
       const int lds = deposit_lds_bytes();
       const int cap = lds / (int)sizeof(double);
-      const int W = nd / nt;
       const unsigned nblk = (unsigned)nt * nt * nt;
 #define EXH_DEPT(WN)                                                                           \
   do {                                                                                         \

@@ -149,11 +149,13 @@ def test_grid_compute(prec, window, il, nd):
 @pytest.mark.parametrize("nL,nd", [(32, 32), (32, 64), (64, 64), (64, 128)])
 def test_grid_compute_lattice_tiles(prec, window, il, nL, nd):
     """Particles in Lagrangian lattice order (np = nL^3, as LPT writes
-    them) take the LDS-tiled deposit.  (32, 32): boxes fit, including one
-    that wraps the whole periodic grid (an outlier at the antipode);
-    (32, 64): 16-particle tiles span 32+ cells a side and deposit directly;
-    (64, *): four lattice tiles per axis, outlier tiles deposit directly.
-    Weighted and redshift-space variants run through the same tiles."""
+    them) take the LDS-tiled deposits.  (32, 32) and (64, 64): fixed
+    windows at each tile's mean displacement; the antipodal outlier in the
+    first tile (also the tile's sampled particle) leaves its window and
+    deposits directly.  (32, 64) and (64, 128): 16-particle tiles span 32
+    cells a side, the window exceeds the LDS and extent-sized boxes take
+    over (boxes that do not fit deposit directly).  Weighted and
+    redshift-space variants run through the same tiles."""
     set_precision(prec)
     R = np.float32 if prec == 4 else np.float64
     L = 128.0
