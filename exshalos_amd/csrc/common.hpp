@@ -180,6 +180,11 @@ bool fft_axis_supported(int n);
 void fft_axis_pass(exh_ctx* c, int prec, bool inverse, int N, void* data, long long stride,
                    long long ncol, long long line_stride, long long nlines);
 void fft_axis_release(exh_ctx* c);
+// real transforms along the contiguous axis of length n (fft.hip): real rows
+// of pitch n <-> half-complex rows of pitch n/2+1
+bool fft_rows_supported(int n);
+void fft_rows_pass(exh_ctx* c, int prec, bool inverse, int N, const void* in, void* out,
+                   long long nrows);
 
 // event timing around a launch sequence
 struct TimedScope {

@@ -367,6 +367,180 @@ OpPoisson<T> poisson_op(const PoissonParams& p) {
   return o;
 }
 
+// ---------------------------------------------------------------------------
+// Real transforms along the contiguous axis (length N, M = N/2): the N reals
+// of a row read as M complex points z_n = x_2n + i x_2n+1, an M-point
+// Stockham FFT in LDS, and the even/odd split
+//   forward:  X_k = (A + B)/2 - i W_N^k (A - B)/2,  A = Z_k, B = conj(Z_{M-k})
+//   inverse:  Z_k = (A + B) + i W_N^-k (A - B),      A = X_k, B = conj(X_{M-k})
+// (k = 0..M forward, 0..M-1 inverse; the inverse ignores the imaginary
+// parts of X_0 and X_M as a real inverse transform does).  A workgroup owns
+// RB rows, G = min(M/8, 64) threads per row, lanes along the row, so every
+// global access is contiguous.  LDS rows are padded by one element in nine.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int rpad(int n) { return n + (n >> 3); }
+
+template <typename C2, int M, int R, int NS, int G, bool INV, bool FROM_GLOBAL, bool TO_GLOBAL>
+__device__ __forceinline__ void row_stage(const C2* gin, C2* gout, bool ok, C2* __restrict__ lrow,
+                                          const C2* __restrict__ tw, int j0, int twstep) {
+  constexpr int NBF = M / R;
+  constexpr int PER = (NBF + G - 1) / G;
+  C2 v[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int j = j0 + p * G;
+    if (NBF % G == 0 || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int n = j + r * NBF;
+        v[p][r] = FROM_GLOBAL ? gin[n] : lrow[rpad(n)];
+      }
+    }
+  }
+  if constexpr (!FROM_GLOBAL) __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int j = j0 + p * G;
+    if (NBF % G == 0 || j < NBF) {
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; r++)
+          v[p][r] = ctw<INV>(v[p][r], tw[k * r * (M / (NS * R)) * twstep]);
+      }
+      dft<R, INV>(v[p]);
+      const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int n = base + r * NS;
+        if constexpr (TO_GLOBAL) {
+          if (ok) gout[n] = v[p][r];
+        } else
+          lrow[rpad(n)] = v[p][r];
+      }
+    }
+  }
+  if constexpr (!TO_GLOBAL) __syncthreads();
+}
+
+template <typename C2, int M, int G, bool INV, bool FIRST_GLOBAL, bool LAST_GLOBAL, int S, int NS>
+__device__ __forceinline__ void row_stages(const C2* gin, C2* gout, bool ok, C2* __restrict__ lrow,
+                                           const C2* __restrict__ tw, int j0, int twstep) {
+  constexpr int NST = fft_nstages<M>();
+  constexpr int R = fft_radix<M>(S);
+  row_stage<C2, M, R, NS, G, INV, FIRST_GLOBAL && S == 0, LAST_GLOBAL && S == NST - 1>(
+      gin, gout, ok, lrow, tw, j0, twstep);
+  if constexpr (S + 1 < NST)
+    row_stages<C2, M, G, INV, FIRST_GLOBAL, LAST_GLOBAL, S + 1, NS * R>(gin, gout, ok, lrow, tw,
+                                                                      j0, twstep);
+}
+
+template <int M>
+struct RowCfg {
+  static constexpr int G = M / 8 < 64 ? M / 8 : 64;  // threads per row
+  static constexpr int RB = FFT_THREADS / G;          // rows per workgroup
+  static constexpr int LDR = M + M / 8;               // padded LDS row
+};
+
+// r2c: real rows (pitch 2M) -> half-complex rows (pitch M+1).
+// c2r: half-complex rows (pitch M+1) -> real rows (pitch 2M), unnormalised.
+template <typename T, int M, bool INV>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(const void* in, void* out,
+                                                          long long nrows,
+                                                          const typename cplx<T>::type* tw) {
+  using C2 = typename cplx<T>::type;
+  using RC = RowCfg<M>;
+  constexpr int N = 2 * M;
+  extern __shared__ __align__(16) unsigned char fft_lds_raw[];
+  C2* lds = reinterpret_cast<C2*>(fft_lds_raw);
+  C2* tws = lds + RC::RB * RC::LDR;
+  const int t = threadIdx.x, rl = t / RC::G, j0 = t % RC::G;
+  for (int i = t; i < N; i += FFT_THREADS) tws[i] = tw[i];
+  const long long row = (long long)blockIdx.x * RC::RB + rl;
+  const bool ok = row < nrows;
+  const long long rr = ok ? row : 0;
+  C2* lrow = lds + rl * RC::LDR;
+  if constexpr (!INV) {
+    const C2* zin = reinterpret_cast<const C2*>(static_cast<const T*>(in)) + rr * M;
+    C2* xout = static_cast<C2*>(out) + rr * (M + 1);
+    __syncthreads();  // twiddles
+    // M-point forward FFT of z (the twiddles W_M^k are W_N^2k)
+    row_stages<C2, M, RC::G, false, true, false, 0, 1>(zin, nullptr, ok, lrow, tws, j0, 2);
+    for (int k = j0; k <= M; k += RC::G) {
+      const C2 A = lrow[rpad(k == M ? 0 : k)];
+      C2 B = lrow[rpad(k == 0 ? 0 : M - k)];
+      B.y = -B.y;
+      C2 E, D, O, X;
+      E.x = (A.x + B.x) * (T)0.5;
+      E.y = (A.y + B.y) * (T)0.5;
+      D.x = (A.x - B.x) * (T)0.5;
+      D.y = (A.y - B.y) * (T)0.5;
+      O.x = D.y;  // -i D
+      O.y = -D.x;
+      const C2 WO = ctw<false>(O, tws[k]);
+      X.x = E.x + WO.x;
+      X.y = E.y + WO.y;
+      if (ok) xout[k] = X;
+    }
+  } else {
+    const C2* xin = static_cast<const C2*>(in) + rr * (M + 1);
+    C2* zout = reinterpret_cast<C2*>(static_cast<T*>(out)) + rr * M;
+    __syncthreads();  // twiddles
+    for (int k = j0; k < M; k += RC::G) {
+      C2 A = ok ? xin[k] : C2{};
+      C2 B = ok ? xin[M - k] : C2{};
+      if (k == 0) {
+        A.y = 0;
+        B.y = 0;
+      }
+      B.y = -B.y;
+      C2 S, D, Z;
+      S.x = A.x + B.x;
+      S.y = A.y + B.y;
+      D.x = A.x - B.x;
+      D.y = A.y - B.y;
+      const C2 WD = ctw<true>(D, tws[k]);  // W_N^-k (A - B)
+      Z.x = S.x - WD.y;                      // + i W D
+      Z.y = S.y + WD.x;
+      lrow[rpad(k)] = Z;
+    }
+    __syncthreads();
+    // M-point inverse FFT, the last stage writes the real row
+    row_stages<C2, M, RC::G, true, false, true, 0, 1>(nullptr, zout, ok, lrow, tws, j0, 2);
+  }
+}
+
+template <typename T, int M, bool INV>
+void launch_rows(exh_ctx* c, const void* in, void* out, long long nrows) {
+  using C2 = typename cplx<T>::type;
+  using RC = RowCfg<M>;
+  const long long nblk = (nrows + RC::RB - 1) / RC::RB;
+  EXH_REQUIRE(nblk < (1LL << 31), EXH_ERR_ARG, "FFT row pass too large");
+  const size_t lds = ((size_t)RC::RB * RC::LDR + 2 * M) * sizeof(C2);
+  auto kern = k_fft_rows<T, M, INV>;
+  EXH_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(FFT_THREADS), lds, c->stream, in, out, nrows,
+                     twiddles<T>(c, 2 * M));
+  EXH_LAUNCH_CHECK();
+}
+
+template <typename T, bool INV>
+void rows_dispatch(exh_ctx* c, int N, const void* in, void* out, long long nrows) {
+  switch (N / 2) {
+#define EXH_ROWS_CASE(MM) \
+  case MM: return launch_rows<T, MM, INV>(c, in, out, nrows);
+    EXH_ROWS_CASE(16)
+    EXH_ROWS_CASE(32)
+    EXH_ROWS_CASE(64)
+    EXH_ROWS_CASE(128)
+    EXH_ROWS_CASE(256)
+    EXH_ROWS_CASE(512)
+#undef EXH_ROWS_CASE
+    default: EXH_REQUIRE(false, EXH_ERR_ARG, "no real row transform of length %d", N);
+  }
+}
+
 }  // namespace
 
 bool fft_axis_supported(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
@@ -396,6 +570,23 @@ void fft_axis0_inverse_poisson(exh_ctx* c, int prec, const void* src, void* dst,
   else
     axis_dispatch<double, true>(c, p.n0, src, dst, plane, plane, (long long)p.n0 * plane, 1,
                                 poisson_op<double>(p));
+}
+
+bool fft_rows_supported(int n) { return n >= 32 && n <= 1024 && (n & (n - 1)) == 0; }
+
+void fft_rows_pass(exh_ctx* c, int prec, bool inverse, int N, const void* in, void* out,
+                   long long nrows) {
+  if (prec == 4) {
+    if (inverse)
+      rows_dispatch<float, true>(c, N, in, out, nrows);
+    else
+      rows_dispatch<float, false>(c, N, in, out, nrows);
+  } else {
+    if (inverse)
+      rows_dispatch<double, true>(c, N, in, out, nrows);
+    else
+      rows_dispatch<double, false>(c, N, in, out, nrows);
+  }
 }
 
 void fft_axis_release(exh_ctx* c) {

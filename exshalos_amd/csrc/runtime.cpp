@@ -142,6 +142,16 @@ static bool use_axis_passes(int n0, int n1) {
   return on && fft_axis_supported(n0) && fft_axis_supported(n1);
 }
 
+// Real transforms of `rows` rows along the contiguous axis: the LDS row
+// kernels of fft.hip where they cover n2, else a batched rocFFT plan.
+static void z_real(exh_ctx* c, int n2, int prec, bool inverse, long long rows, const void* in,
+                   void* out) {
+  if (fft_rows_supported(n2))
+    fft_rows_pass(c, prec, inverse, n2, in, out, rows);
+  else
+    run_plan(c, get_plan(c, 0, 0, n2, prec, inverse ? 1 : 0, rows, 1), in, out);
+}
+
 void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void* in, void* out) {
   TimedScope ts(c, "fft_r2c");
   if (!use_axis_passes(n0, n1)) {
@@ -149,7 +159,7 @@ void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void
     return;
   }
   const long long nz2 = n2 / 2 + 1, plane = (long long)n1 * nz2;
-  run_plan(c, get_plan(c, n0, n1, n2, prec, 0, (long long)batch * n0 * n1, 1), in, out);
+  z_real(c, n2, prec, false, (long long)batch * n0 * n1, in, out);
   fft_axis_pass(c, prec, false, n1, out, nz2, nz2, plane, (long long)batch * n0);
   fft_axis_pass(c, prec, false, n0, out, plane, plane, n0 * plane, batch);
 }
@@ -163,7 +173,7 @@ void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, 
   const long long nz2 = n2 / 2 + 1, plane = (long long)n1 * nz2;
   fft_axis_pass(c, prec, true, n0, in, plane, plane, n0 * plane, batch);
   fft_axis_pass(c, prec, true, n1, in, nz2, nz2, plane, (long long)batch * n0);
-  run_plan(c, get_plan(c, n0, n1, n2, prec, 1, (long long)batch * n0 * n1, 1), in, out);
+  z_real(c, n2, prec, true, (long long)batch * n0 * n1, in, out);
 }
 
 bool fft_c2r_poisson(exh_ctx* c, int prec, const void* src, void* work, void* out,
@@ -173,7 +183,7 @@ bool fft_c2r_poisson(exh_ctx* c, int prec, const void* src, void* work, void* ou
   const long long nz2 = p.n2 / 2 + 1, plane = (long long)p.n1 * nz2;
   fft_axis0_inverse_poisson(c, prec, src, work, p);
   fft_axis_pass(c, prec, true, p.n1, work, nz2, nz2, plane, p.n0);
-  run_plan(c, get_plan(c, p.n0, p.n1, p.n2, prec, 1, (long long)p.n0 * p.n1, 1), work, out);
+  z_real(c, p.n2, prec, true, (long long)p.n0 * p.n1, work, out);
   return true;
 }
 
