@@ -153,6 +153,87 @@ __global__ void k_nonneg_write(const int* v, long long n, const long long* boff,
     if (base + q < n && v[base + q] >= 0) out[o++] = v[base + q];
 }
 
+// Order of the first round: columns of BR x BR cells in (i, j) over all k
+// ("bricks"), one after the other, and within a brick by blocks of ORD_RB
+// consecutive ranks (similar densities, hence similar growth work per
+// wave).  A counting sort: per rank block, a histogram over bricks; one
+// exclusive scan over (brick, block); then each block scatters its ranks.
+static constexpr int ORD_RB = 4096;
+static constexpr int ORD_MAXB = 8192;  // bricks (LDS ints)
+
+__device__ inline int brick_of(int cell, int n1, int n2, int bsh, int nby) {
+  const int r = cell / n2;
+  const int j = r % n1, i = r / n1;
+  return (i >> bsh) * nby + (j >> bsh);
+}
+
+__global__ void k_brick_hist(const int* ccell, long long nc, int n1, int n2, int bsh, int nby,
+                             int nbr, int nblk, int* counts) {
+  __shared__ int hist[ORD_MAXB];
+  for (int b = threadIdx.x; b < nbr; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  const long long r0 = (long long)blockIdx.x * ORD_RB;
+  const long long r1 = r0 + ORD_RB < nc ? r0 + ORD_RB : nc;
+  for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x)
+    atomicAdd(&hist[brick_of(ccell[r], n1, n2, bsh, nby)], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbr; b += blockDim.x)
+    counts[(long long)b * nblk + blockIdx.x] = hist[b];
+}
+
+template <typename T>
+__global__ void k_brick_scatter(const int* ccell, const T* cden, long long nc, int n1, int n2,
+                                int bsh, int nby, int nbr, int nblk, const int* offs, int* prank,
+                                int* pcell, T* pden) {
+  __shared__ int cur[ORD_MAXB];
+  for (int b = threadIdx.x; b < nbr; b += blockDim.x)
+    cur[b] = offs[(long long)b * nblk + blockIdx.x];
+  __syncthreads();
+  const long long r0 = (long long)blockIdx.x * ORD_RB;
+  const long long r1 = r0 + ORD_RB < nc ? r0 + ORD_RB : nc;
+  for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    const int cell = ccell[r];
+    const int pos = atomicAdd(&cur[brick_of(cell, n1, n2, bsh, nby)], 1);
+    prank[pos] = (int)r;
+    pcell[pos] = cell;
+    pden[pos] = cden[r];
+  }
+}
+
+// The candidates' cell and density by list position (cell-order lists).
+template <typename T>
+__global__ void k_pos_gather(const int* prank, long long n, const int* ccell, const T* cden,
+                             int* pcell, T* pden) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int r = prank[t];
+    pcell[t] = ccell[r];
+    pden[t] = cden[r];
+  }
+}
+
+// In-order compaction of the candidates to retry (retry[t] >= 0), carrying
+// their cell and density along.
+template <typename T>
+__global__ void k_retry_write(const int* retry, long long n, const long long* boff,
+                              const int* cell_in, const T* den_in, int* rank_out, int* cell_out,
+                              T* den_out) {
+  __shared__ long long sh[4];
+  long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
+  long long cnt = 0;
+  for (int q = 0; q < PK_ITEMS; q++)
+    if (base + q < n && retry[base + q] >= 0) cnt++;
+  long long tot;
+  long long o = block_exclusive_256<long long>(cnt, sh, &tot) + boff[blockIdx.x];
+  for (int q = 0; q < PK_ITEMS; q++)
+    if (base + q < n && retry[base + q] >= 0) {
+      rank_out[o] = retry[base + q];
+      cell_out[o] = cell_in[base + q];
+      den_out[o] = den_in[base + q];
+      o++;
+    }
+}
+
 // first index with key >= Bmin in the ascending array (one thread)
 template <typename T>
 __global__ void k_lower_bound(const T* key, long long n, T Bmin, long long* out) {
@@ -200,13 +281,18 @@ struct GrowArgs {
   int nB;
   T Bmin;
   T negMtot;
-  const T* cden;   // candidate density by rank (0 = densest)
-  const int* ccell;  // candidate centre cell by rank
   int Nmin;
-  // per-rank results
-  int* G;          // first failing shell (read radius^2); -1 = skipped
-  int* cnt;        // final count
-  int* r2;         // final grows_tmp
+  // the round's candidates by list position: rank (0 = densest), centre
+  // cell and density, and the growth results handed from k_grow to k_commit
+  const int* prank;
+  const int* pcell;
+  const T* pden;
+  int* pG;         // first failing shell (read radius^2); -1 = skipped
+  int* pcnt;       // count
+  int* pr2;        // grows_tmp
+  // final results by rank
+  int* cnt;
+  int* r2;
   unsigned char* kept;
   int* err;
 };
@@ -219,28 +305,37 @@ __device__ inline void decode_cell(int cell, int n1, int n2, int& i, int& j, int
   i = r / n1;
 }
 
+// Workgroup w runs on XCD w mod 8: give each XCD a contiguous range of
+// blocks, so that a round's consecutive candidates (neighbours in space)
+// share one L2.
+__device__ inline int xcd_block() {
+  const unsigned w = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb / 8, rem = nb % 8, x = w % 8;
+  return (int)(x * q + (x < rem ? x : rem) + w / 8);
+}
+
 template <typename T>
-__global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long long tag_hi) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_grow(GrowArgs<T> a, int npend, unsigned long long tag_hi) {
+  int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= npend) return;
-  int rank = pend[t];
-  int cell = a.ccell[rank];
+  const int rank = a.prank[t];
+  const int cell = a.pcell[t];
   if (a.flag[cell] != FLAG_FREE) {  // already inside a denser halo
-    a.G[rank] = -1;
+    a.pG[t] = -1;
     a.kept[rank] = 0;
     return;
   }
   int ci, cj, ck;
   decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
   const double Bmin = (double)a.Bmin;
-  double den = (double)a.cden[rank], den_tmp = den;
+  double den = (double)a.pden[t], den_tmp = den;
   int count = 0, count_tmp = 1, grows = 0, grows_tmp = 0;
   while (den_tmp >= Bmin) {
     if (count < count_tmp) grows = grows_tmp;
     grows_tmp++;
     if (grows_tmp >= a.sh.stab) {
       atomicOr(a.err, 1);
-      a.G[rank] = -1;
+      a.pG[t] = -1;
       return;
     }
     den = den_tmp;
@@ -266,13 +361,13 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
   for (;;) {
     if (count >= a.nB) {
       atomicOr(a.err, 2);
-      a.G[rank] = -1;
+      a.pG[t] = -1;
       return;
     }
     if (!(den < (double)a.Bt[count] && count > 0)) break;
     if (grows < 0) {  // cannot happen while count > 0 on an unclamped table
       atomicOr(a.err, 32);
-      a.G[rank] = -1;
+      a.pG[t] = -1;
       return;
     }
     den_tmp = den;
@@ -294,9 +389,9 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
     if (count < count_tmp) grows_tmp = grows;
     grows--;
   }
-  a.G[rank] = G;
-  a.cnt[rank] = count;
-  a.r2[rank] = grows_tmp;
+  a.pG[t] = G;
+  a.pcnt[t] = count;
+  a.pr2[t] = grows_tmp;
   // claim every cell the final halo could occupy (r^2 < G)
   const unsigned long long tag = tag_hi | (unsigned long long)(0xffffffffu - (unsigned)rank);
   for (int o = 0; o < a.sh.start[G]; o++) {
@@ -305,15 +400,20 @@ __global__ void k_grow(GrowArgs<T> a, const int* pend, int npend, unsigned long 
   }
 }
 
+// retry[t]: the rank again when candidate t must retry next round, else -1
+// (compacted in order, so the next round keeps this round's order).
 template <typename T>
-__global__ void k_commit(GrowArgs<T> a, const int* pend, int npend, unsigned long long tag_hi,
-                         int* next, int* nnext, int tmax) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_commit(GrowArgs<T> a, int npend, unsigned long long tag_hi, int* retry,
+                         int tmax) {
+  int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= npend) return;
-  int rank = pend[t];
-  int G = a.G[rank];
-  if (G < 0) return;  // skipped or failed
-  int cell = a.ccell[rank];
+  const int rank = a.prank[t];
+  const int G = a.pG[t];
+  if (G < 0) {  // skipped or failed
+    retry[t] = -1;
+    return;
+  }
+  const int cell = a.pcell[t];
   int ci, cj, ck;
   decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
   // safe iff no denser pending candidate claims a cell of the read ball
@@ -330,17 +430,14 @@ __global__ void k_commit(GrowArgs<T> a, const int* pend, int npend, unsigned lon
     for (int e = 0; e < 8; e++)
       blocked |= (v[e] & 0xffffffff00000000ull) == tag_hi && (unsigned)(v[e] & 0xffffffffu) > lim;
   }
-  if (blocked) {
-    int p = atomicAdd(nnext, 1);
-    next[p] = rank;
-    return;
-  }
-  int count = a.cnt[rank];
+  retry[t] = blocked ? rank : -1;
+  if (blocked) return;
+  const int count = a.pcnt[t];
   if (count == 0) {
     a.kept[rank] = 0;
     return;
   }
-  int R2 = a.r2[rank];
+  const int R2 = a.pr2[t];
   if (isqrt_floor(R2 - 1) > tmax) {  // flag sphere would wrap (reference writes past its cube)
     atomicOr(a.err, 4);
     a.kept[rank] = 0;
@@ -355,6 +452,8 @@ __global__ void k_commit(GrowArgs<T> a, const int* pend, int npend, unsigned lon
   }
   if (keep) a.flag[cell] = (rank << 1) | 1;
   a.kept[rank] = keep ? 1 : 0;
+  a.cnt[rank] = count;
+  a.r2[rank] = R2;
 }
 
 // ---------------------------------------------------------------------------
@@ -558,6 +657,16 @@ static const std::vector<unsigned>& taus_table() {
   return t;
 }
 
+// Candidates of the first round in cell order instead of brick order
+// (EXH_HALO_ORDER=cell); the results are the same either way.
+static bool halo_cell_order() {
+  static const bool on = [] {
+    const char* e = std::getenv("EXH_HALO_ORDER");
+    return e && std::string(e) == "cell";
+  }();
+  return on;
+}
+
 template <typename T>
 long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o) {
   const int n0 = b->nd[0], n1 = b->nd[1], n2 = b->nd[2];
@@ -674,10 +783,23 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   size_t ncs = (size_t)std::max(nc, 1LL);
   size_t nps = (size_t)std::max(np, 1LL);
   long long nblk_p = (np + PK_TILE - 1) / PK_TILE;
-  size_t o_G = ctake(ncs * 4), o_c = ctake(ncs * 4), o_r = ctake(ncs * 4), o_k = ctake(ncs),
+  long long nblk_c = (nc + PK_TILE - 1) / PK_TILE;
+  // first-round order: bricks of 2^bsh x 2^bsh cells (16 x 16, wider on big
+  // grids so that the brick histogram fits in LDS)
+  int bsh = 4;
+  while ((long long)((n0 + (1 << bsh) - 1) >> bsh) * ((n1 + (1 << bsh) - 1) >> bsh) > ORD_MAXB)
+    bsh++;
+  const int nby = (n1 + (1 << bsh) - 1) >> bsh;
+  const int nbr = ((n0 + (1 << bsh) - 1) >> bsh) * nby;
+  const long long nblk_r = std::max((nc + ORD_RB - 1) / ORD_RB, 1LL);
+  size_t o_c = ctake(ncs * 4), o_r = ctake(ncs * 4), o_k = ctake(ncs),
          o_p0 = ctake(ncs * 4), o_p1 = ctake(ncs * 4), o_kl = ctake((ncs + 1) * 8),
          o_hid = ctake((ncs + 1) * 8), o_cc = ctake(ncs * 4), o_cd = ctake(ncs * sizeof(T)),
-         o_inv = ctake(nps * 4), o_pb = ctake((size_t)(2 * nblk_p + 2) * 8);
+         o_inv = ctake(nps * 4), o_pb = ctake((size_t)(2 * nblk_p + 2) * 8),
+         o_bk = ctake(((size_t)nbr * nblk_r + 1) * 4), o_rf = ctake(ncs * 4),
+         o_pb2 = ctake((size_t)(2 * std::max(nblk_c, 1LL) + 2) * 8), o_pc0 = ctake(ncs * 4),
+         o_pc1 = ctake(ncs * 4), o_pd0 = ctake(ncs * sizeof(T)), o_pd1 = ctake(ncs * sizeof(T)),
+         o_pg = ctake(ncs * 4), o_pn = ctake(ncs * 4), o_pr = ctake(ncs * 4);
   char* cand = (char*)ctx_buf(c, BUF_CAND, coff);
   int* ccell = (int*)(cand + o_cc);
   T* cden = (T*)(cand + o_cd);
@@ -709,27 +831,47 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   ga.nB = nBt;
   ga.Bmin = Bmin;
   ga.negMtot = -(T)b->Mtot;
-  ga.cden = cden;
-  ga.ccell = ccell;
   ga.Nmin = o->Nmin;
-  ga.G = (int*)(cand + o_G);
+  ga.pG = (int*)(cand + o_pg);
+  ga.pcnt = (int*)(cand + o_pn);
+  ga.pr2 = (int*)(cand + o_pr);
   ga.cnt = (int*)(cand + o_c);
   ga.r2 = (int*)(cand + o_r);
   ga.kept = (unsigned char*)(cand + o_k);
   ga.err = derr;
   int* pend = (int*)(cand + o_p0);
   int* pnext = (int*)(cand + o_p1);
-  int* dnext = (int*)(dcnt + 1);
+  int* qcell = (int*)(cand + o_pc0);
+  int* qcell_next = (int*)(cand + o_pc1);
+  T* qden = (T*)(cand + o_pd0);
+  T* qden_next = (T*)(cand + o_pd1);
+  int* retry = (int*)(cand + o_rf);
+  long long* pb2 = (long long*)(cand + o_pb2);
   {
     TimedScope ts(c, "halo_candidates");
     hipLaunchKernelGGL(k_cand_setup<T>, dim3(grid_for(np, 256 * 4)), dim3(256), 0, c->stream,
                        pkey, pidx, pcell, np, lb, ccell, cden, inv);
-    hipLaunchKernelGGL(k_nonneg_count, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
-                       c->stream, inv, np, pb);
     EXH_LAUNCH_CHECK();
-    exclusive_scan<long long>(c, pb, pb + nblk_p + 1, nblk_p);
-    hipLaunchKernelGGL(k_nonneg_write, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
-                       c->stream, inv, np, pb + nblk_p + 1, pend);
+    if (halo_cell_order()) {
+      // candidates in cell order (stable compaction of inv)
+      hipLaunchKernelGGL(k_nonneg_count, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
+                         c->stream, inv, np, pb);
+      EXH_LAUNCH_CHECK();
+      exclusive_scan<long long>(c, pb, pb + nblk_p + 1, nblk_p);
+      hipLaunchKernelGGL(k_nonneg_write, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
+                         c->stream, inv, np, pb + nblk_p + 1, pend);
+      hipLaunchKernelGGL(k_pos_gather<T>, dim3(grid_for(nc, 256 * 4)), dim3(256), 0, c->stream,
+                         pend, nc, ccell, cden, qcell, qden);
+    } else if (nc > 0) {
+      int* bk = (int*)(cand + o_bk);
+      hipLaunchKernelGGL(k_brick_hist, dim3((unsigned)nblk_r), dim3(256), 0, c->stream, ccell, nc,
+                         n1, n2, bsh, nby, nbr, (int)nblk_r, bk);
+      EXH_LAUNCH_CHECK();
+      exclusive_scan<int>(c, bk, bk, (long long)nbr * nblk_r);
+      hipLaunchKernelGGL(k_brick_scatter<T>, dim3((unsigned)nblk_r), dim3(256), 0, c->stream,
+                         ccell, cden, nc, n1, n2, bsh, nby, nbr, (int)nblk_r, bk, pend, qcell,
+                         qden);
+    }
     EXH_LAUNCH_CHECK();
   }
   long long npend = nc;
@@ -739,14 +881,26 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     while (npend > 0) {
       round++;
       unsigned long long tag_hi = (unsigned long long)round << 32;
-      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, dnext, 1, 0);
-      hipLaunchKernelGGL(k_grow<T>, dim3(grid_for(npend, 256)), dim3(256), 0, c->stream, ga, pend,
+      ga.prank = pend;
+      ga.pcell = qcell;
+      ga.pden = qden;
+      hipLaunchKernelGGL(k_grow<T>, dim3(grid_for(npend, 256)), dim3(256), 0, c->stream, ga,
                          (int)npend, tag_hi);
       EXH_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_commit<T>, dim3(grid_for(npend, 256)), dim3(256), 0, c->stream, ga,
-                         pend, (int)npend, tag_hi, pnext, dnext, tmax);
+                         (int)npend, tag_hi, retry, tmax);
       EXH_LAUNCH_CHECK();
-      EXH_CHECK_HIP(hipMemcpyAsync(hbuf, dnext, 4, hipMemcpyDeviceToHost, c->stream));
+      // candidates to retry, in this round's order
+      const long long nbk = (npend + PK_TILE - 1) / PK_TILE;
+      hipLaunchKernelGGL(k_nonneg_count, dim3((unsigned)nbk), dim3(256), 0, c->stream, retry,
+                         npend, pb2);
+      EXH_LAUNCH_CHECK();
+      exclusive_scan<long long>(c, pb2, pb2 + nbk + 1, nbk);
+      hipLaunchKernelGGL(k_retry_write<T>, dim3((unsigned)nbk), dim3(256), 0, c->stream, retry,
+                         npend, pb2 + nbk + 1, qcell, qden, pnext, qcell_next, qden_next);
+      EXH_LAUNCH_CHECK();
+      EXH_CHECK_HIP(hipMemcpyAsync(&((long long*)hbuf)[2], pb2 + nbk + 1 + nbk, 8,
+                                   hipMemcpyDeviceToHost, c->stream));
       EXH_CHECK_HIP(hipMemcpyAsync((int*)hbuf + 1, derr, 4, hipMemcpyDeviceToHost, c->stream));
       EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
       int e = ((int*)hbuf)[1];
@@ -754,8 +908,10 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
                   "halo growth left the supported range (code %d): radius^2 beyond %d, count "
                   "beyond the barrier table, or a sphere wider than half the box",
                   e, stab - 1);
-      npend = ((int*)hbuf)[0];
+      npend = hbuf[2];
       std::swap(pend, pnext);
+      std::swap(qcell, qcell_next);
+      std::swap(qden, qden_next);
       EXH_REQUIRE(round < 1000000, EXH_ERR_INTERNAL, "halo rounds did not converge");
     }
   }
@@ -790,7 +946,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   S.posh = hal + o_ph;
   S.velh = hal + o_vh;
   hipLaunchKernelGGL(k_gather_halos<T>, dim3(grid_for(nc, 256)), dim3(256), 0, c->stream,
-                     ga.kept, hidx, nc, ga.ccell, ga.cnt, ga.r2, n1, n2, S.hx, S.hcount, S.hr2);
+                     ga.kept, hidx, nc, ccell, ga.cnt, ga.r2, n1, n2, S.hx, S.hcount, S.hr2);
   EXH_LAUNCH_CHECK();
   // 5. masses
   if (nh > 0) {
