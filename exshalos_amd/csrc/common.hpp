@@ -218,6 +218,26 @@ __host__ __device__ inline int wrap1(int i, int j, int n) {
   return r;
 }
 
+// Division by a fixed divisor 1 <= d < 2^31 through a 64-bit reciprocal
+// m = ceil(2^64 / d): for 0 <= x < 2^31 the high 64 bits of x m are
+// exactly floor(x / d) (the error x (m - 2^64/d) / 2^64 < 2^-33 cannot
+// carry a fraction of at most (d-1)/d across an integer).
+struct FastDiv {
+  unsigned d;
+  unsigned long long m;
+};
+
+inline FastDiv fast_div(unsigned d) {
+  FastDiv f;
+  f.d = d;
+  f.m = d > 1 ? ~0ull / d + 1 : 0;
+  return f;
+}
+
+__device__ inline unsigned fdiv(unsigned x, const FastDiv& f) {
+  return f.d == 1 ? x : (unsigned)__umul64hi((unsigned long long)x, f.m);
+}
+
 template <typename T>
 struct cplx;
 template <>

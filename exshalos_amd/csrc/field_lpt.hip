@@ -190,26 +190,6 @@ __global__ __launch_bounds__(256) void k_poisson_modes(FieldArgs<T> a, T k_smoot
   }
 }
 
-// Division by a fixed divisor 1 <= d < 2^31 through a 64-bit reciprocal
-// m = ceil(2^64 / d): for 0 <= x < 2^31 the high 64 bits of x m are
-// exactly floor(x / d) (the error x (m - 2^64/d) / 2^64 < 2^-33 cannot
-// carry a fraction of at most (d-1)/d across an integer).
-struct FastDiv {
-  unsigned d;
-  unsigned long long m;
-};
-
-static FastDiv fast_div(unsigned d) {
-  FastDiv f;
-  f.d = d;
-  f.m = d > 1 ? ~0ull / d + 1 : 0;
-  return f;
-}
-
-__device__ inline unsigned fdiv(unsigned x, const FastDiv& f) {
-  return f.d == 1 ? x : (unsigned)__umul64hi((unsigned long long)x, f.m);
-}
-
 // In-place phi_k = c_k * pre / (-k^2) * scale for a given half-complex
 // grid (flat index over modes; 32-bit index arithmetic below 2^31 modes).
 template <typename T>
@@ -422,14 +402,23 @@ static TileGeom tile_geom(int n0, int n1, int n2) {
 }
 
 template <typename T>
-__global__ void k_to_positions(int n0, int n1, int n2, T Lc, T L0, T L1, T L2, T* S) {
+__global__ void k_to_positions(int n0, int n1, int n2, T Lc, T L0, T L1, T L2, FastDiv dz,
+                               FastDiv dy, T* S) {
   size_t ng = (size_t)n0 * n1 * n2;
   for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ng;
        c += (size_t)gridDim.x * blockDim.x) {
-    int k = (int)(c % n2);
-    size_t r = c / n2;
-    int j = (int)(r % n1);
-    int i = (int)(r / n1);
+    int i, j, k;
+    if (ng < (1ull << 31)) {
+      const unsigned r = fdiv((unsigned)c, dz);
+      k = (int)((unsigned)c - r * (unsigned)n2);
+      i = (int)fdiv(r, dy);
+      j = (int)(r - (unsigned)i * (unsigned)n1);
+    } else {
+      k = (int)(c % n2);
+      const size_t r = c / n2;
+      j = (int)(r % n1);
+      i = (int)(r / n1);
+    }
     int q[3] = {i, j, k};
     T L[3] = {L0, L1, L2};
     for (int a = 0; a < 3; a++) {
@@ -639,7 +628,7 @@ void launch_positions(exh_ctx* c, const exh_box* b, T* S) {
   TimedScope ts(c, "positions");
   hipLaunchKernelGGL(k_to_positions<T>, dim3(stream_grid((size_t)b->ng)), dim3(256), 0,
                      c->stream, b->nd[0], b->nd[1], b->nd[2], (T)b->Lc, (T)b->L[0], (T)b->L[1],
-                     (T)b->L[2], S);
+                     (T)b->L[2], fast_div(b->nd[2]), fast_div(b->nd[1]), S);
   EXH_LAUNCH_CHECK();
 }
 

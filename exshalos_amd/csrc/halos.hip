@@ -36,18 +36,24 @@ __device__ inline size_t cell_off(int ci, int cj, int ck, unsigned p, int n0, in
 // ---------------------------------------------------------------------------
 // peaks
 // ---------------------------------------------------------------------------
+// Cells are below 2^30 here, so the coordinates come from 32-bit
+// multiply-high divisions.
 template <typename T>
-__device__ inline bool is_peak(const T* d, size_t c, int n0, int n1, int n2) {
-  int k = (int)(c % n2);
-  size_t r = c / n2;
-  int j = (int)(r % n1);
-  int i = (int)(r / n1);
+__device__ inline bool is_peak(const T* d, size_t c, int n0, int n1, int n2, const FastDiv& dz,
+                               const FastDiv& dy) {
+  const unsigned r = fdiv((unsigned)c, dz);
+  const int k = (int)((unsigned)c - r * (unsigned)n2);
+  const int i = (int)fdiv(r, dy);
+  const int j = (int)(r - (unsigned)i * (unsigned)n1);
   T v = d[c];
   size_t sx = (size_t)n1 * n2;
   size_t bi = c - (size_t)i * sx, bj = c - (size_t)j * n2, bk = c - k;
-  return v > d[bi + (size_t)wrap1(i, 1, n0) * sx] && v > d[bi + (size_t)wrap1(i, -1, n0) * sx] &&
-         v > d[bj + (size_t)wrap1(j, 1, n1) * n2] && v > d[bj + (size_t)wrap1(j, -1, n1) * n2] &&
-         v > d[bk + wrap1(k, 1, n2)] && v > d[bk + wrap1(k, -1, n2)];
+  // every neighbour is loaded (no short-circuit), so the loads of a
+  // thread's cells are all in flight together
+  const T xp = d[bi + (size_t)wrap1(i, 1, n0) * sx], xm = d[bi + (size_t)wrap1(i, -1, n0) * sx];
+  const T yp = d[bj + (size_t)wrap1(j, 1, n1) * n2], ym = d[bj + (size_t)wrap1(j, -1, n1) * n2];
+  const T zp = d[bk + wrap1(k, 1, n2)], zm = d[bk + wrap1(k, -1, n2)];
+  return (v > xp) & (v > xm) & (v > yp) & (v > ym) & (v > zp) & (v > zm);
 }
 
 static constexpr int PK_ITEMS = 8;
@@ -59,14 +65,16 @@ static constexpr int PK_TILE = 256 * PK_ITEMS;
 // ballots, which keeps the reference's lexicographic peak order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_peak_count(const T* d, int n0, int n1, int n2,
-                                                    long long* bcnt) {
+                                                    FastDiv dz, FastDiv dy, long long* bcnt) {
   __shared__ int sh[4];
   const size_t ng = (size_t)n0 * n1 * n2;
   const size_t base = (size_t)blockIdx.x * PK_TILE + threadIdx.x;
   int cnt = 0;
+#pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
     const size_t c = base + (size_t)q * 256;
-    if (c < ng && is_peak(d, c, n0, n1, n2)) cnt++;
+    const size_t cc = c < ng ? c : 0;
+    cnt += (c < ng) & is_peak(d, cc, n0, n1, n2, dz, dy);
   }
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
@@ -76,17 +84,19 @@ __global__ __launch_bounds__(256) void k_peak_count(const T* d, int n0, int n1, 
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_peak_write(const T* d, int n0, int n1, int n2,
-                                                    const long long* boff, T* key, int* cell,
-                                                    int* idx) {
+                                                    FastDiv dz, FastDiv dy, const long long* boff,
+                                                    T* key, int* cell, int* idx) {
   __shared__ int wcnt[PK_ITEMS][4];
   const size_t ng = (size_t)n0 * n1 * n2;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const size_t base = (size_t)blockIdx.x * PK_TILE + threadIdx.x;
   unsigned long long bal[PK_ITEMS];
   T v[PK_ITEMS];
+#pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
     const size_t c = base + (size_t)q * 256;
-    const bool pk = c < ng && is_peak(d, c, n0, n1, n2);
+    const size_t cc = c < ng ? c : 0;
+    const bool pk = (c < ng) & is_peak(d, cc, n0, n1, n2, dz, dy);
     v[q] = pk ? d[c] : (T)0;
     bal[q] = __ballot(pk);
     if (lane == 0) wcnt[q][w] = __popcll(bal[q]);
@@ -743,7 +753,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   {
     TimedScope ts(c, "peaks");
     hipLaunchKernelGGL(k_peak_count<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
-                       n1, n2, bc);
+                       n1, n2, fast_div(n2), fast_div(n1), bc);
     EXH_LAUNCH_CHECK();
     exclusive_scan<long long>(c, bc, bc + nblk + 1, nblk);
   }
@@ -757,7 +767,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   {
     TimedScope ts(c, "peaks_write");
     hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
-                       n1, n2, bc + nblk + 1, pkey, pcell, pidx);
+                       n1, n2, fast_div(n2), fast_div(n1), bc + nblk + 1, pkey, pcell, pidx);
     EXH_LAUNCH_CHECK();
   }
   // 2. sort (keys with the lexicographic peak index as value)

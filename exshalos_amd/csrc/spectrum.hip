@@ -556,16 +556,15 @@ __global__ void k_zero(T* g, size_t n) {
     g[i] = (T)0;
 }
 
+// grid (blockIdx.y) of ngrids: delta = rho / (M / ng) - 1
 template <typename T>
-__global__ void k_normalise(T* grid, size_t ng, int ngrids, int nsub,
-                            const unsigned long long* counts) {
-  size_t total = ng * ngrids;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
-       i += (size_t)gridDim.x * blockDim.x) {
-    int gi = (int)(i / ng);
-    double M = (double)counts[gi / nsub];
-    grid[i] = (T)((double)grid[i] / (M / (double)ng) - 1.0);
-  }
+__global__ void k_normalise(T* grid, size_t ng, int nsub, const unsigned long long* counts) {
+  const double M = (double)counts[blockIdx.y / nsub];
+  const double mean = M / (double)ng;
+  T* g = grid + (size_t)blockIdx.y * ng;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ng;
+       i += (size_t)gridDim.x * blockDim.x)
+    g[i] = (T)((double)g[i] / mean - 1.0);
 }
 
 // Side of the particle lattice when np is a perfect cube, else 0.
@@ -686,8 +685,8 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   }
   if (!mass) {
     TimedScope ts(c, "grid_normalise");
-    hipLaunchKernelGGL(k_normalise<T>, dim3(grid_for(ng * ntype * nsub, 256 * 8)), dim3(256), 0,
-                       c->stream, grid, ng, ntype * nsub, nsub, counts);
+    hipLaunchKernelGGL(k_normalise<T>, dim3(grid_for(ng, 256 * 8), ntype * nsub), dim3(256), 0,
+                       c->stream, grid, ng, nsub, counts);
     EXH_LAUNCH_CHECK();
   }
 }
