@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_peak_count(const T* d, int n0, int n1, 
 template <typename T>
 __global__ __launch_bounds__(256) void k_peak_write(const T* d, int n0, int n1, int n2,
                                                     FastDiv dz, FastDiv dy, const long long* boff,
-                                                    T* key, int* cell, int* idx) {
+                                                    T* key, int* cell) {
   __shared__ int wcnt[PK_ITEMS][4];
   const size_t ng = (size_t)n0 * n1 * n2;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -111,7 +111,6 @@ __global__ __launch_bounds__(256) void k_peak_write(const T* d, int n0, int n1, 
       const long long at = ow + __popcll(bal[q] & below);
       key[at] = v[q];
       cell[at] = (int)(base + (size_t)q * 256);
-      idx[at] = (int)at;
     }
     o += wcnt[q][0] + wcnt[q][1] + wcnt[q][2] + wcnt[q][3];
   }
@@ -119,27 +118,21 @@ __global__ __launch_bounds__(256) void k_peak_write(const T* d, int n0, int n1, 
 
 // Candidates by rank (0 = densest): rank r <-> sorted position np-1-r,
 // kept for sorted positions >= lb (density >= the minimum barrier, and
-// never position 0, which the reference does not visit).  inv maps the
-// lexicographic peak index to its rank (or -1).
+// never position 0, which the reference does not visit).  The sort carried
+// each peak's cell along with its density.
 template <typename T>
-__global__ void k_cand_setup(const T* skey, const int* sidx, const int* pcell, long long np,
-                             long long lb, int* ccell, T* cden, int* inv) {
-  for (long long l = blockIdx.x * (long long)blockDim.x + threadIdx.x; l < np;
+__global__ void k_cand_setup(const T* skey, const int* scell, long long np, long long lb,
+                             int* ccell, T* cden) {
+  for (long long l = lb + blockIdx.x * (long long)blockDim.x + threadIdx.x; l < np;
        l += (long long)gridDim.x * blockDim.x) {
-    int orig = sidx[l];
-    if (l >= lb) {
-      long long r = np - 1 - l;
-      ccell[r] = pcell[orig];
-      cden[r] = skey[l];
-      inv[orig] = (int)r;
-    } else {
-      inv[orig] = -1;
-    }
+    const long long r = np - 1 - l;
+    ccell[r] = scell[l];
+    cden[r] = skey[l];
   }
 }
 
-// Ranks of the candidates listed in cell order (stable compaction of inv):
-// the growth rounds then walk the grid in memory order.
+// Block counts of the non-negative entries of v (PK_TILE entries a block),
+// for the in-order compaction of the retry list.
 __global__ void k_nonneg_count(const int* v, long long n, long long* bcnt) {
   __shared__ long long sh[4];
   long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
@@ -149,18 +142,6 @@ __global__ void k_nonneg_count(const int* v, long long n, long long* bcnt) {
   long long tot;
   block_exclusive_256<long long>(cnt, sh, &tot);
   if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
-}
-
-__global__ void k_nonneg_write(const int* v, long long n, const long long* boff, int* out) {
-  __shared__ long long sh[4];
-  long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
-  long long cnt = 0;
-  for (int q = 0; q < PK_ITEMS; q++)
-    if (base + q < n && v[base + q] >= 0) cnt++;
-  long long tot;
-  long long o = block_exclusive_256<long long>(cnt, sh, &tot) + boff[blockIdx.x];
-  for (int q = 0; q < PK_ITEMS; q++)
-    if (base + q < n && v[base + q] >= 0) out[o++] = v[base + q];
 }
 
 // Order of the first round: columns of BR x BR cells in (i, j) over all k
@@ -207,18 +188,6 @@ __global__ void k_brick_scatter(const int* ccell, const T* cden, long long nc, i
     prank[pos] = (int)r;
     pcell[pos] = cell;
     pden[pos] = cden[r];
-  }
-}
-
-// The candidates' cell and density by list position (cell-order lists).
-template <typename T>
-__global__ void k_pos_gather(const int* prank, long long n, const int* ccell, const T* cden,
-                             int* pcell, T* pden) {
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int r = prank[t];
-    pcell[t] = ccell[r];
-    pden[t] = cden[r];
   }
 }
 
@@ -667,16 +636,6 @@ static const std::vector<unsigned>& taus_table() {
   return t;
 }
 
-// Candidates of the first round in cell order instead of brick order
-// (EXH_HALO_ORDER=cell); the results are the same either way.
-static bool halo_cell_order() {
-  static const bool on = [] {
-    const char* e = std::getenv("EXH_HALO_ORDER");
-    return e && std::string(e) == "cell";
-  }();
-  return on;
-}
-
 template <typename T>
 long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o) {
   const int n0 = b->nd[0], n1 = b->nd[1], n2 = b->nd[2];
@@ -762,16 +721,16 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   const long long np = hbuf[0];
   c->stat_peaks = np;
   T* pkey = (T*)ctx_buf(c, BUF_PEAK_KEY, (size_t)std::max(np, 1LL) * sizeof(T));
-  int* pidx = (int*)ctx_buf(c, BUF_PEAK_IDX, (size_t)std::max(np, 1LL) * sizeof(int));
   int* pcell = (int*)ctx_buf(c, BUF_PEAK_CELL, (size_t)std::max(np, 1LL) * sizeof(int));
   {
     TimedScope ts(c, "peaks_write");
     hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
-                       n1, n2, fast_div(n2), fast_div(n1), bc + nblk + 1, pkey, pcell, pidx);
+                       n1, n2, fast_div(n2), fast_div(n1), bc + nblk + 1, pkey, pcell);
     EXH_LAUNCH_CHECK();
   }
-  // 2. sort (keys with the lexicographic peak index as value)
-  lomuto_sort<T>(c, pkey, pidx, np);
+  // 2. sort (densities in lexicographic peak order; each peak's cell rides
+  // along as the value)
+  lomuto_sort<T>(c, pkey, pcell, np);
 
   // 3. candidates
   const T Bmin = (T)c->halotab_bmin;
@@ -791,8 +750,6 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     return r;
   };
   size_t ncs = (size_t)std::max(nc, 1LL);
-  size_t nps = (size_t)std::max(np, 1LL);
-  long long nblk_p = (np + PK_TILE - 1) / PK_TILE;
   long long nblk_c = (nc + PK_TILE - 1) / PK_TILE;
   // first-round order: bricks of 2^bsh x 2^bsh cells (16 x 16, wider on big
   // grids so that the brick histogram fits in LDS)
@@ -805,7 +762,6 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   size_t o_c = ctake(ncs * 4), o_r = ctake(ncs * 4), o_k = ctake(ncs),
          o_p0 = ctake(ncs * 4), o_p1 = ctake(ncs * 4), o_kl = ctake((ncs + 1) * 8),
          o_hid = ctake((ncs + 1) * 8), o_cc = ctake(ncs * 4), o_cd = ctake(ncs * sizeof(T)),
-         o_inv = ctake(nps * 4), o_pb = ctake((size_t)(2 * nblk_p + 2) * 8),
          o_bk = ctake(((size_t)nbr * nblk_r + 1) * 4), o_rf = ctake(ncs * 4),
          o_pb2 = ctake((size_t)(2 * std::max(nblk_c, 1LL) + 2) * 8), o_pc0 = ctake(ncs * 4),
          o_pc1 = ctake(ncs * 4), o_pd0 = ctake(ncs * sizeof(T)), o_pd1 = ctake(ncs * sizeof(T)),
@@ -813,8 +769,6 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   char* cand = (char*)ctx_buf(c, BUF_CAND, coff);
   int* ccell = (int*)(cand + o_cc);
   T* cden = (T*)(cand + o_cd);
-  int* inv = (int*)(cand + o_inv);
-  long long* pb = (long long*)(cand + o_pb);
   int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
   T* eff = (T*)ctx_buf(c, BUF_EFF, (size_t)ng * sizeof(T));
   EXH_CHECK_HIP(hipMemcpyAsync(eff, delta, (size_t)ng * sizeof(T), hipMemcpyDeviceToDevice,
@@ -859,20 +813,10 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   long long* pb2 = (long long*)(cand + o_pb2);
   {
     TimedScope ts(c, "halo_candidates");
-    hipLaunchKernelGGL(k_cand_setup<T>, dim3(grid_for(np, 256 * 4)), dim3(256), 0, c->stream,
-                       pkey, pidx, pcell, np, lb, ccell, cden, inv);
+    hipLaunchKernelGGL(k_cand_setup<T>, dim3(grid_for(nc, 256 * 4)), dim3(256), 0, c->stream,
+                       pkey, pcell, np, lb, ccell, cden);
     EXH_LAUNCH_CHECK();
-    if (halo_cell_order()) {
-      // candidates in cell order (stable compaction of inv)
-      hipLaunchKernelGGL(k_nonneg_count, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
-                         c->stream, inv, np, pb);
-      EXH_LAUNCH_CHECK();
-      exclusive_scan<long long>(c, pb, pb + nblk_p + 1, nblk_p);
-      hipLaunchKernelGGL(k_nonneg_write, dim3((unsigned)std::max(nblk_p, 1LL)), dim3(256), 0,
-                         c->stream, inv, np, pb + nblk_p + 1, pend);
-      hipLaunchKernelGGL(k_pos_gather<T>, dim3(grid_for(nc, 256 * 4)), dim3(256), 0, c->stream,
-                         pend, nc, ccell, cden, qcell, qden);
-    } else if (nc > 0) {
+    if (nc > 0) {
       int* bk = (int*)(cand + o_bk);
       hipLaunchKernelGGL(k_brick_hist, dim3((unsigned)nblk_r), dim3(256), 0, c->stream, ccell, nc,
                          n1, n2, bsh, nby, nbr, (int)nblk_r, bk);
