@@ -239,6 +239,17 @@ __device__ inline unsigned fdiv(unsigned x, const FastDiv& f) {
 }
 
 template <typename T>
+struct vec3;
+template <>
+struct vec3<float> {
+  using type = float3;
+};
+template <>
+struct vec3<double> {
+  using type = double3;
+};
+
+template <typename T>
 struct cplx;
 template <>
 struct cplx<float> {

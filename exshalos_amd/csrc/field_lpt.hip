@@ -269,6 +269,25 @@ __device__ inline void stage_plane(const TileGeom& g, const T* f, int ncomp, int
   }
 }
 
+// All three components of an [ng][3] field for plane i at once: consecutive
+// threads read consecutive floats of a row's (k, component) run, and each
+// value goes to its component's LDS plane.
+template <typename T>
+__device__ inline void stage_plane3(const TileGeom& g, const T* f, int i, int j0, int k0,
+                                    T (*pl)[ST_PJ * ST_PK]) {
+  constexpr int ROW = 3 * ST_PK;
+  for (int v = threadIdx.x; v < ST_PJ * ROW; v += 256) {
+    const int pj = v / ROW, rem = v - pj * ROW;
+    const int pk = rem / 3, comp = rem - 3 * pk;
+    int j = j0 + pj - ST_H, k = k0 + pk - ST_H;
+    j = j < 0 ? j + g.n1 : (j >= g.n1 ? j - g.n1 : j);
+    k = k < 0 ? k + g.n2 : (k >= g.n2 ? k - g.n2 : k);
+    j = min(j, g.n1 - 1);  // partial tiles smaller than the halo
+    k = min(k, g.n2 - 1);
+    pl[comp][pj * ST_PK + pk] = f[(((size_t)i * g.n1 + j) * g.n2 + k) * 3 + comp];
+  }
+}
+
 __device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // Gradient of a scalar field phi: MODE 1 writes S = -D phi / (60 Lc) and
@@ -311,17 +330,29 @@ __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, do
                             rowj[2 * ST_PK], rowj[3 * ST_PK]);
       const double gz = d6w(rowj[-3], rowj[-2], rowj[-1], rowj[1], rowj[2], rowj[3]);
       if (kin && j < g.n1) {
+        // the cell's three components move as one 3-vector access
+        using V3 = typename vec3<T>::type;
         const size_t c = ((size_t)i * g.n1 + j) * g.n2 + k;
-        const double gg[3] = {gx, gy, gz};
-        for (int a = 0; a < 3; a++) {
-          if (MODE == 1) {
-            const T sv = (T)(-gg[a] * inv60);
-            S[3 * c + a] = sv;
-            if (V) V[3 * c + a] = fv * sv;
-          } else {
-            const T sv = (T)(coef * gg[a] * inv60);
-            S[3 * c + a] += sv;
-            if (V) V[3 * c + a] += fv * sv;
+        V3* Sp = reinterpret_cast<V3*>(S + 3 * c);
+        V3* Vp = V ? reinterpret_cast<V3*>(V + 3 * c) : nullptr;
+        if (MODE == 1) {
+          const T sx = (T)(-gx * inv60), sy = (T)(-gy * inv60), sz = (T)(-gz * inv60);
+          *Sp = V3{sx, sy, sz};
+          if (Vp) *Vp = V3{fv * sx, fv * sy, fv * sz};
+        } else {
+          const T sx = (T)(coef * gx * inv60), sy = (T)(coef * gy * inv60),
+                  sz = (T)(coef * gz * inv60);
+          V3 o = *Sp;
+          o.x += sx;
+          o.y += sy;
+          o.z += sz;
+          *Sp = o;
+          if (Vp) {
+            V3 u = *Vp;
+            u.x += fv * sx;
+            u.y += fv * sy;
+            u.z += fv * sz;
+            *Vp = u;
           }
         }
       }
@@ -361,7 +392,7 @@ __global__ __launch_bounds__(256) void k_source_tiles(TileGeom g, const T* S, do
       win[q][6] = S[(((size_t)ip * g.n1 + j) * g.n2 + kc) * 3];
     }
     __syncthreads();
-    for (int comp = 0; comp < 3; comp++) stage_plane<T>(g, S, 3, comp, i, j0, k0, pl[comp]);
+    stage_plane3<T>(g, S, i, j0, k0, pl);
     __syncthreads();
     for (int q = 0; q < ST_Q; q++) {
       const int jl = ty + 4 * q, j = j0 + jl;
