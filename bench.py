@@ -57,7 +57,7 @@ def algorithmic_bytes(name: str, nd: int, npk: int, interlacing: int, prec: int)
     c = 2 * r
     nsub = 1 + interlacing
     table = {
-        "field_modes": 2 * c * nh,                  # read noise, write delta_k
+        "field_modes": 3 * c * nh,                  # read noise, write delta_k and the kept copy
         "lpt1_kspace": 2 * c * nh,                  # read noise, write phi_k
         "lpt2_kspace": 2 * c * nh,                  # read/write in place
         "lpt_kspace": 2 * c * nh,                   # Poisson kernel where not fused into the FFT

@@ -145,24 +145,41 @@ __device__ inline void field_mode(const FieldArgs<T>& a, const SplineView& sp, i
   if (cj) im = -im;
 }
 
-// Mode kernels walk the (i, j) rows of the half-complex grid (grid-stride
-// over rows, threads along k), so the index arithmetic is per row.
+// Mode (i, j, k) of flat half-complex index id (multiply-high divisions
+// below 2^31 modes).
+__device__ inline void mode_coords(size_t id, size_t nh, int n1, int nz2, const FastDiv& dz,
+                                   const FastDiv& dy, int& i, int& j, int& k) {
+  if (nh < (1ull << 31)) {
+    const unsigned r = fdiv((unsigned)id, dz);
+    k = (int)((unsigned)id - r * (unsigned)nz2);
+    i = (int)fdiv(r, dy);
+    j = (int)(r - (unsigned)i * (unsigned)n1);
+  } else {
+    k = (int)(id % nz2);
+    const size_t r = id / nz2;
+    j = (int)(r % n1);
+    i = (int)(r / n1);
+  }
+}
+
+// Mode kernels: a flat grid-stride walk over the half-complex modes, so
+// consecutive lanes take consecutive modes (a row of nd/2+1 modes does not
+// fit a workgroup evenly).
 template <typename T>
-__global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale,
+__global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale, FastDiv dz, FastDiv dy,
                                                typename cplx<T>::type* out,
                                                typename cplx<T>::type* raw) {
   extern __shared__ double spl[];
   const SplineView sp = stage_spline(a, spl);
-  const long long rows = (long long)a.n0 * a.n1;
-  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int i = (int)(row / a.n1), j = (int)(row % a.n1);
-    for (int k = threadIdx.x; k < a.nz2; k += blockDim.x) {
-      const size_t id = (size_t)row * a.nz2 + k;
-      T re, im;
-      field_mode(a, sp, i, j, k, re, im);
-      if (raw) raw[id] = {re, im};
-      out[id] = {re * scale, im * scale};
-    }
+  const size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
+       id += (size_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    mode_coords(id, nh, a.n1, a.nz2, dz, dy, i, j, k);
+    T re, im;
+    field_mode(a, sp, i, j, k, re, im);
+    if (raw) raw[id] = {re, im};
+    out[id] = {re * scale, im * scale};
   }
 }
 
@@ -170,22 +187,22 @@ __global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale,
 // modes (no stored copy of delta_k is needed).
 template <typename T>
 __global__ __launch_bounds__(256) void k_poisson_modes(FieldArgs<T> a, T k_smooth, T scale,
+                                                       FastDiv dz, FastDiv dy,
                                                        typename cplx<T>::type* out) {
   extern __shared__ double spl[];
   const SplineView sp = stage_spline(a, spl);
-  const long long rows = (long long)a.n0 * a.n1;
-  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int i = (int)(row / a.n1), j = (int)(row % a.n1);
-    for (int k = threadIdx.x; k < a.nz2; k += blockDim.x) {
-      const size_t id = (size_t)row * a.nz2 + k;
-      T fact;
-      if (lpt_fact<T>(a.n0, a.n1, a.n2, a.kl0, a.kl1, a.kl2, k_smooth, i, j, k, fact)) {
-        T re, im;
-        field_mode(a, sp, i, j, k, re, im);
-        out[id] = {re / fact * scale, im / fact * scale};
-      } else {
-        out[id] = {(T)0, (T)0};
-      }
+  const size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
+       id += (size_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    mode_coords(id, nh, a.n1, a.nz2, dz, dy, i, j, k);
+    T fact;
+    if (lpt_fact<T>(a.n0, a.n1, a.n2, a.kl0, a.kl1, a.kl2, k_smooth, i, j, k, fact)) {
+      T re, im;
+      field_mode(a, sp, i, j, k, re, im);
+      out[id] = {re / fact * scale, im / fact * scale};
+    } else {
+      out[id] = {(T)0, (T)0};
     }
   }
 }
@@ -465,9 +482,9 @@ __global__ void k_to_positions(int n0, int n1, int n2, T Lc, T L0, T L1, T L2, F
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-// Row-walking mode kernels: grid and LDS of the staged spline.
-static unsigned row_grid(long long rows) {
-  return (unsigned)std::max(1LL, std::min(rows, 4096LL));
+// Mode kernels: grid (each workgroup stages the spline once) and LDS.
+static unsigned mode_grid(long long nh) {
+  return (unsigned)std::max(1LL, std::min((nh + 255) / 256, 4096LL));
 }
 
 template <typename T>
@@ -566,9 +583,9 @@ void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k
   C2* ck = (C2*)ctx_buf(c, BUF_CK0, nh * sizeof(C2));
   {
     TimedScope ts(c, "field_modes");
-    hipLaunchKernelGGL(k_field<T>, dim3(row_grid((long long)b->nd[0] * b->nd[1])), dim3(256),
-                       spline_lds(a), c->stream, a,
-                       (T)b->Normx, ck, (C2*)deltak_raw);
+    hipLaunchKernelGGL(k_field<T>, dim3(mode_grid((long long)nh)), dim3(256), spline_lds(a),
+                       c->stream, a, (T)b->Normx, fast_div(b->nz2), fast_div(b->nd[1]), ck,
+                       (C2*)deltak_raw);
     EXH_LAUNCH_CHECK();
   }
   fft_c2r(c, b->nd[0], b->nd[1], b->nd[2], b->precision, 1, ck, delta);
@@ -608,8 +625,9 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
     {
       TimedScope ts(c, "lpt1_kspace");
       auto a = make_field_args<T>(c, b, s, 1e5);
-      hipLaunchKernelGGL(k_poisson_modes<T>, dim3(row_grid((long long)n0 * n1)), dim3(256),
-                         spline_lds(a), c->stream, a, (T)k_smooth, Normx, ck);
+      hipLaunchKernelGGL(k_poisson_modes<T>, dim3(mode_grid((long long)nh)), dim3(256),
+                         spline_lds(a), c->stream, a, (T)k_smooth, Normx, fast_div(b->nz2),
+                         fast_div(n1), ck);
       EXH_LAUNCH_CHECK();
     }
     fft_c2r(c, n0, n1, n2, b->precision, 1, ck, phi);

@@ -176,9 +176,19 @@ void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, 
   z_real(c, n2, prec, true, (long long)batch * n0 * n1, in, out);
 }
 
+// EXH_FFT_POISSON=0: the Poisson factor in its own kernel instead of the
+// first inverse pass.
+static bool fuse_poisson() {
+  static const bool on = [] {
+    const char* e = std::getenv("EXH_FFT_POISSON");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
 bool fft_c2r_poisson(exh_ctx* c, int prec, const void* src, void* work, void* out,
                      const PoissonParams& p) {
-  if (!use_axis_passes(p.n0, p.n1)) return false;
+  if (!use_axis_passes(p.n0, p.n1) || !fuse_poisson()) return false;
   TimedScope ts(c, "fft_c2r");
   const long long nz2 = p.n2 / 2 + 1, plane = (long long)p.n1 * nz2;
   fft_axis0_inverse_poisson(c, prec, src, work, p);
