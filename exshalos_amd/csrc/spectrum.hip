@@ -704,6 +704,7 @@ struct PowerArgs {
   // bin index arithmetic
   float knf, kminf, dkf;
   double kn, kmin, dk;
+  FastDiv dz, dnd;  // divisions by nz and nd (flat mode index -> i, j, k)
 };
 
 __device__ inline double legendre_d(int l, double x) {
@@ -737,13 +738,38 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
   __syncthreads();
   const int nd = a.nd, nz = a.nz;
   const size_t nh = (size_t)nd * nd * nz;
-  const long long nrows = (long long)nd * nd;
+  // waves walk 64-mode chunks of the flat mode index (a row of nd/2+1
+  // modes would leave a chunk with one active lane); bins change at row
+  // boundaries, which the segmented scan below handles like any change
+  const long long nchunks = (long long)((nh + 63) / 64);
   const long long gw = (long long)blockIdx.x * nw + wave, tw = (long long)gridDim.x * nw;
-  for (long long row = gw; row < nrows; row += tw) {
-    const int i = (int)(row / nd), j = (int)(row % nd);
-    const int fi = (2 * i < nd) ? i : i - nd, fj = (2 * j < nd) ? j : j - nd;
-    for (int k0 = 0; k0 < nz; k0 += 64) {
-      const int k = k0 + lane;
+  for (long long ch = gw; ch < nchunks; ch += tw) {
+    {
+      const size_t id = (size_t)ch * 64 + lane;
+      int i = 0, j = 0, k = nz;  // k = nz: no mode
+      long long row = (long long)((nh - 1) / nz);
+      if (id < nh) {
+        if (nh < (1ull << 31)) {
+          const unsigned r = fdiv((unsigned)id, a.dz);
+          k = (int)((unsigned)id - r * (unsigned)nz);
+          i = (int)fdiv(r, a.dnd);
+          j = (int)(r - (unsigned)i * (unsigned)nd);
+          row = r;
+        } else {
+          k = (int)(id % nz);
+          const size_t r = id / nz;
+          j = (int)(r % nd);
+          i = (int)(r / nd);
+          row = (long long)r;
+        }
+      }
+      // the chunk's rows: within one row bins never decrease, so a bin has
+      // at most one segment per row; tails update the accumulators one row
+      // at a time (deterministically, without two lanes on one address)
+      const long long row0 = __shfl(row, 0, 64);
+      const int part = (int)(row - row0);
+      const int nparts = (int)(__shfl(row, 63, 64) - row0) + 1;
+      const int fi = (2 * i < nd) ? i : i - nd, fj = (2 * j < nd) ? j : j - nd;
       int b = -1;
       double kmod = 0, mu = 0;
       if (k < nz) {
@@ -786,7 +812,6 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
       double dr[8], di[8];
       const int nt = a.ntype;
       if (valid) {
-        const size_t id = ((size_t)i * nd + j) * nz + k;
         const double2 e1 = a.ex[i], e2 = a.ex[j], e3 = a.ez[k];
         const double wk = a.wx[i] * a.wx[j] * a.wz[k] * a.Normk;
         // e^{-i theta} = e1 e2 e3
@@ -823,9 +848,11 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
           double y = shfl_up_d(x, o);
           if (lane - o >= sstart) x += y;
         }
-        if (tail) acc[(size_t)b * NV + v] += x;
+        for (int p = 0; p < nparts; p++)
+          if (tail && part == p) acc[(size_t)b * NV + v] += x;
       }
-      if (tail) cacc[b] += (unsigned)(lane - sstart + 1);
+      for (int p = 0; p < nparts; p++)
+        if (tail && part == p) cacc[b] += (unsigned)(lane - sstart + 1);
     }
   }
   __syncthreads();
@@ -950,6 +977,8 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
   a.NV = NV;
   a.direction = direction;
   a.kbin = kbin_precision;
+  a.dz = fast_div((unsigned)nz);
+  a.dnd = fast_div((unsigned)nd);
   a.ex = dex;
   a.ez = dez;
   a.wx = dwx;
