@@ -3,6 +3,8 @@
 // codes; no entry point exits the process.
 #include <cstring>
 
+#include <cstdlib>
+
 #include "common.hpp"
 #include "host_tables.hpp"
 #include "kernels.hpp"
@@ -90,6 +92,12 @@ int exh_ctx_destroy(exh_ctx* c) {
     (void)hipEventDestroy(t.stop);
   }
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->aux) {
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   EXH_API_END
@@ -295,25 +303,72 @@ int exh_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smoot
 
 }  // extern "C"
 
+// Runs the caller's launches on another stream of the context until the
+// scope ends.
+struct StreamSwitch {
+  exh_ctx* c;
+  hipStream_t saved;
+  StreamSwitch(exh_ctx* ctx, hipStream_t s) : c(ctx), saved(ctx->stream) { c->stream = s; }
+  ~StreamSwitch() { c->stream = saved; }
+};
+
+static void ensure_aux(exh_ctx* c) {
+  if (c->aux) return;
+  EXH_CHECK_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  EXH_CHECK_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  EXH_CHECK_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+}
+
+// Generate_Halos_Box_from_Pk (box.c:4-68).  The LPT depends only on the
+// field's delta_k and the halo finder only on delta, so the LPT runs on the
+// auxiliary stream while the finder (latency-bound, with a host round trip
+// per sort level and growth round) runs on the main stream; the two share
+// no buffer.  The halo positions wait for both.
 template <typename T>
 static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* modes,
                           const exh_halo_opts* o, double k_smooth, int do_2lpt, int out_vel,
                           void* delta_out, void* S_out, void* V_out) {
   const size_t ng = (size_t)b->ng;
-  T* delta = delta_out ? (T*)delta_out : (T*)ctx_buf(c, BUF_R1, ng * sizeof(T));
+  T* delta = delta_out ? (T*)delta_out : (T*)ctx_buf(c, BUF_R2, ng * sizeof(T));
   // delta_k is kept for the LPT, which divides it by -k^2 as the reference
   // does with its deltak array (box.c:17-44)
   const size_t nhalf = (size_t)b->nd[0] * b->nd[1] * b->nz2;
   void* deltak = ctx_buf(c, BUF_CK1, nhalf * 2 * sizeof(T));
   launch_field<T>(c, b, modes, 1e5, delta, deltak);
-  long long nh = run_find_halos<T>(c, b, delta, o);
   T* S = S_out ? (T*)S_out : (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
   T* V = out_vel ? (V_out ? (T*)V_out : (T*)ctx_buf(c, BUF_V, ng * 3 * sizeof(T))) : nullptr;
-  // the field's BUF_R1 copy may be reused by the 2LPT source from here on
   exh_field_src ks{};
   ks.kind = EXH_SRC_DELTAK;
   ks.data = deltak;
-  launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V);
+  ensure_aux(c);
+  // the LPT joins the finder at the phase EXH_LPT_OVERLAP names (see
+  // run_find_halos; default 2, the growth rounds, whose later rounds leave
+  // most of the GPU idle), or runs before it (-1)
+  static const int fork_at = [] {
+    const char* e = std::getenv("EXH_LPT_OVERLAP");
+    return e ? std::atoi(e) : 2;
+  }();
+  bool forked = false;
+  auto fork_lpt = [&]() {
+    forked = true;
+    EXH_CHECK_HIP(hipEventRecord(c->ev_fork, c->stream));
+    EXH_CHECK_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    StreamSwitch sw(c, c->aux);
+    launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V);
+    EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
+  };
+  if (fork_at < 0) fork_lpt();
+  long long nh;
+  try {
+    nh = run_find_halos<T>(c, b, delta, o, [&](int p) {
+      if (p == fork_at) fork_lpt();
+    });
+  } catch (...) {
+    if (forked) (void)hipStreamSynchronize(c->aux);  // no LPT work outlives a failed call
+    throw;
+  }
+  if (!forked) fork_lpt();  // the finder ended before that phase (or it is > 2)
+  EXH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   run_halo_posh<T>(c, b, S, V, 0);
   c->halo.have_posh = true;
   c->halo.have_velh = V != nullptr;

@@ -54,6 +54,7 @@ enum Slot {
   BUF_CK1,      // second half-complex grid
   BUF_R0,       // real grid (potential)
   BUF_R1,       // real grid (2LPT source / second potential)
+  BUF_R2,       // real grid (the pipeline's density field)
   BUF_S,        // displacement field [ng][3]
   BUF_V,        // velocity field [ng][3]
   BUF_FLAG,     // int32 halo flags
@@ -138,6 +139,10 @@ inline void key_append(std::vector<unsigned char>& key, const U* p, size_t n) {
 struct exh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for work that overlaps the main
+  // stream's, ordered by the two events; created on first use
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool own_stream = false;
   void* buf[exh::BUF_NSLOTS] = {};
   size_t cap[exh::BUF_NSLOTS] = {};

@@ -637,7 +637,12 @@ static const std::vector<unsigned>& taus_table() {
 }
 
 template <typename T>
-long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o) {
+// phase(p), when given, is called as the finder reaches p: 0 = start,
+// 1 = after the peak sort, 2 = before the growth rounds (launches of other
+// streams can be placed there).
+long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o,
+                         const std::function<void(int)>& phase) {
+  if (phase) phase(0);
   const int n0 = b->nd[0], n1 = b->nd[1], n2 = b->nd[2];
   const long long ng = b->ng;
   EXH_REQUIRE(ng < (1LL << 30), EXH_ERR_ARG, "grid too large for int32 cell indices");
@@ -731,6 +736,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   // 2. sort (densities in lexicographic peak order; each peak's cell rides
   // along as the value)
   lomuto_sort<T>(c, pkey, pcell, np);
+  if (phase) phase(1);
 
   // 3. candidates
   const T Bmin = (T)c->halotab_bmin;
@@ -828,6 +834,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     }
     EXH_LAUNCH_CHECK();
   }
+  if (phase) phase(2);
   long long npend = nc;
   int round = 0;
   {
@@ -949,8 +956,10 @@ void run_halo_flag(exh_ctx* c, const exh_box* b, long long* flag_dev) {
   EXH_LAUNCH_CHECK();
 }
 
-template long long run_find_halos<float>(exh_ctx*, const exh_box*, const float*, const exh_halo_opts*);
-template long long run_find_halos<double>(exh_ctx*, const exh_box*, const double*, const exh_halo_opts*);
+template long long run_find_halos<float>(exh_ctx*, const exh_box*, const float*,
+                                        const exh_halo_opts*, const std::function<void(int)>&);
+template long long run_find_halos<double>(exh_ctx*, const exh_box*, const double*,
+                                         const exh_halo_opts*, const std::function<void(int)>&);
 template void run_halo_posh<float>(exh_ctx*, const exh_box*, const float*, const float*, int);
 template void run_halo_posh<double>(exh_ctx*, const exh_box*, const double*, const double*, int);
 

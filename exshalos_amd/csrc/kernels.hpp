@@ -1,6 +1,8 @@
 // Launchers shared between the translation units of the library.
 #pragma once
 
+#include <functional>
+
 #include "common.hpp"
 #include "host_tables.hpp"
 
@@ -21,7 +23,8 @@ struct HaloResult {
   long long nh;
 };
 template <typename T>
-long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o);
+long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh_halo_opts* o,
+                         const std::function<void(int)>& phase = {});
 template <typename T>
 void run_halo_posh(exh_ctx* c, const exh_box* b, const T* S, const T* V, int lagrangian);
 void run_halo_flag(exh_ctx* c, const exh_box* b, long long* flag_dev);
