@@ -65,8 +65,7 @@ def algorithmic_bytes(name: str, nd: int, npk: int, interlacing: int, prec: int)
         "peaks_write": r * ng,                      # read delta again, write peaks
         "lpt1_grad": r * ng + 3 * r * ng,           # read phi, write S1
         "lpt2_source": 3 * r * ng + r * ng,         # read S1, write source
-        "lpt2_grad": r * ng + 6 * r * ng,           # read phi2, read+write S
-        "positions": 6 * r * ng,
+        "lpt2_grad": r * ng + 9 * r * ng,           # read phi2, read+write S, write positions
         "grid_zero": r * ng * nsub,
         "cic_deposit": 3 * r * ng + r * ng * nsub,  # 12 Np + 4 ng (1 + il)
         "grid_normalise": 2 * r * ng * nsub,

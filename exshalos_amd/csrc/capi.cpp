@@ -335,7 +335,9 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
   const size_t nhalf = (size_t)b->nd[0] * b->nd[1] * b->nz2;
   void* deltak = ctx_buf(c, BUF_CK1, nhalf * 2 * sizeof(T));
   launch_field<T>(c, b, modes, 1e5, delta, deltak);
-  T* S = S_out ? (T*)S_out : (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
+  // displacements in scratch (the halo positions average them); the last
+  // LPT pass writes the particle positions into S_out
+  T* S = (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
   T* V = out_vel ? (V_out ? (T*)V_out : (T*)ctx_buf(c, BUF_V, ng * 3 * sizeof(T))) : nullptr;
   exh_field_src ks{};
   ks.kind = EXH_SRC_DELTAK;
@@ -354,7 +356,7 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
     EXH_CHECK_HIP(hipEventRecord(c->ev_fork, c->stream));
     EXH_CHECK_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
     StreamSwitch sw(c, c->aux);
-    launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V);
+    launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V, (T*)S_out);
     EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
   };
   if (fork_at < 0) fork_lpt();
@@ -372,7 +374,6 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
   run_halo_posh<T>(c, b, S, V, 0);
   c->halo.have_posh = true;
   c->halo.have_velh = V != nullptr;
-  if (S_out) launch_positions<T>(c, b, S);
   return nh;
 }
 
@@ -408,12 +409,11 @@ static long long pipeline_grid(exh_ctx* c, const exh_box* b, const T* delta,
     exh_field_src s{};
     s.kind = EXH_SRC_DELTA;
     s.data = delta;
-    T* S = S_out ? S_out : (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
+    T* S = (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
     T* V = out_vel ? (V_out ? V_out : (T*)ctx_buf(c, BUF_V, ng * 3 * sizeof(T))) : nullptr;
-    launch_lpt<T>(c, b, &s, k_smooth, do_2lpt, 0, S, V);
+    launch_lpt<T>(c, b, &s, k_smooth, do_2lpt, 0, S, V, S_out);
     run_halo_posh<T>(c, b, S, V, 0);
     c->halo.have_velh = V != nullptr;
-    if (S_out) launch_positions<T>(c, b, S_out);
   }
   c->halo.have_posh = true;
   return nh;

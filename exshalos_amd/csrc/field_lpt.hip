@@ -8,9 +8,9 @@
 //  k_poisson_*     phi_k = delta_k / (-k^2) (lpt.c:12-44, 141-173)
 //  k_grad_tiles    S1 = -D6(phi) / (60 Lc) (lpt.c:56-93); S2 = -3/7
 //                  Omz^(-1/143) D6(phi2) / (60 Lc) added to S, velocities
-//                  (lpt.c:182-223)
+//                  (lpt.c:182-223); the last pass also writes the
+//                  positions q + S with one periodic wrap (lpt.c:229-242)
 //  k_source_tiles  sum_{a<b} (S_aa S_bb - S_ab^2) (lpt.c:116-134)
-//  to_positions    q + S with one periodic wrap (lpt.c:229-242)
 //
 // Mode kernels stream over the half-complex grid; the stencil kernels
 // march planes through LDS.
@@ -310,9 +310,28 @@ __device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i -
 // Gradient of a scalar field phi: MODE 1 writes S = -D phi / (60 Lc) and
 // V = f1 S (lpt.c:56-93); MODE 2 adds -3/7 Omz^(-1/143) D phi2 / (60 Lc) to
 // S and f2 times it to V (lpt.c:182-223; coef carries the prefactor).
+// Positions written by the last gradient pass: q + S wrapped into the box
+// (Compute_Pos, lpt.c:229-255): the lattice point (q + 1/2) Lc in the
+// field's precision, plus S, wrapped once.
+template <typename T>
+struct PosOut {
+  T* P;  // [ng][3], may alias S; null: none
+  T Lc, L0, L1, L2;
+};
+
+template <typename T>
+__device__ inline T lattice_pos(int q, T Lc, T s, T L) {
+  const T base = (T)((double)((T)q * Lc) + (double)Lc / 2.0);
+  T v = base + s;
+  if (v >= L) v -= L;
+  if (v < (T)0) v += L;
+  return v;
+}
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, double inv60,
-                                                    double coef, T* S, T* V, T fv) {
+                                                    double coef, T* S, T* V, T fv,
+                                                    PosOut<T> po) {
   __shared__ T pl[ST_PJ * ST_PK];
   int i0, j0, k0;
   tile_origin(g, i0, j0, k0);
@@ -352,9 +371,11 @@ __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, do
         const size_t c = ((size_t)i * g.n1 + j) * g.n2 + k;
         V3* Sp = reinterpret_cast<V3*>(S + 3 * c);
         V3* Vp = V ? reinterpret_cast<V3*>(V + 3 * c) : nullptr;
+        V3 snew;
         if (MODE == 1) {
           const T sx = (T)(-gx * inv60), sy = (T)(-gy * inv60), sz = (T)(-gz * inv60);
-          *Sp = V3{sx, sy, sz};
+          snew = V3{sx, sy, sz};
+          *Sp = snew;
           if (Vp) *Vp = V3{fv * sx, fv * sy, fv * sz};
         } else {
           const T sx = (T)(coef * gx * inv60), sy = (T)(coef * gy * inv60),
@@ -364,6 +385,7 @@ __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, do
           o.y += sy;
           o.z += sz;
           *Sp = o;
+          snew = o;
           if (Vp) {
             V3 u = *Vp;
             u.x += fv * sx;
@@ -372,6 +394,10 @@ __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, do
             *Vp = u;
           }
         }
+        if (po.P)  // after S: P may alias it
+          *reinterpret_cast<V3*>(po.P + 3 * c) =
+              V3{lattice_pos<T>(i, po.Lc, snew.x, po.L0), lattice_pos<T>(j, po.Lc, snew.y, po.L1),
+                 lattice_pos<T>(k, po.Lc, snew.z, po.L2)};
       }
     }
     for (int q = 0; q < ST_Q; q++)
@@ -447,36 +473,6 @@ static TileGeom tile_geom(int n0, int n1, int n2) {
   g.ntj = (n1 + ST_TJ - 1) / ST_TJ;
   g.nti = (n0 + ST_TI - 1) / ST_TI;
   return g;
-}
-
-template <typename T>
-__global__ void k_to_positions(int n0, int n1, int n2, T Lc, T L0, T L1, T L2, FastDiv dz,
-                               FastDiv dy, T* S) {
-  size_t ng = (size_t)n0 * n1 * n2;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ng;
-       c += (size_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    if (ng < (1ull << 31)) {
-      const unsigned r = fdiv((unsigned)c, dz);
-      k = (int)((unsigned)c - r * (unsigned)n2);
-      i = (int)fdiv(r, dy);
-      j = (int)(r - (unsigned)i * (unsigned)n1);
-    } else {
-      k = (int)(c % n2);
-      const size_t r = c / n2;
-      j = (int)(r % n1);
-      i = (int)(r / n1);
-    }
-    int q[3] = {i, j, k};
-    T L[3] = {L0, L1, L2};
-    for (int a = 0; a < 3; a++) {
-      T base = (T)((double)((T)q[a] * Lc) + (double)Lc / 2.0);
-      T v = base + S[3 * c + a];
-      if (v >= L[a]) v -= L[a];
-      if (v < (T)0) v += L[a];
-      S[3 * c + a] = v;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -595,7 +591,12 @@ void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k
 // positions when out_pos).  V optional.
 template <typename T>
 void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
-                int do_2lpt, int out_pos, T* S, T* V) {
+                int do_2lpt, int out_pos, T* S, T* V, T* P) {
+  // the last gradient pass also writes the positions: into P, or over S
+  // when out_pos
+  if (out_pos && !P) P = S;
+  const PosOut<T> none{nullptr, 0, 0, 0, 0};
+  const PosOut<T> pos{P, (T)b->Lc, (T)b->L[0], (T)b->L[1], (T)b->L[2]};
   using C2 = typename cplx<T>::type;
   const int n0 = b->nd[0], n1 = b->nd[1], n2 = b->nd[2];
   size_t nh = (size_t)n0 * n1 * b->nz2, ng = (size_t)b->ng;
@@ -646,7 +647,7 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
   {
     TimedScope ts(c, "lpt1_grad");
     hipLaunchKernelGGL((k_grad_tiles<T, 1>), dim3(tblocks), dim3(256), 0, c->stream, tg, phi,
-                       inv60, 1.0, S, V, f1);
+                       inv60, 1.0, S, V, f1, do_2lpt ? none : pos);
     EXH_LAUNCH_CHECK();
   }
   if (do_2lpt) {
@@ -665,27 +666,18 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
     {
       TimedScope ts(c, "lpt2_grad");
       hipLaunchKernelGGL((k_grad_tiles<T, 2>), dim3(tblocks), dim3(256), 0, c->stream, tg, src,
-                         inv60, coef, S, V, f2);
+                         inv60, coef, S, V, f2, pos);
       EXH_LAUNCH_CHECK();
     }
   }
-  if (out_pos) launch_positions<T>(c, b, S);
 }
 
-template <typename T>
-void launch_positions(exh_ctx* c, const exh_box* b, T* S) {
-  TimedScope ts(c, "positions");
-  hipLaunchKernelGGL(k_to_positions<T>, dim3(stream_grid((size_t)b->ng)), dim3(256), 0,
-                     c->stream, b->nd[0], b->nd[1], b->nd[2], (T)b->Lc, (T)b->L[0], (T)b->L[1],
-                     (T)b->L[2], fast_div(b->nd[2]), fast_div(b->nd[1]), S);
-  EXH_LAUNCH_CHECK();
-}
 
 template void launch_field<float>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*, void*);
 template void launch_field<double>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*, void*);
-template void launch_lpt<float>(exh_ctx*, const exh_box*, const exh_field_src*, double, int, int, float*, float*);
-template void launch_lpt<double>(exh_ctx*, const exh_box*, const exh_field_src*, double, int, int, double*, double*);
-template void launch_positions<float>(exh_ctx*, const exh_box*, float*);
-template void launch_positions<double>(exh_ctx*, const exh_box*, double*);
+template void launch_lpt<float>(exh_ctx*, const exh_box*, const exh_field_src*, double, int, int,
+                                float*, float*, float*);
+template void launch_lpt<double>(exh_ctx*, const exh_box*, const exh_field_src*, double, int, int,
+                                 double*, double*, double*);
 
 }  // namespace exh

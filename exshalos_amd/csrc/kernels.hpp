@@ -14,9 +14,7 @@ void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k
                   void* delta, void* deltak_raw);
 template <typename T>
 void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
-                int do_2lpt, int out_pos, T* S, T* V);
-template <typename T>
-void launch_positions(exh_ctx* c, const exh_box* b, T* S);
+                int do_2lpt, int out_pos, T* S, T* V, T* P = nullptr);
 
 // halos.hip
 struct HaloResult {
