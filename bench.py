@@ -105,6 +105,10 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also copy a buffer of this size once (PMC calibration)")
     ap.add_argument("--cpu-nd", type=int, default=256)
+    ap.add_argument("--api", choices=("fused", "calls"), default="fused",
+                    help="fused: exh_halos_box_power (LPT, grid and P(k) beside the halo "
+                         "finder on a second stream); calls: halos_box_from_pk + grid_compute "
+                         "+ power_compute")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
@@ -130,15 +134,16 @@ def main():
         pipe = BoxPipeline(k, P, nd=nd, Lc=4.0, precision=prec, do_2lpt=True, Nk=256,
                            kbin_precision=8)
         pipe.set_modes(noise64.astype(np.float32) if prec == 4 else noise64)
+        fused = args.api == "fused"
         for _ in range(warm):
-            pipe.run()
+            pipe.run(fused)
         print(f"[bench] prec {prec}: warmup done, nh = {getattr(pipe, 'nh', None)}", flush=True)
         torch.cuda.synchronize()
         pipe.timing(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            kk, Pk, cnt, nh = pipe.run()
+            kk, Pk, cnt, nh = pipe.run(fused)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         tim = pipe.timings()
@@ -220,7 +225,9 @@ def main():
         "data": "synthetic (BBKS P(k), numpy default_rng(1) white-noise modes)",
         "config": {"workload": f"nd={nd} Lc=4 L={4 * nd} 2LPT halos CIC-interlaced P(k) Nk=256",
                    "nd": nd, "Lc": 4.0, "Nk": 256, "window": "CIC", "interlacing": True,
-                   "bins": "FP64 accumulators, FP64 bin index"},
+                   "bins": "FP64 accumulators, FP64 bin index",
+                   "api": ("exh_halos_box_power" if args.api == "fused" else
+                           "exh_halos_box_from_pk + exh_grid_compute + exh_power_compute")},
         "fp64": None if r64 is None else {
             "ms_per_step": round(r64["ms"], 3),
             "value": round(nd**3 / (r64["ms"] * 1e-3) / 1e6, 2),

@@ -324,10 +324,23 @@ static void ensure_aux(exh_ctx* c) {
 // auxiliary stream while the finder (latency-bound, with a host round trip
 // per sort level and growth round) runs on the main stream; the two share
 // no buffer.  The halo positions wait for both.
+// Mass assignment and power spectrum of the particle positions, chained
+// after the LPT on the same stream (exh_halos_box_power).
+struct SpectrumStage {
+  bool on = false;
+  int window = 2, interlacing = 1, Nk = 0, kbin = 8;
+  double k_min = 0, k_max = 0;
+  void* grid = nullptr;
+  double* k_out = nullptr;
+  double* P_out = nullptr;
+  long long* count_out = nullptr;
+};
+
 template <typename T>
 static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* modes,
                           const exh_halo_opts* o, double k_smooth, int do_2lpt, int out_vel,
-                          void* delta_out, void* S_out, void* V_out) {
+                          void* delta_out, void* S_out, void* V_out,
+                          const SpectrumStage& sp = SpectrumStage{}) {
   const size_t ng = (size_t)b->ng;
   T* delta = delta_out ? (T*)delta_out : (T*)ctx_buf(c, BUF_R2, ng * sizeof(T));
   // delta_k is kept for the LPT, which divides it by -k^2 as the reference
@@ -343,13 +356,16 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
   ks.kind = EXH_SRC_DELTAK;
   ks.data = deltak;
   ensure_aux(c);
-  // the LPT joins the finder at the phase EXH_LPT_OVERLAP names (see
-  // run_find_halos; default 2, the growth rounds, whose later rounds leave
-  // most of the GPU idle), or runs before it (-1)
-  static const int fork_at = [] {
+  // the second stream's chain joins the finder at the phase EXH_LPT_OVERLAP
+  // names (see run_find_halos), or runs before it (-1).  Defaults, as
+  // measured at nd = 512: the LPT alone forks before the growth rounds,
+  // whose later rounds leave most of the GPU idle; LPT + grid + P(k) forks
+  // at the finder's start.
+  static const int fork_env = [] {
     const char* e = std::getenv("EXH_LPT_OVERLAP");
-    return e ? std::atoi(e) : 2;
+    return e ? std::atoi(e) : -100;
   }();
+  const int fork_at = fork_env != -100 ? fork_env : (sp.on ? 0 : 2);
   bool forked = false;
   auto fork_lpt = [&]() {
     forked = true;
@@ -357,6 +373,13 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
     EXH_CHECK_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
     StreamSwitch sw(c, c->aux);
     launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V, (T*)S_out);
+    if (sp.on) {
+      const int nd = b->nd[0];
+      run_grid<T>(c, (const T*)S_out, nullptr, nullptr, nullptr, (long long)ng, 1, nd, b->L[0],
+                  b->Om0, b->redshift, -1, sp.window, sp.interlacing, 1, (T*)sp.grid);
+      run_power<T>(c, (const T*)sp.grid, 1, nd, b->L[0], sp.window, sp.interlacing, sp.Nk,
+                   sp.k_min, sp.k_max, 0, -1, sp.kbin, nullptr, nullptr, nullptr, true);
+    }
     EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
   };
   if (fork_at < 0) fork_lpt();
@@ -374,6 +397,7 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
   run_halo_posh<T>(c, b, S, V, 0);
   c->halo.have_posh = true;
   c->halo.have_velh = V != nullptr;
+  if (sp.on) power_copy_out(c, sp.k_out, sp.P_out, sp.count_out);
   return nh;
 }
 
@@ -391,6 +415,46 @@ extern "C" int exh_halos_box_from_pk(exh_ctx* c, const exh_box* b, const exh_fie
   long long n = b->precision == 4
                     ? pipeline<float>(c, b, modes, o, k_smooth, do_2lpt, out_vel, delta, S, V)
                     : pipeline<double>(c, b, modes, o, k_smooth, do_2lpt, out_vel, delta, S, V);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  if (nh) *nh = n;
+  EXH_API_END
+}
+
+extern "C" int exh_halos_box_power(exh_ctx* c, const exh_box* b, const exh_field_src* modes,
+                                   const exh_halo_opts* o, double k_smooth, int do_2lpt,
+                                   void* pos, int window, int interlacing, int Nk, double k_min,
+                                   double k_max, int kbin_precision, void* grid, double* k_out,
+                                   double* P_out, long long* count_out, long long* nh) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  check_box(b);
+  check_src(modes);
+  check_halo_opts(o);
+  EXH_REQUIRE(modes->kind == EXH_SRC_MODES, EXH_ERR_ARG, "pipeline needs white-noise modes");
+  EXH_REQUIRE(b->nd[0] == b->nd[1] && b->nd[1] == b->nd[2], EXH_ERR_UNSUPPORTED,
+              "the pipeline needs a cubic grid");
+  EXH_REQUIRE(pos != nullptr && grid != nullptr, EXH_ERR_ARG, "null positions or grid");
+  EXH_REQUIRE(window == EXH_WINDOW_NGP || window == EXH_WINDOW_CIC || window == EXH_WINDOW_TSC,
+              EXH_ERR_UNSUPPORTED, "window %d: only NGP (1), CIC (2) and TSC (5)", window);
+  EXH_REQUIRE(Nk >= 1 && k_max > k_min, EXH_ERR_ARG, "bad binning");
+  EXH_REQUIRE(kbin_precision == 4 || kbin_precision == 8, EXH_ERR_ARG,
+              "kbin_precision must be 4 or 8");
+  SpectrumStage sp;
+  sp.on = true;
+  sp.window = window;
+  sp.interlacing = interlacing;
+  sp.Nk = Nk;
+  sp.kbin = kbin_precision;
+  sp.k_min = k_min;
+  sp.k_max = k_max;
+  sp.grid = grid;
+  sp.k_out = k_out;
+  sp.P_out = P_out;
+  sp.count_out = count_out;
+  long long n =
+      b->precision == 4
+          ? pipeline<float>(c, b, modes, o, k_smooth, do_2lpt, 0, nullptr, pos, nullptr, sp)
+          : pipeline<double>(c, b, modes, o, k_smooth, do_2lpt, 0, nullptr, pos, nullptr, sp);
   EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
   if (nh) *nh = n;
   EXH_API_END

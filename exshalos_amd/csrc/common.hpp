@@ -77,6 +77,7 @@ enum Slot {
   BUF_SHELL,    // shell offset table of the halo finder
   BUF_HALOTAB,  // halo finder tables (barrier, mass function, spheres, taus)
   BUF_PTAB,     // power spectrum phase and window tables
+  BUF_GCNT,     // tracer counts of the grid deposit
   BUF_EFF,      // halo growth: delta with flagged cells set to -Mtot
   BUF_NSLOTS
 };
@@ -163,6 +164,12 @@ struct exh_ctx {
   int halotab_ncells = 0;
   double halotab_bmin = 0;
   std::vector<unsigned char> spline_key;  // P(k) spline in BUF_TAB
+  // P(k) sums left on the device by a deferred run_power (copied out by
+  // power_copy_out): offsets into BUF_PART and the layout
+  struct PowerPending {
+    size_t o_sum = 0, o_cnt = 0;
+    int Nk = 0, NV = 0, NPs = 0, ls = 0;
+  } power_pending;
   double spline_x0 = 0, spline_x1 = 0;    // its first and last knot
   long long shell_noffs = 0;
   // halo finder statistics of the last run

@@ -39,6 +39,9 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
 template <typename T>
 void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window,
                int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
-               int kbin_precision, double* k_out, double* P_out, long long* count_out);
+               int kbin_precision, double* k_out, double* P_out, long long* count_out,
+               bool deferred = false);
+// host outputs of a deferred run_power (waits for the context's stream)
+void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_out);
 
 }  // namespace exh

@@ -615,7 +615,7 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   // reference: vel * (1+z)^3 / H(Om0, z) (gridmodule.c:167-169)
   T Hz = (T)(100.0 * std::sqrt((T)Om0 * std::pow(1.0 + (T)z, 3) + (1.0 - (T)Om0)));
   T rsd = (T)(std::pow(1.0 + (T)z, 3.0) / (double)Hz);
-  unsigned long long* counts = (unsigned long long*)ctx_buf(c, BUF_MISC, 64 * 8);
+  unsigned long long* counts = (unsigned long long*)ctx_buf(c, BUF_GCNT, 64 * 8);
   EXH_REQUIRE(ntype <= 64, EXH_ERR_ARG, "at most 64 tracer types");
   {
     TimedScope ts(c, "grid_zero");
@@ -919,7 +919,8 @@ static double sinc_t(T x) {
 template <typename T>
 void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window,
                int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
-               int kbin_precision, double* k_out, double* P_out, long long* count_out) {
+               int kbin_precision, double* k_out, double* P_out, long long* count_out,
+               bool deferred) {
   using C2 = typename cplx<T>::type;
   EXH_REQUIRE(nd >= 2 && nd % 2 == 0, EXH_ERR_ARG, "nd must be even");
   EXH_REQUIRE(ntype >= 1 && ntype <= 8, EXH_ERR_UNSUPPORTED, "1 to 8 tracer types supported");
@@ -1029,6 +1030,23 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
                        c->stream, part, pcnt, nblocks, Nk, NV, sum, cnt);
     EXH_LAUNCH_CHECK();
   }
+  c->power_pending.o_sum = o_sum;
+  c->power_pending.o_cnt = o_cnt;
+  c->power_pending.Nk = Nk;
+  c->power_pending.NV = NV;
+  c->power_pending.NPs = NPs;
+  c->power_pending.ls = ls;
+  if (!deferred) power_copy_out(c, k_out, P_out, count_out);
+}
+
+// Copies the sums of the last run_power to the host (waits for the stream)
+// and forms k, P and the mode counts per bin.
+void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_out) {
+  const auto& pp = c->power_pending;
+  const int Nk = pp.Nk, NV = pp.NV, NPs = pp.NPs, ls = pp.ls;
+  char* pbuf = (char*)c->buf[BUF_PART];
+  const double* sum = (const double*)(pbuf + pp.o_sum);
+  const unsigned long long* cnt = (const unsigned long long*)(pbuf + pp.o_cnt);
   size_t hbytes = (size_t)Nk * NV * 8 + (size_t)Nk * 8;
   char* h = (char*)ctx_pinned(c, hbytes);
   EXH_CHECK_HIP(hipMemcpyAsync(h, sum, (size_t)Nk * NV * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1056,8 +1074,8 @@ template void run_grid<double>(exh_ctx*, const double*, const double*, const dou
                                long long, int, int, double, double, double, int, int, int, int,
                                double*);
 template void run_power<float>(exh_ctx*, const float*, int, int, double, int, int, int, double,
-                               double, int, int, int, double*, double*, long long*);
+                               double, int, int, int, double*, double*, long long*, bool);
 template void run_power<double>(exh_ctx*, const double*, int, int, double, int, int, int, double,
-                                double, int, int, int, double*, double*, long long*);
+                                double, int, int, int, double*, double*, long long*, bool);
 
 }  // namespace exh

@@ -81,6 +81,10 @@ _SIGS = {
     "exh_halos_box_from_pk": (C.c_int, [_P, C.POINTER(Box), C.POINTER(FieldSrc),
                                         C.POINTER(HaloOpts), C.c_double, C.c_int, C.c_int, _P,
                                         _P, _P, C.POINTER(C.c_longlong)]),
+    "exh_halos_box_power": (C.c_int, [_P, C.POINTER(Box), C.POINTER(FieldSrc),
+                                      C.POINTER(HaloOpts), C.c_double, C.c_int, _P, C.c_int,
+                                      C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P,
+                                      _P, _P, C.POINTER(C.c_longlong)]),
     "exh_halos_box_from_grid": (C.c_int, [_P, C.POINTER(Box), _P, C.POINTER(HaloOpts),
                                           C.c_double, C.c_int, C.c_int, _P, _P, _P, _P,
                                           C.POINTER(C.c_longlong)]),

@@ -63,25 +63,39 @@ class BoxPipeline:
         self.src.data = self.noise.data_ptr()
         self.src.k, self.src.P, self.src.Nk = self.kh.ctypes.data, self.Ph.ctypes.data, len(self.kh)
 
-    def run(self):
-        """One pass: modes -> halos + 2LPT positions -> CIC grids -> P(k)."""
+    def run(self, fused=True):
+        """One pass: modes -> halos + 2LPT positions -> CIC grids -> P(k).
+
+        fused: one exh_halos_box_power call (the LPT, mass assignment and
+        power spectrum on a second stream while the halo finder runs);
+        False: the three calls exh_halos_box_from_pk, exh_grid_compute and
+        exh_power_compute, as the reference's API chains them."""
         lib = N.load()
         nh = C.c_longlong()
-        N.check(lib.exh_halos_box_from_pk(self.ctx, C.byref(self.box), C.byref(self.src),
-                                          C.byref(self.opts), self.k_smooth, self.do_2lpt, 0,
-                                          None, N.ptr(self.pos), None, C.byref(nh)),
-                "halos_box_from_pk")
-        ng = self.nd**3
-        N.check(lib.exh_grid_compute(self.ctx, N.ptr(self.pos), None, None, None, ng, 1, self.nd,
-                                     self.L, 0.31, 0.0, -1, self.window, self.il, 1, self.prec,
-                                     N.ptr(self.grid)), "grid_compute")
         kmin, kmax = 2 * np.pi / self.L, np.pi / self.L * self.nd
         if self.prec == 4:
             kmin, kmax = float(np.float32(kmin)), float(np.float32(kmax))
-        N.check(lib.exh_power_compute(self.ctx, N.ptr(self.grid), 1, self.nd, self.L, self.window,
-                                      self.il, self.Nk, kmin, kmax, 0, -1, self.prec, self.kbin,
-                                      N.ptr(self.kout), N.ptr(self.Pout), N.ptr(self.cnt)),
-                "power_compute")
+        if fused:
+            N.check(lib.exh_halos_box_power(self.ctx, C.byref(self.box), C.byref(self.src),
+                                            C.byref(self.opts), self.k_smooth, self.do_2lpt,
+                                            N.ptr(self.pos), self.window, self.il, self.Nk,
+                                            kmin, kmax, self.kbin, N.ptr(self.grid),
+                                            N.ptr(self.kout), N.ptr(self.Pout), N.ptr(self.cnt),
+                                            C.byref(nh)),
+                    "halos_box_power")
+        else:
+            N.check(lib.exh_halos_box_from_pk(self.ctx, C.byref(self.box), C.byref(self.src),
+                                              C.byref(self.opts), self.k_smooth, self.do_2lpt, 0,
+                                              None, N.ptr(self.pos), None, C.byref(nh)),
+                    "halos_box_from_pk")
+            ng = self.nd**3
+            N.check(lib.exh_grid_compute(self.ctx, N.ptr(self.pos), None, None, None, ng, 1,
+                                         self.nd, self.L, 0.31, 0.0, -1, self.window, self.il, 1,
+                                         self.prec, N.ptr(self.grid)), "grid_compute")
+            N.check(lib.exh_power_compute(self.ctx, N.ptr(self.grid), 1, self.nd, self.L,
+                                          self.window, self.il, self.Nk, kmin, kmax, 0, -1,
+                                          self.prec, self.kbin, N.ptr(self.kout),
+                                          N.ptr(self.Pout), N.ptr(self.cnt)), "power_compute")
         self.nh = nh.value
         return self.kout.copy(), self.Pout.copy(), self.cnt.copy(), self.nh
 

@@ -185,6 +185,23 @@ int exh_halos_box_from_grid(exh_ctx* ctx, const exh_box* box, const void* delta,
                             int out_vel, const void* S_in, const void* V_in, void* S_out,
                             void* V_out, long long* nh);
 
+/* Halos, particle positions, their grid and its power spectrum in one call:
+ * halos_box_from_pk (exshalos.c:710-940) with OUT_LPT, then grid_compute
+ * (spectrum.c:46-108) of the positions (unweighted, real space, one
+ * tracer) and power_compute (spectrum.c:111-197) of that grid (l_max = 0),
+ * the chain pyexshalos users run as three calls.  It has no single
+ * counterpart in the reference: it exists so that the LPT, the mass
+ * assignment and the power spectrum run on a second stream while the halo
+ * finder (latency-bound) runs.  Results equal the three calls': pos: device
+ * [ng][3] T; grid: device [1+interlacing][nd^3] T; k_out, P_out, count_out:
+ * host [Nk]; the catalogue stays on the device as after
+ * exh_halos_box_from_pk. */
+int exh_halos_box_power(exh_ctx* ctx, const exh_box* box, const exh_field_src* modes,
+                        const exh_halo_opts* opts, double k_smooth, int do_2lpt, void* pos,
+                        int window, int interlacing, int Nk, double k_min, double k_max,
+                        int kbin_precision, void* grid, double* k_out, double* P_out,
+                        long long* count_out, long long* nh);
+
 /* -------------------------------------------------------------- spectrum */
 /* Mass assignment: replaces grid_compute (spectrum.c:46-108) / Tracer_Grid
  * (gridmodule.c:146-375).  pos/vel: device [np][3] T; mass: device [np] T

@@ -247,6 +247,34 @@ def test_halos_box_from_pk_config1(bbks, prec):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("do2", [0, 1])
+def test_halos_box_power_matches_three_calls(bbks, prec, do2):
+    """exh_halos_box_power (LPT, grid and P(k) on a second stream beside the
+    halo finder) gives what halos_box_from_pk + grid_compute +
+    power_compute give: positions and halo count bit for bit, mode counts
+    exactly, grids and P(k) to the rounding of the deposit's float atomics."""
+    from exshalos_amd.pipeline import BoxPipeline
+
+    k, P = bbks
+    nd = 64
+    pipe = BoxPipeline(k, P, nd=nd, Lc=LC, precision=prec, do_2lpt=bool(do2), Nk=24)
+    pipe.set_modes(_noise(nd, 9, prec))
+    k1, P1, c1, nh1 = pipe.run(fused=False)
+    pos1 = pipe.pos.cpu().numpy().copy()
+    g1 = pipe.grid.cpu().numpy().copy()
+    k2, P2, c2, nh2 = pipe.run(fused=True)
+    pos2 = pipe.pos.cpu().numpy()
+    g2 = pipe.grid.cpu().numpy()
+    assert nh1 == nh2 > 0
+    np.testing.assert_array_equal(pos1, pos2)
+    np.testing.assert_array_equal(c1, c2)
+    tol = 1e-5 if prec == 4 else 1e-12
+    assert _rel(g2, g1) < tol
+    assert _rel(P2, P1) < tol
+    assert _rel(k2, k1) < tol
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 def test_sort_extreme_pivots_and_ties(bbks, prec):
     """Isolated one-cell spikes on every other cell, heights nearly sorted in
     lexicographic order and quantised: Lomuto's last-element pivots are then
