@@ -347,7 +347,6 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
   // does with its deltak array (box.c:17-44)
   const size_t nhalf = (size_t)b->nd[0] * b->nd[1] * b->nz2;
   void* deltak = ctx_buf(c, BUF_CK1, nhalf * 2 * sizeof(T));
-  launch_field<T>(c, b, modes, 1e5, delta, deltak);
   // displacements in scratch (the halo positions average them); the last
   // LPT pass writes the particle positions into S_out
   T* S = (T*)ctx_buf(c, BUF_S, ng * 3 * sizeof(T));
@@ -356,16 +355,21 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
   ks.kind = EXH_SRC_DELTAK;
   ks.data = deltak;
   ensure_aux(c);
-  // the second stream's chain joins the finder at the phase EXH_LPT_OVERLAP
-  // names (see run_find_halos), or runs before it (-1).  Defaults, as
-  // measured at nd = 512: the LPT alone forks before the growth rounds,
-  // whose later rounds leave most of the GPU idle; LPT + grid + P(k) forks
-  // at the finder's start.
+  // The second stream's chain starts at the point EXH_LPT_OVERLAP names:
+  // -2 once the field's modes (delta_k) are written, -1 after the field,
+  // 0..2 at the halo finder's phases (see run_find_halos), 3 after it.
+  // Defaults, as measured at nd = 512: the LPT alone starts before the
+  // growth rounds, whose later rounds leave most of the GPU idle; LPT +
+  // grid + P(k) starts with the finder (earlier, it slows the field's own
+  // transform more than it gains).
   static const int fork_env = [] {
     const char* e = std::getenv("EXH_LPT_OVERLAP");
     return e ? std::atoi(e) : -100;
   }();
-  const int fork_at = fork_env != -100 ? fork_env : (sp.on ? 0 : 2);
+  int fork_at = fork_env != -100 ? fork_env : (sp.on ? 0 : 2);
+  // the field's inverse transform and the LPT's may overlap only where
+  // neither uses a rocFFT plan (their work buffer would be shared)
+  if (fork_at == -2 && !fft_plan_free(b->nd[0], b->nd[1], b->nd[2])) fork_at = -1;
   bool forked = false;
   auto fork_lpt = [&]() {
     forked = true;
@@ -382,7 +386,15 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
     }
     EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
   };
-  if (fork_at < 0) fork_lpt();
+  try {
+    launch_field<T>(c, b, modes, 1e5, delta, deltak, [&]() {
+      if (fork_at == -2) fork_lpt();
+    });
+  } catch (...) {
+    if (forked) (void)hipStreamSynchronize(c->aux);
+    throw;
+  }
+  if (fork_at == -1) fork_lpt();
   long long nh;
   try {
     nh = run_find_halos<T>(c, b, delta, o, [&](int p) {

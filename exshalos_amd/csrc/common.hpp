@@ -52,6 +52,7 @@ struct Error {
 enum Slot {
   BUF_CK0,      // half-complex grid (field modes, potentials)
   BUF_CK1,      // second half-complex grid
+  BUF_CKF,      // half-complex work grid of the density field's transform
   BUF_R0,       // real grid (potential)
   BUF_R1,       // real grid (2LPT source / second potential)
   BUF_R2,       // real grid (the pipeline's density field)
@@ -187,6 +188,10 @@ void* ctx_pinned(exh_ctx* c, size_t bytes);
 void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void* in, void* out);
 void fft_c2r(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, void* in, void* out);
 void fft_release(exh_ctx* c);
+// true when a 3D transform of this shape runs on the strided and row passes
+// alone: no rocFFT plan, whose work buffer calls with the same shape share,
+// so two such transforms may run on different streams at once
+bool fft_plan_free(int n0, int n1, int n2);
 // strided complex passes of the 3D transforms (fft.hip)
 bool fft_axis_supported(int n);
 void fft_axis_pass(exh_ctx* c, int prec, bool inverse, int N, void* data, long long stride,

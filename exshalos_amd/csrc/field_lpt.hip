@@ -570,13 +570,16 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
   return a;
 }
 
+// after_modes, when given, is called once the mode kernel is launched and
+// before the inverse FFT (work on other streams that needs only deltak_raw
+// can start there; the FFT works in its own buffer).
 template <typename T>
 void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
-                  void* delta, void* deltak_raw) {
+                  void* delta, void* deltak_raw, const std::function<void()>& after_modes) {
   using C2 = typename cplx<T>::type;
   auto a = make_field_args<T>(c, b, s, k_smooth);
   size_t nh = (size_t)b->nd[0] * b->nd[1] * b->nz2;
-  C2* ck = (C2*)ctx_buf(c, BUF_CK0, nh * sizeof(C2));
+  C2* ck = (C2*)ctx_buf(c, BUF_CKF, nh * sizeof(C2));
   {
     TimedScope ts(c, "field_modes");
     hipLaunchKernelGGL(k_field<T>, dim3(mode_grid((long long)nh)), dim3(256), spline_lds(a),
@@ -584,6 +587,7 @@ void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k
                        (C2*)deltak_raw);
     EXH_LAUNCH_CHECK();
   }
+  if (after_modes) after_modes();
   fft_c2r(c, b->nd[0], b->nd[1], b->nd[2], b->precision, 1, ck, delta);
 }
 
@@ -673,8 +677,10 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
 }
 
 
-template void launch_field<float>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*, void*);
-template void launch_field<double>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*, void*);
+template void launch_field<float>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*,
+                                  void*, const std::function<void()>&);
+template void launch_field<double>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*,
+                                   void*, const std::function<void()>&);
 template void launch_lpt<float>(exh_ctx*, const exh_box*, const exh_field_src*, double, int, int,
                                 float*, float*, float*);
 template void launch_lpt<double>(exh_ctx*, const exh_box*, const exh_field_src*, double, int, int,

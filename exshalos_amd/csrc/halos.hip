@@ -213,18 +213,20 @@ __global__ void k_retry_write(const int* retry, long long n, const long long* bo
     }
 }
 
-// first index with key >= Bmin in the ascending array (one thread)
+// number of keys below Bmin (the first candidate's sorted position); *out
+// starts at zero
 template <typename T>
-__global__ void k_lower_bound(const T* key, long long n, T Bmin, long long* out) {
-  long long lo = 0, hi = n;
-  while (lo < hi) {
-    long long mid = (lo + hi) >> 1;
-    if (key[mid] < Bmin)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  *out = lo;
+__global__ void k_count_below(const T* key, long long n, T Bmin, unsigned long long* out) {
+  __shared__ unsigned sh[4];
+  unsigned cnt = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    cnt += key[i] < Bmin;
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicAdd(out, (unsigned long long)(sh[0] + sh[1] + sh[2] + sh[3]));
 }
 
 __global__ void k_fill_i32(int* p, long long n, int v) {
@@ -734,14 +736,19 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     EXH_LAUNCH_CHECK();
   }
   // 2. sort (densities in lexicographic peak order; each peak's cell rides
-  // along as the value)
-  lomuto_sort<T>(c, pkey, pcell, np);
+  // along as the value).  Peaks below the smallest barrier are never
+  // visited: their count is the first visited sorted position, and the
+  // sort leaves their order open.
+  const T Bmin = (T)c->halotab_bmin;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, (unsigned long long*)dcnt, 1,
+                     0ull);
+  hipLaunchKernelGGL(k_count_below<T>, dim3(grid_for(np, 256 * 16)), dim3(256), 0, c->stream,
+                     pkey, np, Bmin, (unsigned long long*)dcnt);
+  EXH_LAUNCH_CHECK();
+  lomuto_sort<T>(c, pkey, pcell, np, (double)Bmin);
   if (phase) phase(1);
 
-  // 3. candidates
-  const T Bmin = (T)c->halotab_bmin;
-  hipLaunchKernelGGL(k_lower_bound<T>, dim3(1), dim3(1), 0, c->stream, pkey, np, Bmin, dcnt);
-  EXH_LAUNCH_CHECK();
+  // 3. candidates (the sort stages its state in the same pinned buffer)
   EXH_CHECK_HIP(hipMemcpyAsync(hbuf, dcnt, 8, hipMemcpyDeviceToHost, c->stream));
   EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
   long long lb = std::max(hbuf[0], 1LL);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <functional>
+#include <limits>
 
 #include "common.hpp"
 #include "host_tables.hpp"
@@ -11,7 +12,8 @@ namespace exh {
 // field_lpt.hip
 template <typename T>
 void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
-                  void* delta, void* deltak_raw);
+                  void* delta, void* deltak_raw,
+                  const std::function<void()>& after_modes = {});
 template <typename T>
 void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
                 int do_2lpt, int out_pos, T* S, T* V, T* P = nullptr);
@@ -27,9 +29,13 @@ template <typename T>
 void run_halo_posh(exh_ctx* c, const exh_box* b, const T* S, const T* V, int lagrangian);
 void run_halo_flag(exh_ctx* c, const exh_box* b, long long* flag_dev);
 
-// sort.hip: Lomuto quicksort emulation (ascending, "<=" left, pivot last)
+// sort.hip: Lomuto quicksort emulation (ascending, "<=" left, pivot last).
+// With m keys below `floor`, positions m.. hold exactly what the full sort
+// puts there; positions below m are unspecified (segments of keys below
+// `floor` are not sorted further).
 template <typename K>
-void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n);
+void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n,
+                 double floor = -std::numeric_limits<double>::infinity());
 
 // spectrum.hip
 template <typename T>

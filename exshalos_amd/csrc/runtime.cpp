@@ -152,6 +152,10 @@ static void z_real(exh_ctx* c, int n2, int prec, bool inverse, long long rows, c
     run_plan(c, get_plan(c, 0, 0, n2, prec, inverse ? 1 : 0, rows, 1), in, out);
 }
 
+bool fft_plan_free(int n0, int n1, int n2) {
+  return use_axis_passes(n0, n1) && fft_rows_supported(n2);
+}
+
 void fft_r2c(exh_ctx* c, int n0, int n1, int n2, int prec, int batch, const void* in, void* out) {
   TimedScope ts(c, "fft_r2c");
   if (!use_axis_passes(n0, n1)) {

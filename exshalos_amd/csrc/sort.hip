@@ -89,6 +89,9 @@ struct LevelBufs {
   SegInfo* info;
   int* seg_need;
   LevelState* st;
+  // keys below `floor` need no order: a left child of a pivot below it
+  // (all its keys <= pivot) is left unsorted
+  double floor;
 };
 
 __device__ inline long long seg_end_tile(const LevelBufs& b, int sg) {
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int*
     keysF[g.lo + g.c] = pivot;
     valsF[g.lo + g.c] = valsA[g.hi];
     if (m > 0) arot[g.lo + g.r - 1] = (int)m;
-    push_child(b, g.lo, g.lo + g.c - 1);
+    if (!((double)pivot < b.floor)) push_child(b, g.lo, g.lo + g.c - 1);
     push_child(b, g.lo + g.c + 1, g.hi);
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -842,7 +845,7 @@ struct SmallCfg<double> {
 };
 
 template <typename K>
-void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
+void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
   TimedScope ts(c, "peak_sort");
   constexpr int SMALL = SmallCfg<K>::N;
   c->stat_sort_levels = 0;
@@ -895,6 +898,7 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
   b.info = (SegInfo*)(aux + o_info);
   b.seg_need = (int*)(aux + o_need);
   b.st = st;
+  b.floor = floor;
   int cur = 0;
   long long nsmall_host = 0;
   if (n <= SMALL) {
@@ -998,7 +1002,7 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n) {
   }
 }
 
-template void lomuto_sort<float>(exh_ctx*, float*, int*, long long);
-template void lomuto_sort<double>(exh_ctx*, double*, int*, long long);
+template void lomuto_sort<float>(exh_ctx*, float*, int*, long long, double);
+template void lomuto_sort<double>(exh_ctx*, double*, int*, long long, double);
 
 }  // namespace exh
