@@ -455,11 +455,15 @@ __global__ __launch_bounds__(256) void k_double_round(LevelBufs b, int* jmp, int
   if (__syncthreads_or(open) && threadIdx.x == 0) round_open[k] = 1;
 }
 
-// Elements whose capped chase did not finish: follow the converged
-// pointers to their slot.
+// Elements whose capped chase did not finish: follow the pointers to their
+// slot.  Every pointer leads forward along its own chain, so an element
+// whose walk reaches the stream's end within the guard has its exact slot,
+// converged or not.  Walks left short set async_open (strict = 0: the host
+// then runs the synchronous rounds and this kernel again) or err (strict).
 template <typename K>
 __global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* valsB, K* keysF,
-                              int* valsF, LevelBufs b, const int* jmp, const int* s0) {
+                              int* valsF, LevelBufs b, const int* jmp, const int* s0,
+                              int strict) {
   const long long t = blockIdx.x;
   const int sg = find_seg(b.toff, b.nseg, t);
   if (!b.seg_need[sg]) return;
@@ -474,8 +478,8 @@ __global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* v
     long long s = s0[j];
     if (s >= g.r) continue;  // placed by the capped chase
     for (int guard = 0; s < g.r && guard < 64; guard++) s = jmp[g.lo + s];
-    if (s < g.r) {  // doubling rounds too few: report, write nothing
-      atomicExch(&b.st->err, 1ull);
+    if (s < g.r) {  // write nothing: later rounds finish it, or report
+      atomicExch(strict ? &b.st->err : &b.st->async_open, 1ull);
       continue;
     }
     const long long dst = g.lo + g.c + 1 + (s - g.r);
@@ -952,31 +956,31 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
     hipLaunchKernelGGL(k_tile_chase<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd, keys,
                        vals, b, arot, s0, jmp, chase_cap());
     EXH_LAUNCH_CHECK();
+    // long chains (segments the chase flagged; the others return at once):
+    // asynchronous pointer jumping, then the walk to the slots, before the
+    // level's one host read
+    hipLaunchKernelGGL(k_double_async, dim3(nt), dim3(256), 0, c->stream, b, jmp, 64);
+    hipLaunchKernelGGL(k_tile_chase2<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd, keys,
+                       vals, b, jmp, s0, 0);
+    EXH_LAUNCH_CHECK();
     EXH_CHECK_HIP(hipMemcpyAsync(hs, st, sizeof(LevelState), hipMemcpyDeviceToHost, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-    if (hs->need_any) {
-      // long chains: finish them by pointer doubling (asynchronous; the
-      // synchronous rounds, bounded by the steps left, only if needed)
-      c->stat_sort_doubling++;
-      hipLaunchKernelGGL(k_double_async, dim3(nt), dim3(256), 0, c->stream, b, jmp, 64);
+    if (hs->need_any) c->stat_sort_doubling++;
+    if (hs->async_open) {
+      // chains still open: synchronous doubling rounds, bounded by the
+      // steps left, then the walk again (strict)
+      c->stat_sort_async_open++;
+      const unsigned long long rm = hs->rmax;
+      int rounds = 1;
+      while ((1ull << (rounds - 1)) < rm && rounds < 62) rounds++;
+      EXH_REQUIRE(rounds <= 40, EXH_ERR_INTERNAL, "peak sort: segment too long for doubling");
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, round_open, 64, 0);
+      for (int k = 0; k < rounds; k++)
+        hipLaunchKernelGGL(k_double_round, dim3(nt), dim3(256), 0, c->stream, b, jmp,
+                           round_open, k);
       EXH_LAUNCH_CHECK();
-      EXH_CHECK_HIP(hipMemcpyAsync(&hs->async_open, &st->async_open, 8, hipMemcpyDeviceToHost,
-                                   c->stream));
-      EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
-      if (hs->async_open) {
-        c->stat_sort_async_open++;
-        const unsigned long long rm = hs->rmax;
-        int rounds = 1;
-        while ((1ull << (rounds - 1)) < rm && rounds < 62) rounds++;
-        EXH_REQUIRE(rounds <= 40, EXH_ERR_INTERNAL, "peak sort: segment too long for doubling");
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, round_open, 64, 0);
-        for (int k = 0; k < rounds; k++)
-          hipLaunchKernelGGL(k_double_round, dim3(nt), dim3(256), 0, c->stream, b, jmp,
-                             round_open, k);
-        EXH_LAUNCH_CHECK();
-      }
       hipLaunchKernelGGL(k_tile_chase2<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd,
-                         keys, vals, b, jmp, s0);
+                         keys, vals, b, jmp, s0, 1);
       EXH_LAUNCH_CHECK();
     }
     const int nseg_next = (int)(hs->pk >> 32);
@@ -995,7 +999,7 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
                        b.shi, b.sbuf);
     EXH_LAUNCH_CHECK();
   }
-  if (c->stat_sort_doubling > 0) {
+  if (c->stat_sort_async_open > 0) {
     EXH_CHECK_HIP(hipMemcpyAsync(&hs->err, &st->err, 8, hipMemcpyDeviceToHost, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
     EXH_REQUIRE(!hs->err, EXH_ERR_INTERNAL, "peak sort: pointer doubling did not converge");
