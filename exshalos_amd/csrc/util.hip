@@ -1,5 +1,6 @@
-// Device-wide exclusive scan: per-block sums, recursive scan of the sums,
-// then a block-local rescan that adds the block offset.
+// Device-wide exclusive scan: one workgroup for up to ONE_ITEMS * 1024
+// entries; beyond, per-block sums, recursive scan of the sums, then a
+// block-local rescan that adds the block offset.
 #include "util.hpp"
 
 namespace exh {
@@ -39,6 +40,45 @@ __global__ void k_scan_apply(const T* in, long long n, const T* offs, T* out) {
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = off + tot;
 }
 
+// Whole scan in one workgroup of 1024 threads (16 waves), each thread
+// holding ONE_ITEMS consecutive entries.
+static constexpr int ONE_ITEMS = 16;
+static constexpr int ONE_THREADS = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(ONE_THREADS) void k_scan_one(const T* in, long long n, T* out) {
+  __shared__ T sh[ONE_THREADS / 64];
+  const long long base = (long long)threadIdx.x * ONE_ITEMS;
+  T v[ONE_ITEMS];
+  T s = 0;
+#pragma unroll
+  for (int q = 0; q < ONE_ITEMS; q++) {
+    v[q] = base + q < n ? in[base + q] : (T)0;
+    s += v[q];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const T y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  T pre = 0, tot = 0;
+  for (int w = 0; w < ONE_THREADS / 64; w++) {
+    const T t = sh[w];
+    if (w < wave) pre += t;
+    tot += t;
+  }
+  pre += x - s;
+#pragma unroll
+  for (int q = 0; q < ONE_ITEMS; q++) {
+    if (base + q < n) out[base + q] = pre;
+    pre += v[q];
+  }
+  if (threadIdx.x == 0) out[n] = tot;
+}
+
 template <typename T>
 static void scan_rec(exh_ctx* c, const T* in, T* out, long long n, T* scratch) {
   long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -59,6 +99,11 @@ void exclusive_scan(exh_ctx* c, const T* in, T* out, long long n) {
   if (n <= 0) {
     hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(256), 0, c->stream, in, 0LL,
                        (const T*)nullptr, out);
+    EXH_LAUNCH_CHECK();
+    return;
+  }
+  if (n <= (long long)ONE_ITEMS * ONE_THREADS) {
+    hipLaunchKernelGGL(k_scan_one<T>, dim3(1), dim3(ONE_THREADS), 0, c->stream, in, n, out);
     EXH_LAUNCH_CHECK();
     return;
   }
