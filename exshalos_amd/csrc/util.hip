@@ -1,4 +1,4 @@
-// Device-wide exclusive scan: one workgroup for up to ONE_ITEMS * 1024
+// Device-wide exclusive scan: one workgroup for up to ONE_ITEMS * 256
 // entries; beyond, per-block sums, recursive scan of the sums, then a
 // block-local rescan that adds the block offset.
 #include "util.hpp"
@@ -40,40 +40,27 @@ __global__ void k_scan_apply(const T* in, long long n, const T* offs, T* out) {
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = off + tot;
 }
 
-// Whole scan in one workgroup of 1024 threads (16 waves), each thread
-// holding ONE_ITEMS consecutive entries.
-static constexpr int ONE_ITEMS = 16;
-static constexpr int ONE_THREADS = 1024;
+// Whole scan in one small workgroup (256 threads, ONE_ITEMS consecutive
+// entries each, independent loads): it fits beside the large workgroups
+// of kernels running on another stream.
+static constexpr int ONE_ITEMS = 32;
+static constexpr int ONE_THREADS = 256;
 
 template <typename T>
 __global__ __launch_bounds__(ONE_THREADS) void k_scan_one(const T* in, long long n, T* out) {
-  __shared__ T sh[ONE_THREADS / 64];
-  const long long base = (long long)threadIdx.x * ONE_ITEMS;
+  __shared__ T sh[4];
+  const int base = threadIdx.x * ONE_ITEMS, nn = (int)n;
   T v[ONE_ITEMS];
+#pragma unroll
+  for (int q = 0; q < ONE_ITEMS; q++) v[q] = base + q < nn ? in[base + q] : (T)0;
   T s = 0;
 #pragma unroll
-  for (int q = 0; q < ONE_ITEMS; q++) {
-    v[q] = base + q < n ? in[base + q] : (T)0;
-    s += v[q];
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  T x = s;
-  for (int o = 1; o < 64; o <<= 1) {
-    const T y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) sh[wave] = x;
-  __syncthreads();
-  T pre = 0, tot = 0;
-  for (int w = 0; w < ONE_THREADS / 64; w++) {
-    const T t = sh[w];
-    if (w < wave) pre += t;
-    tot += t;
-  }
-  pre += x - s;
+  for (int q = 0; q < ONE_ITEMS; q++) s += v[q];
+  T tot;
+  T pre = block_exclusive_256<T>(s, sh, &tot);
 #pragma unroll
   for (int q = 0; q < ONE_ITEMS; q++) {
-    if (base + q < n) out[base + q] = pre;
+    if (base + q < nn) out[base + q] = pre;
     pre += v[q];
   }
   if (threadIdx.x == 0) out[n] = tot;
