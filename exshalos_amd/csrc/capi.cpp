@@ -98,6 +98,7 @@ int exh_ctx_destroy(exh_ctx* c) {
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_spec) (void)hipEventDestroy(c->ev_spec);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   EXH_API_END
@@ -317,6 +318,7 @@ static void ensure_aux(exh_ctx* c) {
   EXH_CHECK_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
   EXH_CHECK_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   EXH_CHECK_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  EXH_CHECK_HIP(hipEventCreateWithFlags(&c->ev_spec, hipEventDisableTiming));
 }
 
 // Generate_Halos_Box_from_Pk (box.c:4-68).  The LPT depends only on the
@@ -357,54 +359,71 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
   ensure_aux(c);
   // The second stream's chain starts at the point EXH_LPT_OVERLAP names:
   // -2 once the field's modes (delta_k) are written, -1 after the field,
-  // 0..2 at the halo finder's phases (see run_find_halos), 3 after it.
-  // Defaults, as measured at nd = 512: the LPT alone starts before the
-  // growth rounds, whose later rounds leave most of the GPU idle; LPT +
-  // grid + P(k) starts with the finder (earlier, it slows the field's own
-  // transform more than it gains).
+  // 0..2 at the halo finder's phases (see run_find_halos), 3 after it.  The
+  // mass assignment and power spectrum follow the LPT on that stream, and
+  // are queued at the point EXH_SPECTRUM_AT names (same numbering, never
+  // before the LPT).  Defaults, as measured at nd = 512: the LPT alone
+  // starts before the growth rounds, whose later rounds leave most of the
+  // GPU idle; LPT + grid + P(k) start together with the finder (queued
+  // later, the grid and P(k) slow the growth rounds more than they gain).
   static const int fork_env = [] {
     const char* e = std::getenv("EXH_LPT_OVERLAP");
+    return e ? std::atoi(e) : -100;
+  }();
+  static const int spec_env = [] {
+    const char* e = std::getenv("EXH_SPECTRUM_AT");
     return e ? std::atoi(e) : -100;
   }();
   int fork_at = fork_env != -100 ? fork_env : (sp.on ? 0 : 2);
   // the field's inverse transform and the LPT's may overlap only where
   // neither uses a rocFFT plan (their work buffer would be shared)
   if (fork_at == -2 && !fft_plan_free(b->nd[0], b->nd[1], b->nd[2])) fork_at = -1;
-  bool forked = false;
+  const int spec_at = std::max(fork_at, spec_env != -100 ? spec_env : fork_at);
+  bool forked = false, spec_queued = !sp.on;
+  auto queue_spectrum = [&]() {
+    spec_queued = true;
+    EXH_CHECK_HIP(hipEventRecord(c->ev_spec, c->stream));
+    EXH_CHECK_HIP(hipStreamWaitEvent(c->aux, c->ev_spec, 0));
+    StreamSwitch sw(c, c->aux);
+    const int nd = b->nd[0];
+    run_grid<T>(c, (const T*)S_out, nullptr, nullptr, nullptr, (long long)ng, 1, nd, b->L[0],
+                b->Om0, b->redshift, -1, sp.window, sp.interlacing, 1, (T*)sp.grid);
+    run_power<T>(c, (const T*)sp.grid, 1, nd, b->L[0], sp.window, sp.interlacing, sp.Nk,
+                 sp.k_min, sp.k_max, 0, -1, sp.kbin, nullptr, nullptr, nullptr, true);
+    EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
+  };
   auto fork_lpt = [&]() {
     forked = true;
     EXH_CHECK_HIP(hipEventRecord(c->ev_fork, c->stream));
     EXH_CHECK_HIP(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-    StreamSwitch sw(c, c->aux);
-    launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V, (T*)S_out);
-    if (sp.on) {
-      const int nd = b->nd[0];
-      run_grid<T>(c, (const T*)S_out, nullptr, nullptr, nullptr, (long long)ng, 1, nd, b->L[0],
-                  b->Om0, b->redshift, -1, sp.window, sp.interlacing, 1, (T*)sp.grid);
-      run_power<T>(c, (const T*)sp.grid, 1, nd, b->L[0], sp.window, sp.interlacing, sp.Nk,
-                   sp.k_min, sp.k_max, 0, -1, sp.kbin, nullptr, nullptr, nullptr, true);
+    {
+      StreamSwitch sw(c, c->aux);
+      launch_lpt<T>(c, b, &ks, k_smooth, do_2lpt, 0, S, V, (T*)S_out);
+      EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
     }
-    EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
+    if (!spec_queued && spec_at == fork_at) queue_spectrum();
+  };
+  auto at_point = [&](int p) {
+    if (p == fork_at) fork_lpt();
+    if (forked && !spec_queued && p == spec_at) queue_spectrum();
   };
   try {
-    launch_field<T>(c, b, modes, 1e5, delta, deltak, [&]() {
-      if (fork_at == -2) fork_lpt();
-    });
+    launch_field<T>(c, b, modes, 1e5, delta, deltak, [&]() { at_point(-2); });
+    at_point(-1);
   } catch (...) {
     if (forked) (void)hipStreamSynchronize(c->aux);
     throw;
   }
-  if (fork_at == -1) fork_lpt();
   long long nh;
   try {
-    nh = run_find_halos<T>(c, b, delta, o, [&](int p) {
-      if (p == fork_at) fork_lpt();
-    });
+    nh = run_find_halos<T>(c, b, delta, o, at_point);
   } catch (...) {
     if (forked) (void)hipStreamSynchronize(c->aux);  // no LPT work outlives a failed call
     throw;
   }
-  if (!forked) fork_lpt();  // the finder ended before that phase (or it is > 2)
+  // the finder ended before those points (or they are > 2)
+  if (!forked) fork_lpt();
+  if (!spec_queued) queue_spectrum();
   EXH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   run_halo_posh<T>(c, b, S, V, 0);
   c->halo.have_posh = true;

@@ -144,7 +144,7 @@ struct exh_ctx {
   // second stream (default priority) for work that overlaps the main
   // stream's, ordered by the two events; created on first use
   hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_spec = nullptr;
   bool own_stream = false;
   void* buf[exh::BUF_NSLOTS] = {};
   size_t cap[exh::BUF_NSLOTS] = {};
