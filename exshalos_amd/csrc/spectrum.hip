@@ -399,7 +399,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
 }
 
 // Window deposit: the tile's particles go into a fixed LDS window of
-// E = W + 2 WIN_HALO cells per axis (W = grid cells per tile side), placed at
+// E = W + 2 h cells per axis (W = grid cells per tile side, margin h), placed at
 // the tile's mean displacement.  The mean is taken over one sampled
 // particle per wave, so no pass over all particles precedes the deposit;
 // each particle's wrap and division are evaluated once per sub-grid.  The
@@ -407,13 +407,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
 // window is flushed row by row, lanes along z, one global atomic per
 // touched cell.
 static constexpr int WIN_THREADS = 1024;
-static constexpr int WIN_HALO = 4;
+static constexpr int WIN_HALO = 4;  // default margin h
 static constexpr int WIN_PPT = TILE_L * TILE_L * TILE_L / WIN_THREADS;
 
 template <typename T, int WIN>
 __global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
-    const T* pos, const T* vel, const T* mass, int nL, int nd, int W, T L, T Lb, int direction,
-    T rsd, int nsub, T* grid, unsigned long long* counts) {
+    const T* pos, const T* vel, const T* mass, int nL, int nd, int W, int wh, T L, T Lb,
+    int direction, T rsd, int nsub, T* grid, unsigned long long* counts) {
   constexpr int NW = WIN == 1 ? 1 : (WIN == 2 ? 2 : 3);
   constexpr int NWAVE = WIN_THREADS / 64;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
   const size_t ng = (size_t)nd * nd * nd;
   const T Ld = L / (T)nd;
   const int nt = nL / TILE_L;
-  const int E = W + 2 * WIN_HALO, sx = E * E;
+  const int E = W + 2 * wh, sx = E * E;
   const int tz = blockIdx.x % nt, ty = (blockIdx.x / nt) % nt, tx = blockIdx.x / (nt * nt);
   // tile centre in grid cells: the middle of the grid tile of the same index
   const int qc[3] = {tx * W + W / 2, ty * W + W / 2, tz * W + W / 2};
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
       }
     }
     const float mean = n ? sum / (float)n : 0.f;
-    wo_s[threadIdx.x] = (int)floorf(mean + 0.5f) - W / 2 - WIN_HALO;
+    wo_s[threadIdx.x] = (int)floorf(mean + 0.5f) - W / 2 - wh;
   }
   __syncthreads();
   const int wo[3] = {wo_s[0], wo_s[1], wo_s[2]};
@@ -599,6 +599,16 @@ static bool deposit_window_enabled() {
   return on;
 }
 
+// Window margin h in cells (EXH_DEPOSIT_MARGIN, 1..8, default WIN_HALO).
+static int deposit_window_margin() {
+  static const int v = [] {
+    const char* e = std::getenv("EXH_DEPOSIT_MARGIN");
+    const int h = e ? std::atoi(e) : WIN_HALO;
+    return h < 1 ? 1 : (h > 8 ? 8 : h);
+  }();
+  return v;
+}
+
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
@@ -638,7 +648,8 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   {
     TimedScope ts(c, window == 2 ? "cic_deposit" : (window == 1 ? "ngp_deposit" : "tsc_deposit"));
     const int W = tiles ? nd / nt : 0;
-    const int E = W + 2 * WIN_HALO;
+    const int wh = deposit_window_margin();
+    const int E = W + 2 * wh;
     const int win_lds = E * E * E * (int)sizeof(double);
     // nd >= E keeps every window cell within (-nd, 2nd), where the flush's
     // single-step wrap is exact
@@ -649,8 +660,8 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
     EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_deposit_window<T, WN>,                      \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, win_lds));     \
     hipLaunchKernelGGL((k_deposit_window<T, WN>), dim3(nblk), dim3(WIN_THREADS), win_lds,        \
-                       c->stream, pos, vel, mass, nL, nd, W, Lf, Lb, direction, rsd, nsub, grid, \
-                       counts);                                                                  \
+                       c->stream, pos, vel, mass, nL, nd, W, wh, Lf, Lb, direction, rsd, nsub,   \
+                       grid, counts);                                                            \
   } while (0)
       if (window == 1) EXH_DEPW(1);
       if (window == 2) EXH_DEPW(2);
