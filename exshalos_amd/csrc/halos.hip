@@ -36,46 +36,69 @@ __device__ inline size_t cell_off(int ci, int cj, int ck, unsigned p, int n0, in
 // ---------------------------------------------------------------------------
 // peaks
 // ---------------------------------------------------------------------------
-// Cells are below 2^30 here, so the coordinates come from 32-bit
-// multiply-high divisions.
-template <typename T>
-__device__ inline bool is_peak(const T* d, size_t c, int n0, int n1, int n2, const FastDiv& dz,
-                               const FastDiv& dy) {
-  const unsigned r = fdiv((unsigned)c, dz);
-  const int k = (int)((unsigned)c - r * (unsigned)n2);
-  const int i = (int)fdiv(r, dy);
-  const int j = (int)(r - (unsigned)i * (unsigned)n1);
-  T v = d[c];
-  size_t sx = (size_t)n1 * n2;
-  size_t bi = c - (size_t)i * sx, bj = c - (size_t)j * n2, bk = c - k;
-  // every neighbour is loaded (no short-circuit), so the loads of a
-  // thread's cells are all in flight together
-  const T xp = d[bi + (size_t)wrap1(i, 1, n0) * sx], xm = d[bi + (size_t)wrap1(i, -1, n0) * sx];
-  const T yp = d[bj + (size_t)wrap1(j, 1, n1) * n2], ym = d[bj + (size_t)wrap1(j, -1, n1) * n2];
-  const T zp = d[bk + wrap1(k, 1, n2)], zm = d[bk + wrap1(k, -1, n2)];
-  return (v > xp) & (v > xm) & (v > yp) & (v > ym) & (v > zp) & (v > zm);
-}
-
+// Cells are below 2^30 here, so indices and offsets are 32-bit.  A thread
+// tests the cells c0 + q * 256 (q < PK_ITEMS) of its block's tile: the
+// coordinates of the first come from two multiply-high divisions, the
+// others by stepping, and the six neighbours from offsets with the
+// periodic wrap as a select.  Every neighbour is loaded (no short-circuit),
+// so the loads of a thread's cells are all in flight together.
 static constexpr int PK_ITEMS = 8;
 static constexpr int PK_TILE = 256 * PK_ITEMS;
+
+struct PeakGeom {
+  int n0, n1, n2;
+  unsigned sx, sy;  // plane and row strides
+  FastDiv dz, dy;
+};
+
+template <typename T>
+__device__ inline bool is_peak_at(const T* d, unsigned c, int i, int j, int k,
+                                  const PeakGeom& g) {
+  const unsigned xp = i == g.n0 - 1 ? c - (unsigned)(g.n0 - 1) * g.sx : c + g.sx;
+  const unsigned xm = i == 0 ? c + (unsigned)(g.n0 - 1) * g.sx : c - g.sx;
+  const unsigned yp = j == g.n1 - 1 ? c - (unsigned)(g.n1 - 1) * g.sy : c + g.sy;
+  const unsigned ym = j == 0 ? c + (unsigned)(g.n1 - 1) * g.sy : c - g.sy;
+  const unsigned zp = k == g.n2 - 1 ? c - (unsigned)(g.n2 - 1) : c + 1;
+  const unsigned zm = k == 0 ? c + (unsigned)(g.n2 - 1) : c - 1;
+  const T v = d[c];
+  const T a = d[xp], b = d[xm], e = d[yp], f = d[ym], h = d[zp], l = d[zm];
+  return (v > a) & (v > b) & (v > e) & (v > f) & (v > h) & (v > l);
+}
+
+// Peak flags of the thread's PK_ITEMS cells (cells past the grid: false).
+template <typename T>
+__device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const PeakGeom& g) {
+  unsigned r = fdiv(c0, g.dz);
+  int k = (int)(c0 - r * (unsigned)g.n2);
+  int i = (int)fdiv(r, g.dy);
+  int j = (int)(r - (unsigned)i * (unsigned)g.n1);
+  unsigned bits = 0;
+#pragma unroll
+  for (int q = 0; q < PK_ITEMS; q++) {
+    const unsigned c = c0 + (unsigned)q * 256u;
+    if (c < ng && is_peak_at(d, c, i, j, k, g)) bits |= 1u << q;
+    k += 256;
+    while (k >= g.n2) {
+      k -= g.n2;
+      if (++j == g.n1) {
+        j = 0;
+        ++i;
+      }
+    }
+  }
+  return bits;
+}
 
 // Peaks are found over tiles of PK_TILE consecutive cells; thread x of a
 // block tests cells tile + q * 256 + x (q < PK_ITEMS), so every load row is
 // coalesced.  The write pass ranks peaks in cell order from per-row wave
 // ballots, which keeps the reference's lexicographic peak order.
 template <typename T>
-__global__ __launch_bounds__(256) void k_peak_count(const T* d, int n0, int n1, int n2,
-                                                    FastDiv dz, FastDiv dy, long long* bcnt) {
+__global__ __launch_bounds__(256) void k_peak_count(const T* d, PeakGeom g, long long* bcnt) {
   __shared__ int sh[4];
-  const size_t ng = (size_t)n0 * n1 * n2;
-  const size_t base = (size_t)blockIdx.x * PK_TILE + threadIdx.x;
-  int cnt = 0;
-#pragma unroll
-  for (int q = 0; q < PK_ITEMS; q++) {
-    const size_t c = base + (size_t)q * 256;
-    const size_t cc = c < ng ? c : 0;
-    cnt += (c < ng) & is_peak(d, cc, n0, n1, n2, dz, dy);
-  }
+  const unsigned ng = (unsigned)g.n0 * g.sx;
+  const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
+  int cnt = __popc(peak_bits(d, base, ng, g));
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
   __syncthreads();
@@ -83,22 +106,17 @@ __global__ __launch_bounds__(256) void k_peak_count(const T* d, int n0, int n1, 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_peak_write(const T* d, int n0, int n1, int n2,
-                                                    FastDiv dz, FastDiv dy, const long long* boff,
+__global__ __launch_bounds__(256) void k_peak_write(const T* d, PeakGeom g, const long long* boff,
                                                     T* key, int* cell) {
   __shared__ int wcnt[PK_ITEMS][4];
-  const size_t ng = (size_t)n0 * n1 * n2;
+  const unsigned ng = (unsigned)g.n0 * g.sx;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const size_t base = (size_t)blockIdx.x * PK_TILE + threadIdx.x;
+  const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
+  const unsigned bits = peak_bits(d, base, ng, g);
   unsigned long long bal[PK_ITEMS];
-  T v[PK_ITEMS];
 #pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
-    const size_t c = base + (size_t)q * 256;
-    const size_t cc = c < ng ? c : 0;
-    const bool pk = (c < ng) & is_peak(d, cc, n0, n1, n2, dz, dy);
-    v[q] = pk ? d[c] : (T)0;
-    bal[q] = __ballot(pk);
+    bal[q] = __ballot((bits >> q) & 1u);
     if (lane == 0) wcnt[q][w] = __popcll(bal[q]);
   }
   __syncthreads();
@@ -109,8 +127,9 @@ __global__ __launch_bounds__(256) void k_peak_write(const T* d, int n0, int n1, 
     for (int u = 0; u < w; u++) ow += wcnt[q][u];
     if ((bal[q] >> lane) & 1ull) {
       const long long at = ow + __popcll(bal[q] & below);
-      key[at] = v[q];
-      cell[at] = (int)(base + (size_t)q * 256);
+      const unsigned c = base + (unsigned)q * 256u;
+      key[at] = d[c];
+      cell[at] = (int)c;
     }
     o += wcnt[q][0] + wcnt[q][1] + wcnt[q][2] + wcnt[q][3];
   }
@@ -715,11 +734,13 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
 
   // 1. peaks
   long long nblk = (ng + PK_TILE - 1) / PK_TILE;
+  const PeakGeom pg{n0, n1, n2, (unsigned)((long long)n1 * n2), (unsigned)n2, fast_div(n2),
+                    fast_div(n1)};
   long long* bc = (long long*)ctx_buf(c, BUF_HALO, (size_t)(2 * nblk + 2) * 8);
   {
     TimedScope ts(c, "peaks");
-    hipLaunchKernelGGL(k_peak_count<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
-                       n1, n2, fast_div(n2), fast_div(n1), bc);
+    hipLaunchKernelGGL(k_peak_count<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, pg,
+                       bc);
     EXH_LAUNCH_CHECK();
     exclusive_scan<long long>(c, bc, bc + nblk + 1, nblk);
   }
@@ -731,8 +752,8 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   int* pcell = (int*)ctx_buf(c, BUF_PEAK_CELL, (size_t)std::max(np, 1LL) * sizeof(int));
   {
     TimedScope ts(c, "peaks_write");
-    hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, n0,
-                       n1, n2, fast_div(n2), fast_div(n1), bc + nblk + 1, pkey, pcell);
+    hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, pg,
+                       bc + nblk + 1, pkey, pcell);
     EXH_LAUNCH_CHECK();
   }
   // 2. sort (densities in lexicographic peak order; each peak's cell rides
