@@ -387,8 +387,8 @@ static long long pipeline(exh_ctx* c, const exh_box* b, const exh_field_src* mod
     StreamSwitch sw(c, c->aux);
     const int nd = b->nd[0];
     run_grid<T>(c, (const T*)S_out, nullptr, nullptr, nullptr, (long long)ng, 1, nd, b->L[0],
-                b->Om0, b->redshift, -1, sp.window, sp.interlacing, 1, (T*)sp.grid);
-    run_power<T>(c, (const T*)sp.grid, 1, nd, b->L[0], sp.window, sp.interlacing, sp.Nk,
+                b->Om0, b->redshift, -1, sp.window, 0.0, 0.0, sp.interlacing, 1, (T*)sp.grid);
+    run_power<T>(c, (const T*)sp.grid, 1, nd, b->L[0], sp.window, 0.0, sp.interlacing, sp.Nk,
                  sp.k_min, sp.k_max, 0, -1, sp.kbin, nullptr, nullptr, nullptr, true);
     EXH_CHECK_HIP(hipEventRecord(c->ev_join, c->stream));
   };
@@ -542,8 +542,8 @@ int exh_halos_box_from_grid(exh_ctx* c, const exh_box* b, const void* delta,
 
 int exh_grid_compute(exh_ctx* c, const void* pos, const void* vel, const void* mass,
                      const int* type, long long np, int ntype, int nd, double L, double Om0,
-                     double z, int direction, int window, int interlacing, int folds,
-                     int precision, void* grid) {
+                     double z, int direction, int window, double R, double R_times,
+                     int interlacing, int folds, int precision, void* grid) {
   EXH_API_BEGIN
   check_ctx(c);
   EXH_REQUIRE(precision == 4 || precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
@@ -553,16 +553,18 @@ int exh_grid_compute(exh_ctx* c, const void* pos, const void* vel, const void* m
   EXH_REQUIRE(direction >= -1 && direction <= 2, EXH_ERR_ARG, "direction must be -1, 0, 1 or 2");
   if (precision == 4)
     run_grid<float>(c, (const float*)pos, (const float*)vel, (const float*)mass, type, np, ntype,
-                    nd, L, Om0, z, direction, window, interlacing, folds, (float*)grid);
+                    nd, L, Om0, z, direction, window, R, R_times, interlacing, folds,
+                    (float*)grid);
   else
     run_grid<double>(c, (const double*)pos, (const double*)vel, (const double*)mass, type, np,
-                     ntype, nd, L, Om0, z, direction, window, interlacing, folds, (double*)grid);
+                     ntype, nd, L, Om0, z, direction, window, R, R_times, interlacing, folds,
+                     (double*)grid);
   EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
   EXH_API_END
 }
 
 int exh_power_compute(exh_ctx* c, const void* grid, int ntype, int nd, double L, int window,
-                      int interlacing, int Nk, double k_min, double k_max, int l_max,
+                      double R, int interlacing, int Nk, double k_min, double k_max, int l_max,
                       int direction, int precision, int kbin_precision, double* k_out,
                       double* P_out, long long* count_out) {
   EXH_API_BEGIN
@@ -573,12 +575,13 @@ int exh_power_compute(exh_ctx* c, const void* grid, int ntype, int nd, double L,
   EXH_REQUIRE(grid != nullptr, EXH_ERR_ARG, "null grid");
   EXH_REQUIRE(k_max > k_min, EXH_ERR_ARG, "k_max must exceed k_min");
   EXH_REQUIRE(direction >= -1 && direction <= 2, EXH_ERR_ARG, "direction must be -1, 0, 1 or 2");
+  EXH_REQUIRE(window != 3 || R > 0, EXH_ERR_ARG, "the sphere window needs R > 0");
   if (precision == 4)
-    run_power<float>(c, (const float*)grid, ntype, nd, L, window, interlacing, Nk, k_min, k_max,
-                     l_max, direction, kbin_precision, k_out, P_out, count_out);
+    run_power<float>(c, (const float*)grid, ntype, nd, L, window, R, interlacing, Nk, k_min,
+                     k_max, l_max, direction, kbin_precision, k_out, P_out, count_out);
   else
-    run_power<double>(c, (const double*)grid, ntype, nd, L, window, interlacing, Nk, k_min, k_max,
-                      l_max, direction, kbin_precision, k_out, P_out, count_out);
+    run_power<double>(c, (const double*)grid, ntype, nd, L, window, R, interlacing, Nk, k_min,
+                      k_max, l_max, direction, kbin_precision, k_out, P_out, count_out);
   EXH_API_END
 }
 

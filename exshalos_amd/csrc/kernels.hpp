@@ -41,9 +41,9 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n,
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
-              int window, int interlacing, int folds, T* grid);
+              int window, double R, double R_times, int interlacing, int folds, T* grid);
 template <typename T>
-void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window,
+void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window, double R,
                int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
                int kbin_precision, double* k_out, double* P_out, long long* count_out,
                bool deferred = false);

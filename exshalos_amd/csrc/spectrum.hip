@@ -173,6 +173,94 @@ __global__ void k_deposit(const T* pos, const T* vel, const T* mass, const int* 
   if (threadIdx.x < 64 && sh_cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sh_cnt[threadIdx.x]);
 }
 
+// Spherical (window 3) and exponential (window 4) assignment
+// (gridmodule.c:57-119): every cell of the cube of half-side
+// times = floor(1/2 + R_eff / Ld) around the particle's cell whose centre lies
+// within R_eff gets m (sphere, R_eff = R) or m exp(-r^2 / 2R^2) (R_eff =
+// R R_times).  As in the reference, each offset is evaluated in double and
+// stored in T, the periodic image is L - dx, and r^2 is summed in T.
+// Returns the deposited weight.
+__device__ inline int wrap_any(int x, int n) {
+  int r = x % n;
+  return r < 0 ? r + n : r;
+}
+
+template <typename T, bool EXPW>
+__device__ inline double dep_sphere(T* g, const T* x, int nd, T L, T Ld, T R, T Rt, T m) {
+  int xt[3];
+  for (int a = 0; a < 3; a++) {
+    xt[a] = (int)floor(x[a] / Ld);
+    if (xt[a] == nd) xt[a] -= 1;
+  }
+  const int times = EXPW ? (int)floor(0.5 + (double)(R * Rt / Ld)) : (int)floor(0.5 + (double)(R / Ld));
+  const T R2 = R * R, Re2 = R * R * Rt * Rt;
+  const double halfL = (double)L / 2.0;
+  auto offset = [&](int a, int c) {
+    T d = (T)((double)x[a] - ((double)c + 0.5) * (double)Ld);
+    if ((double)d > halfL) d = L - d;
+    if ((double)d < -halfL) d = L + d;
+    return d;
+  };
+  double M = 0.0;
+  for (int i = -times; i <= times; i++) {
+    const int ci = wrap_any(xt[0] + i, nd);
+    const T d0 = offset(0, ci);
+    for (int j = -times; j <= times; j++) {
+      const int cj = wrap_any(xt[1] + j, nd);
+      const T d1 = offset(1, cj);
+      T* row = g + ((size_t)ci * nd + cj) * nd;
+      for (int k = -times; k <= times; k++) {
+        const int ck = wrap_any(xt[2] + k, nd);
+        const T d2 = offset(2, ck);
+        const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (!EXPW) {
+          if (r2 <= R2) {
+            atomic_add_t(row + ck, m);
+            M += (double)m;
+          }
+        } else if (r2 <= Re2) {
+          const T w = (T)exp(-0.5 * (double)r2 / (double)R / (double)R);
+          atomic_add_t(row + ck, m * w);
+          M += (double)m * (double)w;
+        }
+      }
+    }
+  }
+  return M;
+}
+
+// One thread per particle; wsum[t * nsub + s] receives the deposited weight
+// of each grid (the normalisation's total, in place of the particle count).
+template <typename T, bool EXPW>
+__global__ void k_deposit_sphere(const T* pos, const T* vel, const T* mass, const int* type,
+                                 long long np, int nd, T L, T Lb, T R, T Rt, int direction, T rsd,
+                                 int nsub, int ngrids, T* grid, double* wsum) {
+  __shared__ double sh[128];
+  const size_t ng = (size_t)nd * nd * nd;
+  const T Ld = L / (T)nd;
+  if (threadIdx.x < 128) sh[threadIdx.x] = 0.0;
+  __syncthreads();
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < np;
+       j += (long long)gridDim.x * blockDim.x) {
+    T x[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    if (direction >= 0) x[direction] = wrap_while<T>(x[direction] + vel[3 * j + direction] * rsd, L);
+    const int t = type ? type[j] : 0;
+    const T m = mass ? mass[j] : (T)1;
+    T p[3];
+    for (int a = 0; a < 3; a++) p[a] = wrap_while<T>(x[a], L);
+    if (p[0] < (T)0 || p[1] < (T)0 || p[2] < (T)0) continue;  // non-finite position
+    T* g = grid + (size_t)t * nsub * ng;
+    atomicAdd(&sh[t * nsub], dep_sphere<T, EXPW>(g, p, nd, L, Ld, R, Rt, m));
+    if (nsub == 2) {
+      const T sft = (T)(-(double)Lb / 2.0);
+      for (int a = 0; a < 3; a++) p[a] = wrap_while<T>(x[a] + sft, L);
+      atomicAdd(&sh[t * nsub + 1], dep_sphere<T, EXPW>(g + ng, p, nd, L, Ld, R, Rt, m));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ngrids && sh[threadIdx.x] != 0.0) atomicAdd(&wsum[threadIdx.x], sh[threadIdx.x]);
+}
+
 // ---------------------------------------------------------------------------
 // LDS-privatised deposit for particles stored in Lagrangian lattice order
 // (particle (i, j, k) of an nL^3 lattice at index (i nL + j) nL + k, as the
@@ -567,6 +655,16 @@ __global__ void k_normalise(T* grid, size_t ng, int nsub, const unsigned long lo
     g[i] = (T)((double)g[i] / mean - 1.0);
 }
 
+// grid (blockIdx.y): delta = rho / (W / ng) - 1, W the deposited weight
+template <typename T>
+__global__ void k_normalise_w(T* grid, size_t ng, const double* wsum) {
+  const double mean = wsum[blockIdx.y] / (double)ng;
+  T* g = grid + (size_t)blockIdx.y * ng;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ng;
+       i += (size_t)gridDim.x * blockDim.x)
+    g[i] = (T)((double)g[i] / mean - 1.0);
+}
+
 // Side of the particle lattice when np is a perfect cube, else 0.
 static int lattice_side(long long np) {
   if (np <= 0) return 0;
@@ -612,10 +710,11 @@ static int deposit_window_margin() {
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
-              int window, int interlacing, int folds, T* grid) {
-  EXH_REQUIRE(window == EXH_WINDOW_NGP || window == EXH_WINDOW_CIC || window == EXH_WINDOW_TSC,
-              EXH_ERR_UNSUPPORTED, "window %d: only NGP (1), CIC (2) and TSC (5) are implemented",
-              window);
+              int window, double R, double R_times, int interlacing, int folds, T* grid) {
+  EXH_REQUIRE(window >= EXH_WINDOW_NGP && window <= EXH_WINDOW_TSC, EXH_ERR_UNSUPPORTED,
+              "window %d: NGP (1), CIC (2), sphere (3), exponential (4) or TSC (5)", window);
+  EXH_REQUIRE(window < 3 || window > 4 || (R > 0 && (window == 3 || R_times > 0)), EXH_ERR_ARG,
+              "the sphere and exponential windows need R > 0 (and R_times > 0)");
   EXH_REQUIRE(nd > 0 && ntype >= 1 && folds >= 1, EXH_ERR_ARG, "bad grid arguments");
   EXH_REQUIRE(direction < 0 || vel != nullptr, EXH_ERR_ARG, "redshift space needs velocities");
   const int nsub = interlacing ? 2 : 1;
@@ -641,6 +740,30 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   // a grid side divisible by the number of tiles per axis, are deposited
   // tile by tile through LDS (correct for any order; fast when the array is
   // in lattice order, as LPT particles are)
+  if (window == 3 || window == 4) {
+    // spherical / exponential windows: normalised by the deposited weight
+    double* wsum = (double*)ctx_buf(c, BUF_GCNT, 128 * 8);
+    hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(128), 0, c->stream, wsum, (size_t)128);
+    {
+      TimedScope ts(c, window == 3 ? "sphere_deposit" : "exp_deposit");
+      const unsigned gs = std::min(grid_for(np, 256), 65536u);
+#define EXH_DEPS(E)                                                                              \
+  hipLaunchKernelGGL((k_deposit_sphere<T, E>), dim3(gs), dim3(256), 0, c->stream, pos, vel, mass, \
+                     type, np, nd, Lf, Lb, (T)R, (T)R_times, direction, rsd, nsub, ntype * nsub,  \
+                     grid, wsum)
+      if (window == 3) EXH_DEPS(false);
+      if (window == 4) EXH_DEPS(true);
+#undef EXH_DEPS
+      EXH_LAUNCH_CHECK();
+    }
+    if (!mass) {
+      TimedScope ts(c, "grid_normalise");
+      hipLaunchKernelGGL(k_normalise_w<T>, dim3(grid_for(ng, 256 * 8), ntype * nsub), dim3(256),
+                         0, c->stream, grid, ng, wsum);
+      EXH_LAUNCH_CHECK();
+    }
+    return;
+  }
   const int nL = lattice_side(np);
   const int nt = nL / TILE_L;
   const bool tiles = type == nullptr && nL >= TILE_L && nL % TILE_L == 0 && nd % nt == 0 &&
@@ -712,6 +835,8 @@ struct PowerArgs {
   const double* wx;   // per-axis window^-1 factors
   const double* wz;
   double Normk;
+  int rwin;         // 3 (sphere) or 4 (exponential): window per mode, else 0
+  double knT, RT;   // kn and R in the grid's precision
   // bin index arithmetic
   float knf, kminf, dkf;
   double kn, kmin, dk;
@@ -734,6 +859,23 @@ __device__ inline double shfl_up_d(double v, int o) {
 }
 
 static constexpr int PW_MAXV = 64;  // max values per mode (pairs x multipoles + 1)
+
+// W of the sphere (3) and exponential (4) windows at mode (fi, fj, fk), in
+// the grid's precision with the reference's double intermediates
+// (spectrum_h.c:60-67).
+template <typename T>
+__device__ inline T window_r(const PowerArgs& a, int fi, int fj, int fk) {
+  const T kn = (T)a.knT, R = (T)a.RT;
+  const T k1 = (T)fi * kn, k2 = (T)fj * kn, k3 = (T)fk * kn;
+  if (a.rwin == 3) {
+    const T kmod = (T)sqrt((double)(k1 * k1 + k2 * k2 + k3 * k3));
+    const T kR = kmod * R;
+    const T sc = kR != (T)0 ? (T)(sin((double)kR) / (double)kR) : (T)1;
+    return (T)(3.0 / kmod / kmod / R / R * ((double)sc - cos((double)kR)));
+  }
+  const T kmod = k1 * k1 + k2 * k2 + k3 * k3;
+  return (T)exp((double)(-kmod * R * R) / 2.0);
+}
 
 template <typename T>
 __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, double* part,
@@ -824,7 +966,8 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
       const int nt = a.ntype;
       if (valid) {
         const double2 e1 = a.ex[i], e2 = a.ex[j], e3 = a.ez[k];
-        const double wk = a.wx[i] * a.wx[j] * a.wz[k] * a.Normk;
+        const double wk = a.rwin ? a.Normk / (double)window_r<T>(a, fi, fj, (2 * k < nd) ? k : k - nd)
+                                 : a.wx[i] * a.wx[j] * a.wz[k] * a.Normk;
         // e^{-i theta} = e1 e2 e3
         double pr = e1.x * e2.x - e1.y * e2.y, pi = e1.x * e2.y + e1.y * e2.x;
         double er = pr * e3.x - pi * e3.y, ei = pr * e3.y + pi * e3.x;
@@ -928,7 +1071,7 @@ static double sinc_t(T x) {
 }
 
 template <typename T>
-void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window,
+void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window, double R,
                int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
                int kbin_precision, double* k_out, double* P_out, long long* count_out,
                bool deferred) {
@@ -936,8 +1079,8 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
   EXH_REQUIRE(nd >= 2 && nd % 2 == 0, EXH_ERR_ARG, "nd must be even");
   EXH_REQUIRE(ntype >= 1 && ntype <= 8, EXH_ERR_UNSUPPORTED, "1 to 8 tracer types supported");
   EXH_REQUIRE(Nk >= 1 && l_max >= 0, EXH_ERR_ARG, "bad binning");
-  EXH_REQUIRE(window == 0 || window == 1 || window == 2 || window == 5, EXH_ERR_UNSUPPORTED,
-              "window %d cannot be deconvolved (Sphere/Exp windows are not implemented)", window);
+  EXH_REQUIRE(window >= 0 && window <= 5, EXH_ERR_UNSUPPORTED, "window %d cannot be deconvolved",
+              window);
   const int nsub = interlacing ? 2 : 1;
   const int nz = nd / 2 + 1;
   const size_t nh = (size_t)nd * nd * nz;
@@ -952,7 +1095,9 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
   T kn = (T)(2 * M_PI / (double)Lr);
   T Lb = Lr / (T)nd;
   T Normk = (T)std::pow((double)(Lr / (T)(nd * nd)), 3.0 / 2.0);
-  int pw = window == 0 ? 0 : (window == 1 ? 1 : (window == 2 ? 2 : 3));
+  // NGP, CIC, TSC: per-axis sinc powers; the sphere and exponential windows
+  // are functions of |k|, evaluated per mode in the kernel
+  int pw = (window == 0 || window == 3 || window == 4) ? 0 : (window == 1 ? 1 : (window == 2 ? 2 : 3));
   std::vector<double2> ex(nd), ez(nz);
   std::vector<double> wx(nd), wz(nz);
   for (int i = 0; i < nd; i++) {
@@ -996,6 +1141,9 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
   a.wx = dwx;
   a.wz = dwz;
   a.Normk = (double)Normk;
+  a.rwin = (window == 3 || window == 4) ? window : 0;
+  a.knT = (double)kn;
+  a.RT = (double)(T)R;
   // bin edges: fft_real arithmetic for kbin 4 (spectrum_h.c:39-48, powermodule.c:13)
   a.knf = (float)(2 * M_PI / (double)(float)L);
   a.kminf = (float)k_min;
@@ -1079,14 +1227,15 @@ void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_o
 }
 
 template void run_grid<float>(exh_ctx*, const float*, const float*, const float*, const int*,
-                              long long, int, int, double, double, double, int, int, int, int,
-                              float*);
+                              long long, int, int, double, double, double, int, int, double,
+                              double, int, int, float*);
 template void run_grid<double>(exh_ctx*, const double*, const double*, const double*, const int*,
-                               long long, int, int, double, double, double, int, int, int, int,
-                               double*);
-template void run_power<float>(exh_ctx*, const float*, int, int, double, int, int, int, double,
-                               double, int, int, int, double*, double*, long long*, bool);
-template void run_power<double>(exh_ctx*, const double*, int, int, double, int, int, int, double,
-                                double, int, int, int, double*, double*, long long*, bool);
+                               long long, int, int, double, double, double, int, int, double,
+                               double, int, int, double*);
+template void run_power<float>(exh_ctx*, const float*, int, int, double, int, double, int, int,
+                               double, double, int, int, int, double*, double*, long long*, bool);
+template void run_power<double>(exh_ctx*, const double*, int, int, double, int, double, int, int,
+                                double, double, int, int, int, double*, double*, long long*,
+                                bool);
 
 }  // namespace exh

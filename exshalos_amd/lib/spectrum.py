@@ -5,8 +5,8 @@ grid_compute, power_compute, with the same positional arguments.
 Additions (keyword-only): ``device_out`` (return torch device tensors) and
 ``kbin_precision`` for power_compute (4: the reference's float bin index,
 the default in FP32 mode; 8: double).  Inputs may be numpy arrays or torch
-device tensors.  Divergences (documented): windows 3/4 (Sphere, Exp) raise
-NotImplementedError; window 5 is TSC (not in the reference); positions are
+device tensors.  Divergences (documented): window 5 is TSC (not in the
+reference); positions are
 not modified in place by the redshift-space shift (gridmodule.c:167-169
 edits the caller's array); type indices must lie in [0, ntype).
 """
@@ -51,7 +51,8 @@ def grid_compute(pos, vel, mass, nmass, type, ntype, nd, L, Om0, z, direction, w
                        dtype=torch.float32 if prec == 4 else torch.float64, device=p.device)
     N.check(N.load().exh_grid_compute(ctx, N.ptr(p), N.ptr(v), N.ptr(m), N.ptr(t), npart, ntype,
                                       int(nd), float(L), float(Om0), float(z), direction,
-                                      int(window), il, int(folds), prec, N.ptr(grid)),
+                                      int(window), float(R), float(R_times), il, int(folds), prec,
+                                      N.ptr(grid)),
             "grid_compute")
     return grid if device_out else to_numpy(grid)
 
@@ -72,9 +73,9 @@ def power_compute(grid, ntype, nd, L, window, R, interlacing, Nk, k_min, k_max, 
     Pout = np.zeros((NPs, ls, Nk), dtype=np.float64)
     cnt = np.zeros(Nk, dtype=np.int64)
     kb = prec if kbin_precision is None else int(kbin_precision)
-    N.check(N.load().exh_power_compute(ctx, N.ptr(g), ntype, nd, float(L), int(window), il, Nk,
-                                       float(k_min), float(k_max), l_max, int(direction), prec,
-                                       kb, N.ptr(kout), N.ptr(Pout), N.ptr(cnt)),
+    N.check(N.load().exh_power_compute(ctx, N.ptr(g), ntype, nd, float(L), int(window), float(R),
+                                       il, Nk, float(k_min), float(k_max), l_max, int(direction),
+                                       prec, kb, N.ptr(kout), N.ptr(Pout), N.ptr(cnt)),
             "power_compute")
-    R = np_real(prec)
-    return {"k": kout.astype(R), "Pk": Pout.astype(R), "Nk": cnt}
+    Rt = np_real(prec)
+    return {"k": kout.astype(Rt), "Pk": Pout.astype(Rt), "Nk": cnt}

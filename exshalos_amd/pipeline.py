@@ -90,10 +90,11 @@ class BoxPipeline:
                     "halos_box_from_pk")
             ng = self.nd**3
             N.check(lib.exh_grid_compute(self.ctx, N.ptr(self.pos), None, None, None, ng, 1,
-                                         self.nd, self.L, 0.31, 0.0, -1, self.window, self.il, 1,
-                                         self.prec, N.ptr(self.grid)), "grid_compute")
+                                         self.nd, self.L, 0.31, 0.0, -1, self.window, 4.0,
+                                         5.0, self.il, 1, self.prec, N.ptr(self.grid)),
+                    "grid_compute")
             N.check(lib.exh_power_compute(self.ctx, N.ptr(self.grid), 1, self.nd, self.L,
-                                          self.window, self.il, self.Nk, kmin, kmax, 0, -1,
+                                          self.window, 4.0, self.il, self.Nk, kmin, kmax, 0, -1,
                                           self.prec, self.kbin, N.ptr(self.kout),
                                           N.ptr(self.Pout), N.ptr(self.cnt)), "power_compute")
         self.nh = nh.value

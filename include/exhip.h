@@ -47,6 +47,8 @@ extern "C" {
 #define EXH_WINDOW_NONE 0
 #define EXH_WINDOW_NGP 1
 #define EXH_WINDOW_CIC 2
+#define EXH_WINDOW_SPHERE 3      /* gridmodule.c:57-84, spectrum_h.c:60-63 */
+#define EXH_WINDOW_EXPONENTIAL 4 /* gridmodule.c:86-119, spectrum_h.c:64-67 */
 #define EXH_WINDOW_TSC 5 /* not in the reference; deconvolved by sinc^6 */
 
 /* Field sources accepted by the LPT and pipeline entry points. */
@@ -205,27 +207,30 @@ int exh_halos_box_power(exh_ctx* ctx, const exh_box* box, const exh_field_src* m
 /* -------------------------------------------------------------- spectrum */
 /* Mass assignment: replaces grid_compute (spectrum.c:46-108) / Tracer_Grid
  * (gridmodule.c:146-375).  pos/vel: device [np][3] T; mass: device [np] T
- * or NULL; type: device [np] int32 or NULL (ntype == 1).  window: NGP, CIC
- * or TSC.  grid: device [ntype][1+interlacing][nd^3] T, overwritten.
- * Unweighted grids are normalised to rho/rho_bar - 1, weighted grids are
- * left as mass sums (gridmodule.c:311). */
+ * or NULL; type: device [np] int32 or NULL (ntype == 1).  window: NGP, CIC,
+ * SPHERE (radius R), EXPONENTIAL (scale R, cut at R * R_times) or TSC; R
+ * and R_times are read only by the last two, as in the reference.  grid:
+ * device [ntype][1+interlacing][nd^3] T, overwritten.  Unweighted grids
+ * are normalised to rho/rho_bar - 1 by the deposited weight, weighted grids
+ * are left as mass sums (gridmodule.c:311). */
 int exh_grid_compute(exh_ctx* ctx, const void* pos, const void* vel, const void* mass,
                      const int* type, long long np, int ntype, int nd, double L,
-                     double Om0, double z, int direction, int window, int interlacing,
-                     int folds, int precision, void* grid);
+                     double Om0, double z, int direction, int window, double R,
+                     double R_times, int interlacing, int folds, int precision, void* grid);
 
 /* Power spectra of all tracer pairs: replaces power_compute
  * (spectrum.c:111-197) / Compute_gridk + Power_Spectrum
  * (spectrum_h.c:83-167, powermodule.c:4-108).  grid: device
  * [ntype][1+interlacing][nd^3] T (not modified).  Outputs (host):
  * k_out[Nk], P_out[NPs][l_max/2+1][Nk], count_out[Nk] with NPs =
- * ntype(ntype+1)/2.  kbin_precision selects the arithmetic of the bin
- * index: 4 follows the reference's float evaluation, 8 double.  Bin sums
- * are accumulated in double. */
+ * ntype(ntype+1)/2.  window: the assignment to deconvolve (R: the
+ * smoothing length of SPHERE and EXPONENTIAL).  kbin_precision selects the
+ * arithmetic of the bin index: 4 follows the reference's float evaluation,
+ * 8 double.  Bin sums are accumulated in double. */
 int exh_power_compute(exh_ctx* ctx, const void* grid, int ntype, int nd, double L,
-                      int window, int interlacing, int Nk, double k_min, double k_max,
-                      int l_max, int direction, int precision, int kbin_precision,
-                      double* k_out, double* P_out, long long* count_out);
+                      int window, double R, int interlacing, int Nk, double k_min,
+                      double k_max, int l_max, int direction, int precision,
+                      int kbin_precision, double* k_out, double* P_out, long long* count_out);
 
 #ifdef __cplusplus
 }

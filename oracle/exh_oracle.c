@@ -896,11 +896,55 @@ static void deposit_tsc(REAL *g, const REAL *x, int nd, REAL L, REAL m) {
       }
 }
 
-static long double deposit_one(REAL *g, int nd, REAL L, const REAL *x, REAL m, int window) {
+/* Spherical (window 3) and exponential (window 4) assignment
+ * (gridmodule.c:57-119): every cell of the cube of half-side
+ * times = floor(1/2 + R_eff/Ld) around the particle's cell whose centre lies
+ * within R_eff (minimum image per axis as the reference writes it, L - dx)
+ * gets m, or m exp(-r^2 / 2R^2) with R_eff = R R_times.  The offsets are
+ * evaluated in double and stored in REAL, r^2 in REAL, as in the reference.
+ * Returns the deposited weight. */
+static long double deposit_sphere(REAL *g, const REAL *x, int nd, REAL L, REAL R, REAL R_times,
+                                  REAL m, int expw) {
+  REAL Ld = L / nd;
+  int xt[3];
+  cell_index(x, xt, Ld, nd);
+  const int times = expw ? (int)floor(0.5 + R * R_times / Ld) : (int)floor(0.5 + R / Ld);
+  long double M = 0.0;
+  for (int i = -times; i <= times; i++)
+    for (int j = -times; j <= times; j++)
+      for (int k = -times; k <= times; k++) {
+        const int c[3] = {wrap_int(xt[0], i, nd), wrap_int(xt[1], j, nd),
+                          wrap_int(xt[2], k, nd)};
+        REAL dx[3];
+        for (int a = 0; a < 3; a++) {
+          dx[a] = x[a] - (c[a] + 1.0 / 2.0) * Ld;
+          if (dx[a] > L / 2.0) dx[a] = L - dx[a];
+          if (dx[a] < -L / 2.0) dx[a] = L + dx[a];
+        }
+        REAL r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+        size_t at = (size_t)c[0] * nd * nd + (size_t)c[1] * nd + c[2];
+        if (!expw) {
+          if (r2 <= R * R) {
+            g[at] += m;
+            M += (long double)m;
+          }
+        } else if (r2 <= R * R * R_times * R_times) {
+          REAL w = exp(-0.5 * r2 / R / R);
+          g[at] += m * w;
+          M += (long double)m * w;
+        }
+      }
+  return M;
+}
+
+static long double deposit_one(REAL *g, int nd, REAL L, const REAL *x, REAL m, int window,
+                               REAL R, REAL R_times) {
   if (window == 1)
     deposit_ngp(g, x, nd, L, m);
   else if (window == 2)
     deposit_cic(g, x, nd, L, m);
+  else if (window == 3 || window == 4)
+    return deposit_sphere(g, x, nd, L, R, R_times, m, window == 4);
   else
     deposit_tsc(g, x, nd, L, m);
   return (long double)m;
@@ -908,14 +952,15 @@ static long double deposit_one(REAL *g, int nd, REAL L, const REAL *x, REAL m, i
 
 /* Tracer_Grid (gridmodule.c:146-375) with nthreads = 1 semantics (the
  * reference's multi-threaded scatter races, gridmodule.c:181-187).
- * window: 1 NGP, 2 CIC, 5 TSC.  grid: [ntype][1+interlacing][nd^3], zeroed
- * by the caller.  pos is modified in place by the redshift-space shift, as
- * in the reference.  Returns 0, or -1 for an unsupported window. */
+ * window: 1 NGP, 2 CIC, 3 sphere, 4 exponential (R, R_times), 5 TSC.
+ * grid: [ntype][1+interlacing][nd^3], zeroed by the caller.  pos is
+ * modified in place by the redshift-space shift, as in the reference.
+ * Returns 0, or -1 for an unsupported window. */
 int FN(ora_tracer_grid)(REAL *grid, int nd, REAL L, int direction, REAL *pos,
                         const REAL *vel, long long np, const REAL *mass, const int *type,
-                        int ntype, int window, int interlacing, REAL Om0, REAL z,
-                        int folds) {
-  if (window != 1 && window != 2 && window != 5) return -1;
+                        int ntype, int window, REAL R, REAL R_times, int interlacing, REAL Om0,
+                        REAL z, int folds) {
+  if (window < 1 || window > 5) return -1;
   L = L / folds;
   REAL Lb = L / nd;
   size_t ng = (size_t)nd * nd * nd;
@@ -934,10 +979,11 @@ int FN(ora_tracer_grid)(REAL *grid, int nd, REAL L, int direction, REAL *pos,
     REAL m = mass ? mass[j] : 1.0;
     REAL post[3];
     for (int a = 0; a < 3; a++) post[a] = wrap_while(pos[3 * j + a], 0.0, L);
-    M1[t] += deposit_one(grid + ((size_t)t * nsub) * ng, nd, L, post, m, window);
+    M1[t] += deposit_one(grid + ((size_t)t * nsub) * ng, nd, L, post, m, window, R, R_times);
     if (interlacing) {
       for (int a = 0; a < 3; a++) post[a] = wrap_while(pos[3 * j + a], -Lb / 2.0, L);
-      M2[t] += deposit_one(grid + ((size_t)t * nsub + 1) * ng, nd, L, post, m, window);
+      M2[t] += deposit_one(grid + ((size_t)t * nsub + 1) * ng, nd, L, post, m, window, R,
+                           R_times);
     }
   }
   if (!mass) {
@@ -964,12 +1010,21 @@ static REAL sincr(REAL x) {
   return r;
 }
 
-static REAL window_k(REAL k1, REAL k2, REAL k3, REAL Lb, int window) {
-  REAL r = 1.0;
+/* W (spectrum_h.c:51-70); windows 3 and 4 with the smoothing length R. */
+static REAL window_k(REAL k1, REAL k2, REAL k3, REAL Lb, REAL R, int window) {
+  REAL r = 1.0, kmod;
   if (window == 0) r = 1.0;
   if (window == 1) r = sincr(k1 * Lb / 2.0) * sincr(k2 * Lb / 2.0) * sincr(k3 * Lb / 2.0);
   if (window == 2)
     r = pow(sincr(k1 * Lb / 2.0) * sincr(k2 * Lb / 2.0) * sincr(k3 * Lb / 2.0), 2.0);
+  if (window == 3) {
+    kmod = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+    r = 3.0 / kmod / kmod / R / R * (sincr(kmod * R) - cos(kmod * R));
+  }
+  if (window == 4) {
+    kmod = k1 * k1 + k2 * k2 + k3 * k3;
+    r = exp(-kmod * R * R / 2.0);
+  }
   if (window == 5)
     r = pow(sincr(k1 * Lb / 2.0) * sincr(k2 * Lb / 2.0) * sincr(k3 * Lb / 2.0), 3.0);
   return r;
@@ -990,7 +1045,8 @@ static double legendre(int l, double x) {
 /* Compute_gridk for one tracer (spectrum_h.c:83-167): f1 (and f2 for the
  * interlaced copy, may be NULL) are the raw r2c outputs [nd][nd][nd/2+1][2];
  * out receives the combined, deconvolved, normalised modes. */
-void FN(ora_gridk)(int nd, REAL L, int window, const REAL *f1, const REAL *f2, REAL *out) {
+void FN(ora_gridk)(int nd, REAL L, int window, REAL R, const REAL *f1, const REAL *f2,
+                   REAL *out) {
   REAL Normk = pow(L / (nd * nd), 3.0 / 2.0);
   REAL kn = 2 * M_PI / L;
   REAL Lb = L / nd;
@@ -1002,7 +1058,7 @@ void FN(ora_gridk)(int nd, REAL L, int window, const REAL *f1, const REAL *f2, R
       for (int k = 0; k < nz; k++) {
         REAL kz = (2 * k < nd) ? k * kn : (k - nd) * kn;
         size_t t = (size_t)nd * nz * i + (size_t)nz * j + k;
-        REAL W = window_k(kx, ky, kz, Lb, window);
+        REAL W = window_k(kx, ky, kz, Lb, R, window);
         if (f2) {
           out[2 * t] = (f1[2 * t] + f2[2 * t] * cos((kx + ky + kz) * Lb / 2.0) +
                         f2[2 * t + 1] * sin((kx + ky + kz) * Lb / 2.0)) /

@@ -341,30 +341,33 @@ def halos_box_from_pk(k, P, nd, Lc, noise, Om0=0.31, z=0.0, k_smooth=1e4, dc=-1.
 # deposit and power spectrum
 # --------------------------------------------------------------------------
 def grid_compute(pos, nd, L, window=2, interlacing=False, mass=None, types=None,
-                 ntype=1, direction=-1, vel=None, Om0=0.31, z=0.0, folds=1, precision=8):
-    """Tracer_Grid: returns grid [ntype][1+interlacing][nd][nd][nd]."""
-    R = _real(precision)
-    pos = np.ascontiguousarray(pos, dtype=R).copy()
+                 ntype=1, direction=-1, vel=None, Om0=0.31, z=0.0, folds=1, precision=8,
+                 R=4.0, R_times=5.0):
+    """Tracer_Grid: returns grid [ntype][1+interlacing][nd][nd][nd].  window 3
+    (sphere) and 4 (exponential) use R and R_times (gridmodule.c:57-119)."""
+    dt = _real(precision)
+    pos = np.ascontiguousarray(pos, dtype=dt).copy()
     npart = pos.shape[0]
-    grid = np.zeros((ntype, 1 + int(interlacing), nd, nd, nd), dtype=R)
-    mass = None if mass is None else np.ascontiguousarray(mass, dtype=R)
+    grid = np.zeros((ntype, 1 + int(interlacing), nd, nd, nd), dtype=dt)
+    mass = None if mass is None else np.ascontiguousarray(mass, dtype=dt)
     types = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
-    vel = None if vel is None else np.ascontiguousarray(vel, dtype=R)
+    vel = None if vel is None else np.ascontiguousarray(vel, dtype=dt)
     RC = _rc(precision)
     rc = getattr(lib(), "ora_tracer_grid" + _sfx(precision))(
         _ptr(grid), C.c_int(nd), RC(L), C.c_int(direction), _ptr(pos), _ptr(vel),
-        C.c_longlong(npart), _ptr(mass), _ptr(types), C.c_int(ntype), C.c_int(window),
-        C.c_int(int(interlacing)), RC(Om0), RC(z), C.c_int(folds))
+        C.c_longlong(npart), _ptr(mass), _ptr(types), C.c_int(ntype), C.c_int(window), RC(R),
+        RC(R_times), C.c_int(int(interlacing)), RC(Om0), RC(z), C.c_int(folds))
     if rc != 0:
         raise ValueError(f"oracle tracer_grid failed ({rc})")
     return grid
 
 
 def power_compute(grid, ntype, nd, L, window=2, interlacing=False, Nk=25, k_min=None,
-                  k_max=None, l_max=0, direction=-1, precision=8):
-    """Compute_gridk + Power_Spectrum.  grid: [ntype][1+il][nd][nd][nd]."""
-    R = _real(precision)
-    grid = np.asarray(grid, dtype=R).reshape(ntype, 1 + int(interlacing), nd, nd, nd)
+                  k_max=None, l_max=0, direction=-1, precision=8, R=4.0):
+    """Compute_gridk + Power_Spectrum.  grid: [ntype][1+il][nd][nd][nd].  R: the
+    smoothing length of windows 3 and 4 (spectrum_h.c:51-70)."""
+    dt = _real(precision)
+    grid = np.asarray(grid, dtype=dt).reshape(ntype, 1 + int(interlacing), nd, nd, nd)
     if k_min is None:
         k_min = 2.0 * np.pi / L
     if k_max is None:
@@ -373,11 +376,11 @@ def power_compute(grid, ntype, nd, L, window=2, interlacing=False, Nk=25, k_min=
     sfx = _sfx(precision)
     gks = []
     for t in range(ntype):
-        f1 = np.ascontiguousarray(r2c(grid[t, 0]).astype(R))
-        f2 = np.ascontiguousarray(r2c(grid[t, 1]).astype(R)) if interlacing else None
+        f1 = np.ascontiguousarray(r2c(grid[t, 0]).astype(dt))
+        f2 = np.ascontiguousarray(r2c(grid[t, 1]).astype(dt)) if interlacing else None
         out = np.zeros_like(f1)
-        getattr(lib(), "ora_gridk" + sfx)(C.c_int(nd), RC(L), C.c_int(window), _ptr(f1),
-                                          _ptr(f2), _ptr(out))
+        getattr(lib(), "ora_gridk" + sfx)(C.c_int(nd), RC(L), C.c_int(window), RC(R),
+                                          _ptr(f1), _ptr(f2), _ptr(out))
         gks.append(out)
     arr = (C.c_void_p * ntype)(*[g.ctypes.data for g in gks])
     NPs = ntype * (ntype + 1) // 2
@@ -386,6 +389,6 @@ def power_compute(grid, ntype, nd, L, window=2, interlacing=False, Nk=25, k_min=
     Km = np.zeros(Nk, dtype=np.float64)
     cnt = np.zeros(Nk, dtype=np.int64)
     getattr(lib(), "ora_power_bins" + sfx)(
-        arr, C.c_int(ntype), C.c_int(nd), RC(L), C.c_int(Nk), RC(R(k_min)), RC(R(k_max)),
+        arr, C.c_int(ntype), C.c_int(nd), RC(L), C.c_int(Nk), RC(dt(k_min)), RC(dt(k_max)),
         C.c_int(l_max), C.c_int(direction), _ptr(Pk), _ptr(Km), _ptr(cnt))
-    return dict(k=Km.astype(R), Pk=Pk.astype(R), Nk=cnt, Pk64=Pk, k64=Km)
+    return dict(k=Km.astype(dt), Pk=Pk.astype(dt), Nk=cnt, Pk64=Pk, k64=Km)

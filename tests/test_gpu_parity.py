@@ -197,8 +197,40 @@ def test_grid_compute_weighted_types_rsd(prec):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("window,R,R_times", [(3, 5.0, 1.0), (3, 9.0, 1.0), (4, 3.0, 2.0)])
+@pytest.mark.parametrize("il", [0, 1])
+def test_grid_compute_sphere_exponential(prec, window, R, R_times, il):
+    """Windows 3 and 4 (gridmodule.c:57-119): every cell centre within R (or
+    R R_times, weighted by exp(-r^2 / 2R^2)); unweighted grids normalised by
+    the deposited weight.  Then masses, three tracer types and the
+    redshift-space shift on the same kernel."""
+    set_precision(prec)
+    rng = np.random.default_rng(21 + window + il)
+    nd, L, n = 24, 96.0, 3000
+    RT = np.float32 if prec == 4 else np.float64
+    pos = (rng.random((n, 3)) * L).astype(RT)
+    pos[0] = (0.001, L - 0.001, 0.5 * L / nd)  # near the edges and on a cell centre
+    ref = O.grid_compute(pos, nd, L, window=window, interlacing=bool(il), precision=prec, R=R,
+                         R_times=R_times)
+    g = SP.grid_compute(pos, None, None, 0, None, 1, nd, L, 0.31, 0.0, -1, window, R, R_times, il,
+                        1, 0, 1)
+    tol = 2e-5 if prec == 4 else 1e-12
+    assert g.shape == ref.shape
+    assert _rel(g, ref) < tol
+    vel = (rng.normal(size=(n, 3)) * 300).astype(RT)
+    mass = rng.random(n).astype(RT)
+    types = rng.integers(0, 3, n).astype(np.int32)
+    ref = O.grid_compute(pos, nd, L, window=window, interlacing=bool(il), mass=mass, types=types,
+                         ntype=3, direction=2, vel=vel, precision=prec, R=R, R_times=R_times)
+    g = SP.grid_compute(pos, vel, mass, 1, types, 3, nd, L, 0.31, 0.0, 2, window, R, R_times, il,
+                        1, 0, 1)
+    assert _rel(g, ref) < tol
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("window,il,ntype,lmax,direction",
-                         [(2, 0, 1, 0, -1), (2, 1, 1, 0, -1), (5, 1, 2, 4, 2), (1, 0, 3, 2, 0)])
+                         [(2, 0, 1, 0, -1), (2, 1, 1, 0, -1), (5, 1, 2, 4, 2), (1, 0, 3, 2, 0),
+                          (3, 0, 1, 2, 1), (4, 1, 2, 0, -1)])
 def test_power_compute(prec, window, il, ntype, lmax, direction):
     set_precision(prec)
     rng = np.random.default_rng(9)

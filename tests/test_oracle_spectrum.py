@@ -3,16 +3,20 @@ power spectrum (spectrum_h.c:30-167, powermodule.c:4-108) on the CPU.
 
 Mass assignment is compared with the textbook kernels evaluated at every
 cell centre (NGP: the containing cell; CIC: prod(1 - |d|); TSC: 3/4 - d^2
-or (3/2 - |d|)^2 / 2), with mass conservation checked by hypothesis.  The
-power spectrum is compared with a single Fourier mode, whose power,
-window, interlacing and multipole factors are known in closed form, and
-with mode counts enumerated by numpy.
+or (3/2 - |d|)^2 / 2; sphere: the cells whose centre lies within R;
+exponential: exp(-r^2 / 2R^2) within R R_times), with mass conservation
+checked by hypothesis.  The power spectrum is compared with a single
+Fourier mode, whose power, window (sinc powers; 3 j1(kR)/(kR) for the
+sphere; exp(-k^2 R^2 / 2)), interlacing and multipole factors are known in
+closed form, and with mode counts enumerated by numpy.
 """
 
 import numpy as np
 import pytest
 from hypothesis import given, settings
 from hypothesis import strategies as st
+
+from scipy.special import spherical_jn
 
 from oracle import pipeline as O
 
@@ -52,6 +56,53 @@ def test_deposit_weights_match_kernels(window):
     mass = rng.uniform(0.5, 2.0, 40)
     g = O.grid_compute(pos, nd, L, window=window, mass=mass)[0, 0]
     np.testing.assert_allclose(g, _brute_grid(pos, nd, L, window, mass), atol=1e-12)
+
+
+def _brute_sphere(pos, nd, L, R, R_times, expw, mass=None):
+    """Every cell centre at periodic distance r from the particle: m if
+    r <= R (sphere), m exp(-r^2 / 2R^2) if r <= R R_times (exponential)."""
+    Ld = L / nd
+    c = (np.indices((nd, nd, nd)) + 0.5) * Ld
+    g = np.zeros((nd, nd, nd))
+    M = 0.0
+    for n, x in enumerate(pos):
+        d = x[:, None, None, None] - c
+        d = (d + L / 2) % L - L / 2
+        r2 = (d * d).sum(axis=0)
+        m = 1.0 if mass is None else mass[n]
+        if expw:
+            w = np.where(r2 <= (R * R_times) ** 2, np.exp(-0.5 * r2 / R**2), 0.0)
+        else:
+            w = (r2 <= R * R).astype(float)
+        g += m * w
+        M += m * w.sum()
+    return g, M
+
+
+@pytest.mark.parametrize("window,R,R_times", [(3, 5.0, 1.0), (3, 9.0, 1.0), (4, 3.0, 2.0),
+                                              (4, 4.5, 1.5)])
+def test_sphere_and_exponential_deposits(window, R, R_times):
+    """Windows 3 and 4 (gridmodule.c:57-119) against every cell centre: with
+    masses the grid is the weighted sum; without, it is normalised by the
+    deposited weight (not the particle count) to rho / rho_bar - 1."""
+    nd, L = 12, 48.0
+    rng = np.random.default_rng(window + int(R))
+    pos = rng.uniform(0, L, (25, 3))
+    pos[0] = [0.01, L - 0.02, 47.999]  # near the periodic edges
+    pos[1] = [2.0, 2.0, 2.0]           # a cell centre
+    mass = rng.uniform(0.5, 2.0, 25)
+    g = O.grid_compute(pos, nd, L, window=window, mass=mass, R=R, R_times=R_times)[0, 0]
+    ref, _ = _brute_sphere(pos, nd, L, R, R_times, window == 4, mass)
+    np.testing.assert_allclose(g, ref, atol=1e-12)
+    g = O.grid_compute(pos, nd, L, window=window, R=R, R_times=R_times)[0, 0]
+    ref, M = _brute_sphere(pos, nd, L, R, R_times, window == 4)
+    np.testing.assert_allclose(g, ref / (M / nd**3) - 1.0, atol=1e-12)
+    if window == 3:  # a particle at a cell centre fills the lattice ball
+        one = O.grid_compute(pos[1:2], nd, L, window=3, mass=np.ones(1), R=R)[0, 0]
+        i = np.arange(-4, 5)
+        n_ball = int(((i[:, None, None] ** 2 + i[None, :, None] ** 2 + i[None, None, :] ** 2)
+                      * (L / nd) ** 2 <= R * R).sum())
+        assert one.sum() == n_ball
 
 
 def test_single_particle_at_a_cell_centre():
@@ -152,7 +203,7 @@ def test_total_mode_count_closed_form():
     assert out["Nk"][0] == nd * nd * (nd // 2 - 1) + 2 * (nd // 2 - 1) * (nd - 2)
 
 
-@pytest.mark.parametrize("window", [0, 1, 2, 5])
+@pytest.mark.parametrize("window", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("il", [False, True])
 def test_single_z_mode_power(window, il):
     """grid = A cos(2 pi m z / nd): the only non-zero kept mode is (0, 0, m)
@@ -161,19 +212,26 @@ def test_single_z_mode_power(window, il):
     spectrum_h.c:58-59), multiplied by cos^2(theta/2), theta = pi m / nd,
     when an identical interlaced copy is combined (spectrum_h.c:130-131),
     and averaged over the bin's modes; the l = 2 multipole with the z
-    direction has P_2(1) = 1 and the factor 5."""
-    nd, L, Nk, m, A = 16, 64.0, 8, 3, 0.6
+    direction has P_2(1) = 1 and the factor 5.  Windows 3 and 4 divide by
+    the sphere's 3 j1(kR)/(kR) and by exp(-k^2 R^2 / 2) (spectrum_h.c:60-67)."""
+    nd, L, Nk, m, A, R = 16, 64.0, 8, 3, 0.6, 5.0
     kmin, kmax = 2 * np.pi / L, np.pi / L * nd
     z = np.indices((nd, nd, nd))[2]
     g = A * np.cos(2 * np.pi * m * z / nd)
     grid = np.stack([g, g])[None] if il else g[None, None]
     out = O.power_compute(grid, 1, nd, L, window=window, interlacing=il, Nk=Nk, k_min=kmin,
-                          k_max=kmax, l_max=2, direction=2)
+                          k_max=kmax, l_max=2, direction=2, R=R)
     kmod, b = _mode_table(nd, L, kmin, kmax, Nk)
     counts = np.bincount(b, minlength=Nk)
     bm = int(np.floor((2 * np.pi * m / L - kmin) / ((kmax - kmin) / Nk)))
     u = np.pi * m / nd
-    W = 1.0 if window == 0 else (np.sin(u) / u) ** {1: 1, 2: 2, 5: 3}[window]
+    kz = 2 * np.pi * m / L
+    if window == 3:
+        W = 3 * spherical_jn(1, kz * R) / (kz * R)
+    elif window == 4:
+        W = np.exp(-0.5 * (kz * R) ** 2)
+    else:
+        W = 1.0 if window == 0 else (np.sin(u) / u) ** {1: 1, 2: 2, 5: 3}[window]
     P = A**2 * L**3 / 4 / W**2 / counts[bm]
     if il:
         P *= np.cos(u / 2) ** 2
@@ -183,7 +241,7 @@ def test_single_z_mode_power(window, il):
     np.testing.assert_allclose(out["Pk64"][0, 1], 5 * expect0, rtol=1e-10, atol=1e-12 * P)
     # along x, mu = 0 and P_2(0) = -1/2
     out = O.power_compute(grid, 1, nd, L, window=window, interlacing=il, Nk=Nk, k_min=kmin,
-                          k_max=kmax, l_max=2, direction=0)
+                          k_max=kmax, l_max=2, direction=0, R=R)
     np.testing.assert_allclose(out["Pk64"][0, 1], -2.5 * expect0, rtol=1e-10, atol=1e-12 * P)
 
 
