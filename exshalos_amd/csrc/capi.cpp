@@ -586,3 +586,26 @@ int exh_power_compute(exh_ctx* c, const void* grid, int ntype, int nd, double L,
 }
 
 }  // extern "C"
+
+int exh_abundance_compute(exh_ctx* c, const void* Mh, long long nh, double Mmin, double Mmax,
+                          int Nm, double Lc, int ndx, int ndy, int ndz, int precision,
+                          double* Mmean_out, double* dn_out, double* dn_err_out) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_REQUIRE(precision == 4 || precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
+  EXH_REQUIRE(Lc > 0 && ndx > 0 && ndy > 0 && ndz > 0, EXH_ERR_ARG, "bad box");
+  if (!Mh) {  // the masses of the context's last halo catalogue
+    EXH_REQUIRE(c->halo.Mh != nullptr || c->nh == 0, EXH_ERR_ARG, "no halo catalogue");
+    EXH_REQUIRE(c->last_prec == precision, EXH_ERR_ARG,
+                "the last catalogue was computed at precision %d", c->last_prec);
+    Mh = c->halo.Mh;
+    nh = c->nh;
+  }
+  if (precision == 4)
+    run_abundance<float>(c, (const float*)Mh, nh, Mmin, Mmax, Nm, Lc, ndx, ndy, ndz, Mmean_out,
+                         dn_out, dn_err_out);
+  else
+    run_abundance<double>(c, (const double*)Mh, nh, Mmin, Mmax, Nm, Lc, ndx, ndy, ndz, Mmean_out,
+                          dn_out, dn_err_out);
+  EXH_API_END
+}

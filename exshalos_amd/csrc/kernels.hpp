@@ -47,6 +47,11 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
                int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
                int kbin_precision, double* k_out, double* P_out, long long* count_out,
                bool deferred = false);
+// differential halo abundance of masses Mh (device, nh); outputs on the host
+template <typename T>
+void run_abundance(exh_ctx* c, const T* Mh, long long nh, double Mmin, double Mmax, int Nm,
+                   double Lc, int ndx, int ndy, int ndz, double* Mmean_out, double* dn_out,
+                   double* dn_err_out);
 // host outputs of a deferred run_power (waits for the context's stream)
 void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_out);
 

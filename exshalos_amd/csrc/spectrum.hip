@@ -1226,6 +1226,136 @@ void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_o
   }
 }
 
+// ---------------------------------------------------------------------------
+// halo abundance: abundance_compute (spectrum.c:535-600) and
+// Measure_Abundance (abundance.c:4-36)
+// ---------------------------------------------------------------------------
+// Per-block smallest and largest mass (at most 1024 blocks, reduced on the
+// host).
+template <typename T>
+__global__ void k_mass_range(const T* Mh, long long n, T* bmin, T* bmax) {
+  __shared__ T smin[256], smax[256];
+  T lo = (T)1e30, hi = (T)-1e30;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const T m = Mh[i];
+    lo = m < lo ? m : lo;
+    hi = m > hi ? m : hi;
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smin[threadIdx.x] = smin[threadIdx.x + w] < smin[threadIdx.x] ? smin[threadIdx.x + w]
+                                                                    : smin[threadIdx.x];
+      smax[threadIdx.x] = smax[threadIdx.x + w] > smax[threadIdx.x] ? smax[threadIdx.x + w]
+                                                                    : smax[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bmin[blockIdx.x] = smin[0];
+    bmax[blockIdx.x] = smax[0];
+  }
+}
+
+// Counts and mass sums per bin floor((ln M - ln Mmin) / dlnM), kept for
+// 0 < bin < Nm as the reference does; LDS histograms, one atomic per block
+// and bin.
+template <typename T>
+__global__ void k_abundance(const T* Mh, long long n, double lnMmin, float dlnM, int Nm,
+                            unsigned long long* cnt, double* msum) {
+  extern __shared__ double ab_lds[];
+  double* ssum = ab_lds;
+  unsigned* scnt = (unsigned*)(ab_lds + Nm);
+  for (int b = threadIdx.x; b < Nm; b += blockDim.x) {
+    ssum[b] = 0.0;
+    scnt[b] = 0;
+  }
+  __syncthreads();
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double f = floor((log((double)Mh[i]) - lnMmin) / (double)dlnM);
+    if (f > 0.0 && f < (double)Nm) {
+      const int b = (int)f;
+      atomicAdd(&scnt[b], 1u);
+      atomicAdd(&ssum[b], (double)Mh[i]);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < Nm; b += blockDim.x)
+    if (scnt[b]) {
+      atomicAdd(&cnt[b], (unsigned long long)scnt[b]);
+      atomicAdd(&msum[b], ssum[b]);
+    }
+}
+
+template <typename T>
+void run_abundance(exh_ctx* c, const T* Mh, long long nh, double Mmin_in, double Mmax_in, int Nm,
+                   double Lc_in, int ndx, int ndy, int ndz, double* Mmean_out, double* dn_out,
+                   double* dn_err_out) {
+  EXH_REQUIRE(Nm >= 1 && Nm <= 4096, EXH_ERR_ARG, "Nm must lie in [1, 4096]");
+  EXH_REQUIRE(nh >= 0 && (nh == 0 || Mh != nullptr), EXH_ERR_ARG, "null masses");
+  T Mmin = (T)Mmin_in, Mmax = (T)Mmax_in;
+  const T Lc = (T)Lc_in;
+  const T Lx = Lc * (T)ndx, Ly = Lc * (T)ndy, Lz = Lc * (T)ndz;
+  const unsigned nb = std::max(1u, std::min(grid_for(nh, 256), 1024u));
+  char* buf = (char*)ctx_buf(c, BUF_MISC, 256 + 2 * 1024 * sizeof(T) + (size_t)Nm * 16);
+  if (Mmin < (T)0 || Mmax < (T)0) {
+    // abundance_compute's defaults: 0.9999 x the smallest, 1.0001 x the largest
+    T* bmin = (T*)(buf + 256);
+    T* bmax = bmin + 1024;
+    hipLaunchKernelGGL(k_mass_range<T>, dim3(nb), dim3(256), 0, c->stream, Mh, nh, bmin, bmax);
+    EXH_LAUNCH_CHECK();
+    std::vector<T> hmin(nb), hmax(nb);
+    EXH_CHECK_HIP(hipMemcpyAsync(hmin.data(), bmin, nb * sizeof(T), hipMemcpyDeviceToHost,
+                                 c->stream));
+    EXH_CHECK_HIP(hipMemcpyAsync(hmax.data(), bmax, nb * sizeof(T), hipMemcpyDeviceToHost,
+                                 c->stream));
+    EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (Mmin < (T)0) {
+      T lo = (T)1e+20;
+      for (long long b = 0; b < (long long)nb && nh > 0; b++) lo = hmin[b] < lo ? hmin[b] : lo;
+      Mmin = (T)((double)lo * 0.9999);
+    }
+    if (Mmax < (T)0) {
+      for (long long b = 0; b < (long long)nb && nh > 0; b++) Mmax = hmax[b] > Mmax ? hmax[b] : Mmax;
+      Mmax = (T)((double)Mmax * 1.0001);
+    }
+  }
+  const float dlnM = (float)((std::log((double)Mmax) - std::log((double)Mmin)) / Nm);
+  unsigned long long* cnt = (unsigned long long*)(buf + 256 + 2 * 1024 * sizeof(T));
+  double* msum = (double*)(cnt + Nm);
+  hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(grid_for(Nm, 256)), dim3(256), 0, c->stream,
+                     cnt, (size_t)Nm);
+  hipLaunchKernelGGL(k_zero<double>, dim3(grid_for(Nm, 256)), dim3(256), 0, c->stream, msum,
+                     (size_t)Nm);
+  if (nh > 0)
+    hipLaunchKernelGGL(k_abundance<T>, dim3(nb), dim3(256), (size_t)Nm * 12, c->stream, Mh, nh,
+                       std::log((double)Mmin), dlnM, Nm, cnt, msum);
+  EXH_LAUNCH_CHECK();
+  std::vector<unsigned long long> hc(Nm);
+  std::vector<double> hs(Nm);
+  EXH_CHECK_HIP(hipMemcpyAsync(hc.data(), cnt, Nm * 8, hipMemcpyDeviceToHost, c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(hs.data(), msum, Nm * 8, hipMemcpyDeviceToHost, c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  // the reference's arithmetic: T sums and quotients, float dlnM
+  const T V = Lx * Ly * Lz;
+  for (int b = 0; b < Nm; b++) {
+    const int N = (int)hc[b];
+    const T mean = N > 0 ? (T)((T)hs[b] / (T)N) : (T)0;
+    if (Mmean_out) Mmean_out[b] = (double)mean;
+    if (dn_out) dn_out[b] = (double)(T)((T)N / V / (T)dlnM);
+    if (dn_err_out) dn_err_out[b] = (double)(T)(std::sqrt((double)N) / (double)V / (double)dlnM);
+  }
+}
+
+template void run_abundance<float>(exh_ctx*, const float*, long long, double, double, int,
+                                   double, int, int, int, double*, double*, double*);
+template void run_abundance<double>(exh_ctx*, const double*, long long, double, double, int,
+                                    double, int, int, int, double*, double*, double*);
+
 template void run_grid<float>(exh_ctx*, const float*, const float*, const float*, const int*,
                               long long, int, int, double, double, double, int, int, double,
                               double, int, int, float*);

@@ -94,6 +94,9 @@ _SIGS = {
     "exh_power_compute": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
                                     C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                     C.c_int, C.c_int, _P, _P, _P]),
+    "exh_abundance_compute": (C.c_int, [_P, _P, C.c_longlong, C.c_double, C.c_double, C.c_int,
+                                        C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P,
+                                        _P]),
 }
 
 _lib = None

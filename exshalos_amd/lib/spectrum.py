@@ -1,6 +1,7 @@
 """Mirror of the reference's ``pyexshalos.lib.spectrum`` extension module
 (hot-path methods of src/spectrum/spectrum.c:28-38): check_precision,
-grid_compute, power_compute, with the same positional arguments.
+grid_compute, power_compute, abundance_compute, with the same positional
+arguments.
 
 Additions (keyword-only): ``device_out`` (return torch device tensors) and
 ``kbin_precision`` for power_compute (4: the reference's float bin index,
@@ -79,3 +80,26 @@ def power_compute(grid, ntype, nd, L, window, R, interlacing, Nk, k_min, k_max, 
             "power_compute")
     Rt = np_real(prec)
     return {"k": kout.astype(Rt), "Pk": Pout.astype(Rt), "Nk": cnt}
+
+
+def abundance_compute(Mh_array, Mmin, Mmax, Nm, Lc, ndx, ndy, ndz, verbose):
+    """Differential halo abundance (spectrum.c:535-600): dict Mh (mean mass
+    per bin), dn, dn_err.  Mh_array: numpy or torch device masses; None takes
+    the masses of the last halo catalogue, which stay on the device."""
+    prec = get_precision()
+    ctx = N.context()
+    Nm = int(Nm)
+    if Mh_array is None:
+        m, nh = None, 0
+    else:
+        m = to_device(Mh_array, np_real(prec)).reshape(-1)
+        nh = int(m.numel())
+    Mmean = np.zeros(Nm, dtype=np.float64)
+    dn = np.zeros(Nm, dtype=np.float64)
+    dn_err = np.zeros(Nm, dtype=np.float64)
+    N.check(N.load().exh_abundance_compute(ctx, N.ptr(m), nh, float(Mmin), float(Mmax), Nm,
+                                           float(Lc), int(ndx), int(ndy), int(ndz), prec,
+                                           N.ptr(Mmean), N.ptr(dn), N.ptr(dn_err)),
+            "abundance_compute")
+    Rt = np_real(prec)
+    return {"Mh": Mmean.astype(Rt), "dn": dn.astype(Rt), "dn_err": dn_err.astype(Rt)}

@@ -1,5 +1,6 @@
-"""Density grids and power spectra: the hot-path part of
-pyexshalos.simulation (simulation.py:9-311), same signatures and defaults.
+"""Density grids, power spectra and halo abundances: the hot-path part of
+pyexshalos.simulation (simulation.py:9-311, 719-806), same signatures and
+defaults.
 
 Extra keyword-only arguments: ``device_out`` (torch device tensors) and,
 for Compute_Power_Spectrum, ``kbin_precision`` (4: float bin index as the
@@ -147,3 +148,31 @@ def Compute_Power_Spectrum(
     elif l_max == 0:
         x["Pk"] = x["Pk"].reshape([int(ntypes * (ntypes + 1) / 2), Nk])
     return x
+
+
+def Compute_Abundance(
+    Mh,
+    Mmin: Optional[float] = None,
+    Mmax: Optional[float] = None,
+    Nm: int = 25,
+    Lc: float = 2.0,
+    nd: int = 256,
+    ndx: Optional[int] = None,
+    ndy: Optional[int] = None,
+    ndz: Optional[int] = None,
+    verbose: bool = False,
+) -> Dict[str, np.ndarray]:
+    """Differential abundance of halo masses (reference simulation.py:719-806):
+    dict Mh (mean mass per bin), dn, dn_err.  Mh may be a numpy array, a
+    torch device tensor, or None for the last halo catalogue's masses (kept
+    on the device)."""
+    from .lib.spectrum import abundance_compute, check_precision
+
+    Mmin = -1.0 if Mmin is None else Mmin
+    Mmax = -1.0 if Mmax is None else Mmax
+    Rt = np.float32 if check_precision() == 4 else np.float64
+    ndx = nd if ndx is None else ndx
+    ndy = nd if ndy is None else ndy
+    ndz = nd if ndz is None else ndz
+    return abundance_compute(Mh, Rt(Mmin), Rt(Mmax), np.int32(Nm), Rt(Lc), np.int32(ndx),
+                             np.int32(ndy), np.int32(ndz), np.int32(verbose))

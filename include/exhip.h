@@ -232,6 +232,18 @@ int exh_power_compute(exh_ctx* ctx, const void* grid, int ntype, int nd, double 
                       double k_max, int l_max, int direction, int precision,
                       int kbin_precision, double* k_out, double* P_out, long long* count_out);
 
+/* Differential halo abundance: replaces abundance_compute
+ * (spectrum.c:535-600) / Measure_Abundance (abundance.c:4-36).  Mh: device
+ * [nh] T, or NULL for the masses of the context's last halo catalogue.
+ * Mmin or Mmax < 0 default to 0.9999 x the smallest and 1.0001 x the
+ * largest mass.  Nm logarithmic bins of float width; a mass counts in bin
+ * floor((ln M - ln Mmin) / dlnM) only for 0 < bin < Nm, as in the
+ * reference.  Outputs (host) [Nm]: mean mass, dn/dlnM and its Poisson
+ * error per volume (Lc ndx)(Lc ndy)(Lc ndz), in T arithmetic. */
+int exh_abundance_compute(exh_ctx* ctx, const void* Mh, long long nh, double Mmin, double Mmax,
+                          int Nm, double Lc, int ndx, int ndy, int ndz, int precision,
+                          double* Mmean_out, double* dn_out, double* dn_err_out);
+
 #ifdef __cplusplus
 }
 #endif

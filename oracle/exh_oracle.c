@@ -1135,3 +1135,43 @@ void FN(ora_power_bins)(const REAL *const *gk, int ntype, int nd, REAL L, int Nk
   free(P);
   free(Km);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Halo abundance: abundance_compute (spectrum.c:535-600) and                 */
+/* Measure_Abundance (abundance.c:4-36).                                      */
+/* ------------------------------------------------------------------------ */
+/* Mmin/Mmax < 0: 0.9999 x the smallest and 1.0001 x the largest mass.  The
+ * bin width is a float in both builds (abundance.c:7); a mass lands in bin
+ * floor((ln M - ln Mmin) / dlnM), counted only for 0 < bin < Nm (bin 0
+ * stays empty, abundance.c:23).  Mmean sums in REAL in catalogue order. */
+void FN(ora_abundance)(const REAL *Mh, long long nh, REAL Mmin, REAL Mmax, int Nm, REAL Lc,
+                       int ndx, int ndy, int ndz, REAL *Mmean, REAL *dn, REAL *dn_err) {
+  REAL Lx = Lc * ndx, Ly = Lc * ndy, Lz = Lc * ndz;
+  if (Mmin < 0.0) {
+    Mmin = 1e+20;
+    for (long long i = 0; i < nh; i++)
+      if (Mh[i] < Mmin) Mmin = Mh[i];
+    Mmin = Mmin * 0.9999;
+  }
+  if (Mmax < 0.0) {
+    for (long long i = 0; i < nh; i++)
+      if (Mh[i] > Mmax) Mmax = Mh[i];
+    Mmax = Mmax * 1.0001;
+  }
+  float dlnM = (log(Mmax) - log(Mmin)) / Nm;
+  int *N = (int *)calloc(Nm, sizeof(int));
+  for (int i = 0; i < Nm; i++) Mmean[i] = 0.0;
+  for (long long i = 0; i < nh; i++) {
+    int ind = floor((log(Mh[i]) - log(Mmin)) / dlnM);
+    if (ind > 0 && ind < Nm) {
+      N[ind] += 1;
+      Mmean[ind] += Mh[i];
+    }
+  }
+  for (int i = 0; i < Nm; i++) {
+    if (N[i] > 0) Mmean[i] = Mmean[i] / N[i];
+    dn[i] = N[i] / (Lx * Ly * Lz) / dlnM;
+    dn_err[i] = sqrt(N[i]) / (Lx * Ly * Lz) / dlnM;
+  }
+  free(N);
+}

@@ -21,6 +21,8 @@ Entry points mirror the reference's C functions:
 * ``grid_compute``       -- Tracer_Grid (gridmodule.c:146-375)
 * ``power_compute``      -- Compute_gridk + Power_Spectrum (spectrum_h.c:83-167,
                             powermodule.c:4-108)
+* ``abundance``          -- abundance_compute / Measure_Abundance (spectrum.c:535-600,
+                            abundance.c:4-36)
 """
 
 from __future__ import annotations
@@ -392,3 +394,17 @@ def power_compute(grid, ntype, nd, L, window=2, interlacing=False, Nk=25, k_min=
         arr, C.c_int(ntype), C.c_int(nd), RC(L), C.c_int(Nk), RC(dt(k_min)), RC(dt(k_max)),
         C.c_int(l_max), C.c_int(direction), _ptr(Pk), _ptr(Km), _ptr(cnt))
     return dict(k=Km.astype(dt), Pk=Pk.astype(dt), Nk=cnt, Pk64=Pk, k64=Km)
+
+
+def abundance(Mh, Mmin=-1.0, Mmax=-1.0, Nm=25, Lc=2.0, ndx=256, ndy=256, ndz=256, precision=8):
+    """Differential halo abundance: dict Mh (mean mass per bin), dn, dn_err."""
+    dt = _real(precision)
+    RC = _rc(precision)
+    m = np.ascontiguousarray(Mh, dtype=dt)
+    Mmean = np.zeros(Nm, dtype=dt)
+    dn = np.zeros(Nm, dtype=dt)
+    dn_err = np.zeros(Nm, dtype=dt)
+    getattr(lib(), "ora_abundance" + _sfx(precision))(
+        _ptr(m), C.c_longlong(len(m)), RC(Mmin), RC(Mmax), C.c_int(Nm), RC(Lc), C.c_int(ndx),
+        C.c_int(ndy), C.c_int(ndz), _ptr(Mmean), _ptr(dn), _ptr(dn_err))
+    return {"Mh": Mmean, "dn": dn, "dn_err": dn_err}

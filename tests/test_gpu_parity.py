@@ -369,3 +369,84 @@ def test_sort_with_peaks_below_barrier(bbks, prec):
     np.testing.assert_array_equal(cat["count"], ref["count"])
     np.testing.assert_array_equal(out["Mh"], ref["Mh"])
     np.testing.assert_array_equal(out["flag"].reshape(-1), ref["flag"].reshape(-1))
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("given_range", [False, True])
+def test_abundance_compute(prec, given_range):
+    """abundance_compute (spectrum.c:535-600): identical bin counts (dn and
+    dn_err are functions of the count in the reference's arithmetic), mean
+    masses to the summation order's rounding."""
+    set_precision(prec)
+    RT = np.float32 if prec == 4 else np.float64
+    rng = np.random.default_rng(31)
+    M = np.exp(rng.uniform(np.log(2e12), np.log(3e15), 20000)).astype(RT)
+    Mmin, Mmax = (RT(5e12), RT(1e15)) if given_range else (-1.0, -1.0)
+    ref = O.abundance(M, Mmin, Mmax, 20, 4.0, 64, 64, 64, precision=prec)
+    out = SP.abundance_compute(M, Mmin, Mmax, 20, 4.0, 64, 64, 64, 0)
+    np.testing.assert_array_equal(out["dn"], ref["dn"])
+    np.testing.assert_array_equal(out["dn_err"], ref["dn_err"])
+    np.testing.assert_allclose(out["Mh"], ref["Mh"], rtol=1e-4 if prec == 4 else 1e-12)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_abundance_of_the_last_catalogue_and_barrier_chi2(bbks, prec):
+    """Masses left on the device by the halo finder give the abundance of the
+    catalogue; the barrier fit's chi^2 at a fixed (a, beta, alpha) equals the
+    one built from the oracle's halo finder and abundance (utils.py:374-404),
+    and a short Nelder-Mead run returns its best evaluated point, no worse
+    than the start."""
+    from exshalos_amd import simulation as SIM
+    from exshalos_amd import utils as U
+    from scipy.interpolate import interp1d
+
+    k, P = bbks
+    set_precision(prec)
+    nd, Nm, Nmin = 64, 8, 1
+    delta, _, _ = O.field_from_modes(k, P, nd, LC, _noise(nd, 3, prec), precision=prec)
+    X.find_halos(delta, k, P, LC, 0.31, 0.0, -1.0, Nmin, 0.8, 0.2, 0.5, 0, 0, 0)
+    ref_h = O.find_halos(delta, k, P, LC, dc=-1.0, Nmin=Nmin, a=0.8, beta=0.2, alpha=0.5,
+                         seed=12345, precision=prec)
+    out = SIM.Compute_Abundance(None, Nm=Nm, Lc=LC, nd=nd)
+    ref = O.abundance(ref_h["Mh"], -1.0, -1.0, Nm, LC, nd, nd, nd, precision=prec)
+    np.testing.assert_array_equal(out["dn"], ref["dn"])
+    np.testing.assert_allclose(out["Mh"], ref["Mh"], rtol=1e-5 if prec == 4 else 1e-12)
+    # target: the oracle's abundance at (0.8, 0.2, 0.5) with 10% errors
+    Mt, dnt = ref["Mh"][ref["dn"] > 0], ref["dn"][ref["dn"] > 0]
+    err = 0.1 * dnt
+    dc = U._deltac(0.0, 0.31)
+    theta = (0.6, 0.35, 0.65)
+    h = O.find_halos(delta, k, P, LC, dc=dc, Nmin=Nmin, a=theta[0], beta=theta[1],
+                     alpha=theta[2], seed=12345, precision=prec)
+    dnh = O.abundance(h["Mh"], -1.0, -1.0, Nm, LC, nd, nd, nd, precision=prec)
+    fdn = interp1d(np.log(Mt), dnt, bounds_error=False, fill_value=0.0)
+    fdn_err = interp1d(np.log(Mt), err, bounds_error=False, fill_value=0.0)
+    mask = dnh["dn"] > 0
+    chi2_ref = np.sum((dnh["dn"][mask] - fdn(np.log(dnh["Mh"][mask]))) ** 2 /
+                      (dnh["dn_err"][mask] ** 2 + fdn_err(dnh["Mh"][mask]) ** 2)) / (Nm - 4)
+    import scipy.optimize as so
+
+    real_minimize = so.minimize
+    evals = []
+
+    def recording(fun, x0, **kw):  # theta first, then the fit; every chi^2 kept
+        def f(x):
+            v = fun(x)
+            evals.append((np.array(x), v))
+            return v
+
+        f(np.array(theta))
+        return real_minimize(f, x0, **kw)
+
+    x0 = np.array([0.55, 0.4, 0.7])
+    so.minimize = recording
+    try:
+        xb = U.Fit_Barrier(k, P, Mt, dnt, err, grid=delta, Nm=Nm, nd=nd, Lc=LC, Nmin=Nmin,
+                           x0=x0, Max_iter=12)
+    finally:
+        so.minimize = real_minimize
+    np.testing.assert_allclose(evals[0][1], chi2_ref, rtol=1e-4 if prec == 4 else 1e-9)
+    f0 = [v for x, v in evals[1:] if np.allclose(x, x0)][0]
+    fb = min(v for x, v in evals[1:] if np.allclose(x, xb))
+    assert fb <= f0 and fb == min(v for _, v in evals[1:])
+    assert np.all(xb >= [0.1, 0.0, 0.0]) and np.all(xb <= [2.0, 1.0, 1.0])

@@ -255,3 +255,37 @@ def test_cross_spectrum_of_two_tracers():
     P = out["Pk64"][:, 0]
     np.testing.assert_allclose(P[1], 2 * P[0], rtol=1e-12)
     np.testing.assert_allclose(P[2], 4 * P[0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+@pytest.mark.parametrize("given_range", [False, True])
+def test_abundance(precision, given_range):
+    """abundance_compute / Measure_Abundance (spectrum.c:535-600,
+    abundance.c:4-36): counts per logarithmic bin against np.histogram on the
+    edges ln Mmin + i dlnM (dlnM a float), bin 0 left empty as the reference
+    counts only 0 < bin < Nm; each bin's mean mass lies inside the bin;
+    dn V dlnM is the integer count and dn_err the Poisson error."""
+    R = np.float32 if precision == 4 else np.float64
+    rng = np.random.default_rng(7)
+    M = np.exp(rng.uniform(np.log(2e12), np.log(3e15), 5000)).astype(R)
+    Nm, Lc, nd = 12, 4.0, 64
+    if given_range:
+        Mmin, Mmax = R(5e12), R(1e15)
+    else:
+        Mmin, Mmax = R(M.min() * R(0.9999)), R(M.max() * R(1.0001))
+    out = O.abundance(M, Mmin if given_range else -1.0, Mmax if given_range else -1.0, Nm, Lc,
+                      nd, nd, nd, precision=precision)
+    dlnM = np.float32((np.log(np.float64(Mmax)) - np.log(np.float64(Mmin))) / Nm)
+    edges = np.log(np.float64(Mmin)) + np.arange(Nm + 1) * np.float64(dlnM)
+    counts, _ = np.histogram(np.log(M.astype(np.float64)), bins=edges)
+    counts[0] = 0
+    V = (Lc * nd) ** 3
+    N = out["dn"].astype(np.float64) * V * np.float64(dlnM)
+    np.testing.assert_allclose(N, counts, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(out["dn_err"].astype(np.float64) * V * np.float64(dlnM),
+                               np.sqrt(counts), rtol=1e-5)
+    full = counts > 0
+    lnm = np.log(out["Mh"][full].astype(np.float64))
+    lo = edges[:-1][full]
+    assert np.all(lnm >= lo - 1e-6) and np.all(lnm < lo + dlnM + 1e-6)
+    assert out["Mh"][0] == 0 and out["dn"][0] == 0
