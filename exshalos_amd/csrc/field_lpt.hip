@@ -520,6 +520,10 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
   std::vector<unsigned char> key;
   const int kp[2] = {(int)sizeof(T), nk};
   key_append(key, kp, 2);
+  // Inter_Power truncates xi(r) when R_max (a T) is below 1e5
+  const double rmax = (double)(T)s->R_max;
+  const bool trunc = s->R_max > 0 && rmax < 100000.0;
+  key_append(key, &rmax, trunc ? 1 : 0);
   key_append(key, s->k, nk);
   key_append(key, s->P, nk);
   if (key != c->spline_key) {
@@ -530,6 +534,14 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
     for (int i = 0; i < nk; i++) {
       x[i] = (double)(T)s->k[i];
       y[i] = (double)(T)s->P[i];
+    }
+    if (trunc) {
+      // on the double copies of the T knots, splined without rounding
+      std::vector<double> xin(x, x + nk), yin(y, y + nk);
+      for (int i = 1; i < nk; i++)
+        EXH_REQUIRE(xin[i] > xin[i - 1] && xin[0] > 0, EXH_ERR_ARG,
+                    "the FFTLog truncation needs increasing positive k");
+      pk_truncate(xin.data(), yin.data(), nk, rmax, x, y);
     }
     spline_second_derivs(x, y, nk, m.data());
     for (int lo = 0; lo + 1 < nk; lo++) {

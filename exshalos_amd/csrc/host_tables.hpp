@@ -25,6 +25,11 @@ R barrier_value(R S, int do_eb, R dc, R a, R beta, R alpha);
 std::vector<int> cells_in_spheres(int smax);
 void shell_offsets(int smax, int tmax, std::vector<int>& start, std::vector<unsigned>& packed);
 
+// Inter_Power's truncation: xi(r) of the log-spaced P(k) by FFTLog, zero
+// beyond R_max, back to P(k) on the transform's grid (k_out, P_out [n]).
+void pk_truncate(const double* k, const double* P, int n, double R_max, double* k_out,
+                 double* P_out);
+
 void taus_jump_tables(uint32_t* mats);
 void taus_seed_state(unsigned long s, uint32_t st[3]);
 

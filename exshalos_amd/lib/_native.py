@@ -44,7 +44,7 @@ class Box(C.Structure):
 class FieldSrc(C.Structure):
     _fields_ = [
         ("kind", C.c_int), ("data", C.c_void_p), ("k", C.c_void_p), ("P", C.c_void_p),
-        ("Nk", C.c_int), ("fixed", C.c_int), ("phase", C.c_double),
+        ("Nk", C.c_int), ("fixed", C.c_int), ("phase", C.c_double), ("R_max", C.c_double),
     ]
 
 

@@ -13,8 +13,8 @@ Additions (keyword-only, additive):
   device_out=  return torch device tensors instead of numpy arrays.
 Divergences (documented): every requested output is returned for any flag
 combination (the reference returns {} for combinations it does not list,
-exshalos.c:865-937); OUT_PROF and R_max < 1e5 (FFTLog truncation) raise
-NotImplementedError; the flag grid of find_halos has shape [nd, nd, nd]
+exshalos.c:865-937); OUT_PROF raises NotImplementedError; the flag grid of
+find_halos has shape [nd, nd, nd]
 (exshalos.c:558 passes a pointer as the dimension).
 """
 
@@ -45,7 +45,9 @@ def _noise_for(nd3, seed, prec, noise):
     return t
 
 
-def _modes_src(kh, Ph, noise_t, fixed, phase):
+def _modes_src(kh, Ph, noise_t, fixed, phase, R_max=100000.0):
+    """White-noise modes with P(k); R_max < 1e5 cuts xi(r) beyond R_max
+    (Inter_Power, density_grid.c:52-59)."""
     s = N.FieldSrc()
     s.kind = N.SRC_MODES
     s.data = noise_t.data_ptr()
@@ -54,6 +56,7 @@ def _modes_src(kh, Ph, noise_t, fixed, phase):
     s.Nk = len(kh)
     s.fixed = int(bool(fixed))
     s.phase = float(phase)
+    s.R_max = float(R_max)
     return s
 
 
@@ -73,25 +76,18 @@ def _out(t, device_out):
     return t if device_out else to_numpy(t)
 
 
-def _check_rmax(R_max):
-    if float(R_max) < 100000.0:
-        raise NotImplementedError(
-            "R_max < 1e5 (FFTLog truncation of P(k), density_grid.c:53-59) is not implemented")
-
-
 def density_grid_compute(k, P, R_max, Ndx, Ndy, Ndz, Lc, outk, seed, fixed, phase, k_smooth,
                          verbose, nthreads, *, noise=None, device_out=False):
     """Gaussian density grid (exshalos.c:46-121, density_grid.c:71-217)."""
     import torch
 
-    _check_rmax(R_max)
     prec = get_precision()
     ctx = N.context()
     nd3 = (int(Ndx), int(Ndy), int(Ndz))
     b = N.box(nd3, Lc, 0.31, 0.0, 1.686, prec)
     kh, Ph = host_f64(k), host_f64(P)
     noise_t = _noise_for(nd3, seed, prec, noise)
-    src = _modes_src(kh, Ph, noise_t, fixed, phase)
+    src = _modes_src(kh, Ph, noise_t, fixed, phase, np_real(prec)(R_max))
     dt = torch.float32 if prec == 4 else torch.float64
     grid = torch.empty(nd3, dtype=dt, device=noise_t.device)
     gridk = torch.empty((nd3[0], nd3[1], nd3[2] // 2 + 1, 2), dtype=dt, device=noise_t.device) \
@@ -203,14 +199,13 @@ def halos_box_from_pk(k, P, R_max, Ndx, Ndy, Ndz, Lc, seed, k_smooth, Om0, redsh
 
     if OUT_PROF:
         raise NotImplementedError("OUT_PROF (Lagrangian profiles) is not implemented")
-    _check_rmax(R_max)
     prec = get_precision()
     ctx = N.context()
     nd3 = (int(Ndx), int(Ndy), int(Ndz))
     b = N.box(nd3, Lc, Om0, redshift, dc, prec)
     kh, Ph = host_f64(k), host_f64(P)
     noise_t = _noise_for(nd3, seed, prec, noise)
-    src = _modes_src(kh, Ph, noise_t, fixed, phase)
+    src = _modes_src(kh, Ph, noise_t, fixed, phase, np_real(prec)(R_max))
     o = _halo_opts(kh, Ph, Nmin, a, beta, alpha, seed, prec)
     dt = torch.float32 if prec == 4 else torch.float64
     ng = nd3[0] * nd3[1] * nd3[2]

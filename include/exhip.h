@@ -80,6 +80,9 @@ typedef struct {
   int Nk;
   int fixed;         /* fixed amplitudes (phase from the noise pair) */
   double phase;      /* constant phase rotation */
+  double R_max;      /* 0 < R_max < 1e5: xi(r) of P(k) cut beyond R_max by FFTLog before
+                        the spline (Inter_Power, density_grid.c:52-59; the knots must be
+                        log-spaced); otherwise P(k) as given */
 } exh_field_src;
 
 /* Halo-finder options (find_halos / halos_box_from_* arguments). */

@@ -13,6 +13,8 @@ source); ``precision=8`` is the FP64 oracle.
 
 Entry points mirror the reference's C functions:
 
+* ``pk_truncate``        -- Inter_Power's FFTLog truncation (density_grid.c:52-59,
+                            fftlog.c:49-157)
 * ``field_from_modes``   -- Compute_Den (density_grid.c:71-217)
 * ``halo_tables``        -- Compute_Sig / Compute_nh / Barrier (find_halos.c:4-93, 200-212)
 * ``find_halos``         -- Find_Halos (find_halos.c:479-563)
@@ -124,13 +126,65 @@ def spline_eval(x, y, xq):
 
 
 # --------------------------------------------------------------------------
+# P(k) truncation in configuration space (FFTLog)
+# --------------------------------------------------------------------------
+def _fht(x, a, mu):
+    """Hamilton's (2000) FFTLog fast Hankel transform of order mu, bias 0, on
+    the log-spaced grid x (fftlog.c:93-131): low-ringing k r (fftlog.c:49-61),
+    coefficients u_m = (k0 r0)^(-2 pi i m / L) U_mu(2 pi i m / L) with
+    U_mu(z) = 2^z Gamma((mu+1+z)/2) / Gamma((mu+1-z)/2), the convolution by
+    FFTs, the reversal, and the output grid (k0 r0 / x_0) e^(n L / N).
+    Returns (y, b)."""
+    from scipy.special import loggamma
+
+    n = len(x)
+    L = np.log(x[-1] / x[0]) * n / (n - 1.0)
+    h = (mu + 1.0) / 2.0
+    # low-ringing kr: ln(2/kr) N/L + 2 arg Gamma(h + i pi N/(2L)) / pi an integer
+    kr = 1.0
+    ga = np.imag(loggamma(h + 1j * np.pi * n / (2.0 * L)))
+    arg = np.log(2.0 / kr) * n / L + 2.0 * ga / np.pi
+    kr *= np.exp((arg - np.round(arg)) * L / n)
+    k0r0 = kr * np.exp(-L)
+    m = np.arange(n // 2 + 1)
+    y = np.pi / L
+    u = np.exp(1j * (-2.0 * m * y * np.log(k0r0 / 2.0) + 2.0 * np.imag(loggamma(h + 1j * m * y))))
+    full = np.concatenate([u, np.conj(u[1:n - n // 2][::-1])])
+    if n % 2 == 0:
+        full[n // 2] = full[n // 2].real
+    b = np.fft.ifft(np.fft.fft(a) * full)[::-1]
+    out = k0r0 / x[0] * np.exp(np.arange(n) * L / n)
+    return out, b
+
+
+def pk_truncate(k, P, R_max):
+    """Inter_Power's truncation (density_grid.c:52-59, fftlog.c:133-157):
+    xi(r) of P(k) by FFTLog (l = 0, k^(3/2) weights), xi = 0 for r > R_max,
+    then P(k) again on the transform's k grid.  Returns (k', P') in double."""
+    k = np.asarray(k, dtype=np.float64)
+    P = np.asarray(P, dtype=np.float64)
+    r, b = _fht(k, k**1.5 * P, 0.5)
+    xi = np.real((2.0 * np.pi * r) ** -1.5 * b)
+    xi[r > R_max] = 0.0
+    k2, b2 = _fht(r, r**1.5 * xi, 0.5)
+    P2 = np.real((2.0 * np.pi * k2) ** -1.5 * b2) * (2.0 * np.pi) ** 3
+    return k2, P2, r, xi
+
+
+# --------------------------------------------------------------------------
 # field
 # --------------------------------------------------------------------------
-def field_modes(p, k, P, noise, precision=8, k_smooth=1e5, fixed=False, phase=0.0):
+def field_modes(p, k, P, noise, precision=8, k_smooth=1e5, fixed=False, phase=0.0,
+                R_max=1e5):
+    """Compute_Den's modes; R_max < 1e5 truncates xi(r) first (Inter_Power),
+    splining the double-precision result as the reference does."""
     R = _real(precision)
     nd = tuple(p.nd)
     k64 = np.ascontiguousarray(np.asarray(k, dtype=R).astype(np.float64))
     P64 = np.ascontiguousarray(np.asarray(P, dtype=R).astype(np.float64))
+    if R(R_max) < 100000.0:
+        k64, P64, _, _ = pk_truncate(k64, P64, float(R(R_max)))
+        k64, P64 = np.ascontiguousarray(k64), np.ascontiguousarray(P64)
     noise = np.ascontiguousarray(noise, dtype=R)
     assert noise.shape == (nd[0], nd[1], nd[2] // 2 + 1, 2)
     deltak = np.zeros_like(noise)
@@ -158,11 +212,11 @@ def r2c(x: np.ndarray) -> np.ndarray:
 
 
 def field_from_modes(k, P, nd, Lc, noise, Om0=0.31, z=0.0, precision=8, k_smooth=1e5,
-                     fixed=False, phase=0.0):
+                     fixed=False, phase=0.0, R_max=1e5):
     """Compute_Den: returns (delta [nd]^3, deltak [nd][nd][nd/2+1][2], params)."""
     R = _real(precision)
     p = params(nd, Lc, Om0, z, -1.0, precision)
-    dk = field_modes(p, k, P, noise, precision, k_smooth, fixed, phase)
+    dk = field_modes(p, k, P, noise, precision, k_smooth, fixed, phase, R_max)
     delta = (c2r(dk, tuple(p.nd)).astype(R) * R(p.Normx)).astype(R)
     return delta, dk, p
 
@@ -323,11 +377,13 @@ def halo_positions(p, flag, S1, S2, halos, precision=8, with_vel=False):
 
 def halos_box_from_pk(k, P, nd, Lc, noise, Om0=0.31, z=0.0, k_smooth=1e4, dc=-1.0,
                       Nmin=1, a=1.0, beta=0.0, alpha=0.0, seed=12345, do_2lpt=False,
-                      out_lpt=False, precision=8):
-    """Generate_Halos_Box_from_Pk (box.c:4-68) with explicit noise modes."""
+                      out_lpt=False, precision=8, R_max=1e5):
+    """Generate_Halos_Box_from_Pk (box.c:4-68) with explicit noise modes.
+    R_max < 1e5 truncates the field's P(k) only; the halo tables use the
+    P(k) as given (box.c:18-21)."""
     R = _real(precision)
     p = params(nd, Lc, Om0, z, dc, precision)
-    dk = field_modes(p, k, P, noise, precision, 1e5)
+    dk = field_modes(p, k, P, noise, precision, 1e5, R_max=R_max)
     delta = (c2r(dk, tuple(p.nd)).astype(R) * R(p.Normx)).astype(R)
     hal = find_halos(delta, k, P, Lc, Om0, z, dc, Nmin, a, beta, alpha, seed, precision, p)
     S1, S2 = lpt(dk, p, k_smooth, do_2lpt, precision)

@@ -450,3 +450,29 @@ def test_abundance_of_the_last_catalogue_and_barrier_chi2(bbks, prec):
     fb = min(v for x, v in evals[1:] if np.allclose(x, xb))
     assert fb <= f0 and fb == min(v for _, v in evals[1:])
     assert np.all(xb >= [0.1, 0.0, 0.0]) and np.all(xb <= [2.0, 1.0, 1.0])
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_pk_truncation_field_and_halos(bbks, prec):
+    """R_max < 1e5 (Inter_Power, density_grid.c:52-59): the field's P(k) is
+    cut in configuration space by FFTLog before the spline; the halo
+    tables keep the P(k) as given (box.c:18-21).  Field against the
+    oracle's FFTLog (pinned against scipy.fft.fht), halos bit-exact."""
+    k, P = bbks
+    set_precision(prec)
+    nd, R_max = 64, 60.0
+    noise = _noise(nd, 4, prec)
+    g = X.density_grid_compute(k, P, R_max, nd, nd, nd, LC, 1, 3, 0, 0.0, 1e5, 0, 1,
+                               noise=noise)
+    delta, dk, _ = O.field_from_modes(k, P, nd, LC, noise, precision=prec, R_max=R_max)
+    full, _, _ = O.field_from_modes(k, P, nd, LC, noise, precision=prec)
+    tol = 2e-6 if prec == 4 else 1e-11
+    assert _rel(g["grid"], delta) < tol
+    assert _rel(g["gridk"], dk) < (1e-6 if prec == 4 else 1e-13)
+    assert _rel(delta, full) > 1e-3  # the cut changes the field
+    out = X.halos_box_from_pk(k, P, R_max, nd, nd, nd, LC, 1, 1e4, 0.31, 0.0, -1.0, 1, 1.0, 0.0,
+                              0.0, 0, 0.0, 1, 0, 0, 0, 1, 0, 0, 1, noise=noise)
+    ref = O.halos_box_from_pk(k, P, nd, LC, noise, seed=1, precision=prec, R_max=R_max)
+    assert _rel(out["grid"], ref["delta"]) < tol
+    ref_h = O.find_halos(out["grid"], k, P, LC, seed=1, precision=prec)
+    np.testing.assert_array_equal(out["Mh"], ref_h["Mh"])

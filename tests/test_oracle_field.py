@@ -150,3 +150,57 @@ def test_field_variance_matches_the_spectrum():
     lhs = np.sum(delta.astype(np.float64) ** 2)
     rhs = nd**3 * np.sum(w * np.abs(z) ** 2) * p.Normx**2
     assert abs(lhs / rhs - 1.0) < 1e-10
+
+
+def test_fftlog_matches_scipy_fht():
+    """The oracle's FFTLog (fftlog.c:49-131) equals scipy.fft.fht, an
+    independent implementation of Hamilton's algorithm, with the same
+    low-ringing offset (scipy.fft.fhtoffset), value for value."""
+    from scipy.fft import fht, fhtoffset
+
+    n = 500
+    k = np.logspace(-4, 2, n)
+    a = k**1.5 * np.exp(-0.5 * (3.0 * k) ** 2) * k**0.3
+    dln = np.log(k[1] / k[0])
+    r, b = O._fht(k, a, 0.5)
+    A = fht(a, dln, mu=0.5, offset=fhtoffset(dln, 0.5, initial=0.0))
+    assert np.max(np.abs(b.real - A)) < 1e-11 * np.max(np.abs(A))
+    assert np.max(np.abs(b.imag)) < 1e-12 * np.max(np.abs(A))
+
+
+def test_fftlog_gaussian_pair_and_grid_label():
+    """P(k) = exp(-k^2 s^2 / 2) has xi(r) = (2 pi s^2)^(-3/2) exp(-r^2 / 2 s^2).
+    The values of the reference's transform sit one log step above the grid
+    it labels them with (fht's k[0] = k0 r0 / r[0] with L = N dln,
+    fftlog.c:93-124); the round trip P -> xi -> P returns the input grid."""
+    n, s = 500, 2.0
+    k = np.logspace(-4, 2, n)
+    P = np.exp(-0.5 * (k * s) ** 2)
+    r, b = O._fht(k, k**1.5 * P, 0.5)
+    rt = r * (k[1] / k[0])
+    xi = np.real((2 * np.pi * rt) ** -1.5 * b)
+    exact = (2 * np.pi * s * s) ** -1.5 * np.exp(-0.5 * (rt / s) ** 2)
+    sel = (rt > 0.1) & (rt < 5 * s)
+    assert np.max(np.abs(xi - exact)[sel]) < 1e-4 * exact.max()
+    k2, P2, _, _ = O.pk_truncate(k, P, 1e30)
+    np.testing.assert_allclose(k2, k, rtol=1e-13)
+    sel = (k > 1e-3) & (k < 1.0)
+    np.testing.assert_allclose(P2[sel], P[sel], rtol=1e-9, atol=1e-12)
+
+
+def test_pk_truncation_is_a_projection():
+    """Truncating xi at R_max twice equals once (both transforms share the
+    grid), and the truncated spectrum's xi vanishes beyond R_max."""
+    from exshalos_amd import synth
+
+    k, P = synth.bbks_spectrum()
+    k, P = np.asarray(k, np.float64), np.asarray(P, np.float64)
+    k1, P1, r, xi1 = O.pk_truncate(k, P, 60.0)
+    k2, P2, r2, xi2 = O.pk_truncate(k1, P1, 60.0)
+    np.testing.assert_allclose(k2, k1, rtol=1e-13)
+    np.testing.assert_allclose(P2, P1, rtol=0, atol=1e-9 * np.max(np.abs(P1)))
+    r0, b = O._fht(k1, k1**1.5 * P1, 0.5)
+    xi = np.real((2 * np.pi * r0) ** -1.5 * b)
+    assert np.max(np.abs(xi[r0 > 60.0])) < 1e-12 * np.max(np.abs(xi))
+    # truncation removes power: P1 differs from P where xi had support
+    assert np.max(np.abs(P1 - P)) > 1e-3 * np.max(P)
