@@ -609,3 +609,29 @@ int exh_abundance_compute(exh_ctx* c, const void* Mh, long long nh, double Mmin,
                           dn_out, dn_err_out);
   EXH_API_END
 }
+
+int exh_bispectrum_size(int Nk) {
+  if (Nk < 1 || Nk > 4096) return EXH_ERR_ARG;
+  std::vector<std::array<int, 3>> tri;
+  bispectrum_configs(Nk, tri);
+  return (int)tri.size();
+}
+
+int exh_bispectrum_compute(exh_ctx* c, const void* grid, int ntype, int nd, double L, int window,
+                           double R, int interlacing, int Nk, double k_min, double k_max,
+                           int precision, long long cap, double* kP, double* Pk, double* Nk_out,
+                           double* kB, double* Bk, double* Ntri_out, int* ntri) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  EXH_REQUIRE(precision == 4 || precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
+  EXH_REQUIRE(grid && kP && Pk && Nk_out && kB && Bk && Ntri_out && ntri, EXH_ERR_ARG,
+              "null array");
+  EXH_REQUIRE(window != 3 || R > 0, EXH_ERR_ARG, "the sphere window needs R > 0");
+  *ntri = precision == 4
+              ? run_bispectrum<float>(c, (const float*)grid, ntype, nd, L, window, R, interlacing,
+                                      Nk, k_min, k_max, cap, kP, Pk, Nk_out, kB, Bk, Ntri_out)
+              : run_bispectrum<double>(c, (const double*)grid, ntype, nd, L, window, R,
+                                       interlacing, Nk, k_min, k_max, cap, kP, Pk, Nk_out, kB, Bk,
+                                       Ntri_out);
+  EXH_API_END
+}

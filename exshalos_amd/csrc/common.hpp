@@ -79,6 +79,8 @@ enum Slot {
   BUF_HALOTAB,  // halo finder tables (barrier, mass function, spheres, taus)
   BUF_PTAB,     // power spectrum phase and window tables
   BUF_GCNT,     // tracer counts of the grid deposit
+  BUF_SHELLS,   // bispectrum: real fields of every k shell and tracer
+  BUF_SHELLK,   // bispectrum: combined modes, shell index per mode, shell work grids
   BUF_EFF,      // halo growth: delta with flagged cells set to -Mtot
   BUF_NSLOTS
 };

@@ -1,6 +1,7 @@
 // Launchers shared between the translation units of the library.
 #pragma once
 
+#include <array>
 #include <functional>
 #include <limits>
 
@@ -52,6 +53,14 @@ template <typename T>
 void run_abundance(exh_ctx* c, const T* Mh, long long nh, double Mmin, double Mmax, int Nm,
                    double Lc, int ndx, int ndy, int ndz, double* Mmean_out, double* dn_out,
                    double* dn_err_out);
+// bispectra and shell power spectra (bi_compute); returns the number of
+// configurations written (cap: capacity of kB, Bk, Ntri_out)
+template <typename T>
+int run_bispectrum(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window, double R,
+                   int interlacing, int Nk, double k_min, double k_max, long long cap,
+                   double* kP, double* Pk, double* Nk_out, double* kB, double* Bk,
+                   double* Ntri_out);
+void bispectrum_configs(int Nk, std::vector<std::array<int, 3>>& tri);
 // host outputs of a deferred run_power (waits for the context's stream)
 void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_out);
 

@@ -23,6 +23,7 @@
 #include "kernels.hpp"
 #include "util.hpp"
 
+#include <array>
 #include <cmath>
 #include <cstdlib>
 
@@ -1355,6 +1356,345 @@ template void run_abundance<float>(exh_ctx*, const float*, long long, double, do
                                    double, int, int, int, double*, double*, double*);
 template void run_abundance<double>(exh_ctx*, const double*, long long, double, double, int,
                                     double, int, int, int, double*, double*, double*);
+
+// ---------------------------------------------------------------------------
+// bispectrum: bi_compute (spectrum.c:291-406) / Bi_Spectrum (bimodule.c)
+// ---------------------------------------------------------------------------
+// Combined, deconvolved modes of every tracer (Compute_gridk,
+// spectrum_h.c:83-167, in the grid's precision as the reference evaluates
+// it) and the shell of every mode, Indice(|k|, k_min, dk) (bimodule.c:72-75);
+// per-shell sums of |k| and mode counts.
+struct ShellArgs {
+  int nd, nz, ntype, nsub, Nk, window;
+  double kn, Lb, Normk, R, kmin, dk;  // kn, Lb, Normk, R, kmin, dk hold values of T
+  FastDiv dz, dnd;
+};
+
+template <typename T>
+__device__ inline T sinc_dev(T x) {
+  return x != (T)0 ? (T)(sin((double)x) / (double)x) : (T)1;
+}
+
+template <typename T>
+__global__ void k_shell_modes(ShellArgs a, const typename cplx<T>::type* raw,
+                              typename cplx<T>::type* gk, short* shell, double* ksum,
+                              unsigned long long* kcnt) {
+  using C2 = typename cplx<T>::type;
+  extern __shared__ double sm_lds[];
+  double* sks = sm_lds;
+  unsigned* scs = (unsigned*)(sm_lds + a.Nk);
+  for (int q = threadIdx.x; q < a.Nk; q += blockDim.x) {
+    sks[q] = 0.0;
+    scs[q] = 0;
+  }
+  __syncthreads();
+  const size_t nh = (size_t)a.nd * a.nd * a.nz;
+  const T kn = (T)a.kn, Lb = (T)a.Lb, Normk = (T)a.Normk, kmin = (T)a.kmin, dk = (T)a.dk;
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
+       id += (size_t)gridDim.x * blockDim.x) {
+    const unsigned r = fdiv((unsigned)id, a.dz);
+    const int k = (int)((unsigned)id - r * (unsigned)a.nz);
+    const int i = (int)fdiv(r, a.dnd);
+    const int j = (int)(r - (unsigned)i * (unsigned)a.nd);
+    const T kx = (T)((2 * i < a.nd) ? i : i - a.nd) * kn;
+    const T ky = (T)((2 * j < a.nd) ? j : j - a.nd) * kn;
+    const T kz = (T)((2 * k < a.nd) ? k : k - a.nd) * kn;
+    // W (spectrum_h.c:51-70)
+    T W = (T)1;
+    if (a.window == 1 || a.window == 2 || a.window == 5) {
+      const T s3 = sinc_dev<T>((T)((double)kx * (double)Lb / 2.0)) *
+                   sinc_dev<T>((T)((double)ky * (double)Lb / 2.0)) *
+                   sinc_dev<T>((T)((double)kz * (double)Lb / 2.0));
+      W = a.window == 1 ? s3 : (T)pow((double)s3, a.window == 2 ? 2.0 : 3.0);
+    } else if (a.window == 3) {
+      const T km = (T)sqrt((double)(kx * kx + ky * ky + kz * kz));
+      const T kR = km * (T)a.R;
+      W = (T)(3.0 / km / km / (T)a.R / (T)a.R * ((double)sinc_dev<T>(kR) - cos((double)kR)));
+    } else if (a.window == 4) {
+      const T km = kx * kx + ky * ky + kz * kz;
+      W = (T)exp((double)(-km * (T)a.R * (T)a.R) / 2.0);
+    }
+    for (int t = 0; t < a.ntype; t++) {
+      const C2 f1 = raw[(size_t)(t * a.nsub) * nh + id];
+      C2 o;
+      if (a.nsub == 2) {
+        const C2 f2 = raw[(size_t)(t * a.nsub + 1) * nh + id];
+        const double th = (double)((kx + ky + kz) * Lb) / 2.0;
+        const double c = cos(th), sn = sin(th);
+        o.x = (T)(((double)f1.x + (double)f2.x * c + (double)f2.y * sn) / 2.0 * (double)Normk /
+                  (double)W);
+        o.y = (T)(((double)f1.y + (double)f2.y * c - (double)f2.x * sn) / 2.0 * (double)Normk /
+                  (double)W);
+      } else {
+        o.x = f1.x * Normk / W;
+        o.y = f1.y * Normk / W;
+      }
+      gk[(size_t)t * nh + id] = o;
+    }
+    const T km = (T)sqrt((double)(kx * kx + ky * ky + kz * kz));
+    const double f = floor((double)((km - kmin) / dk));
+    const int b = (f >= 0.0 && f < (double)a.Nk) ? (int)f : -1;
+    shell[id] = (short)b;
+    if (b >= 0) {
+      atomicAdd(&sks[b], (double)km);
+      atomicAdd(&scs[b], 1u);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < a.Nk; q += blockDim.x)
+    if (scs[q]) {
+      atomicAdd(&ksum[q], sks[q]);
+      atomicAdd(&kcnt[q], (unsigned long long)scs[q]);
+    }
+}
+
+// The modes of shell s of every tracer and the shell's indicator (1 + 0i),
+// zero elsewhere: ntype + 1 half-complex grids for one batched c2r.
+template <typename T>
+__global__ void k_shell_fill(const typename cplx<T>::type* gk, const short* shell, size_t nh,
+                             int ntype, int s, typename cplx<T>::type* out) {
+  using C2 = typename cplx<T>::type;
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
+       id += (size_t)gridDim.x * blockDim.x) {
+    const bool in = shell[id] == s;
+    for (int t = 0; t < ntype; t++) {
+      C2 v = gk[(size_t)t * nh + id];
+      if (!in) v.x = v.y = (T)0;
+      out[(size_t)t * nh + id] = v;
+    }
+    C2 one;
+    one.x = in ? (T)1 : (T)0;
+    one.y = (T)0;
+    out[(size_t)ntype * nh + id] = one;
+  }
+}
+
+// Sums over the cells of products of two or three shell fields, one double
+// per product ("term": fields fa, fb and fc, fc < 0 for a pair).  A block
+// stages BS_CH cells of every field in LDS, then each thread runs its
+// terms over those cells (all lanes read the same cell: LDS broadcast).
+static constexpr int BS_TPT = 4;  // terms per thread
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_shell_products(const T* fields, int nfield, size_t ng,
+                                                        int ch, const int4* terms, int nterm,
+                                                        double* sums) {
+  extern __shared__ __align__(16) unsigned char bs_raw[];
+  T* lf = (T*)bs_raw;  // [nfield][ch]
+  int4 tm[BS_TPT];
+  double acc[BS_TPT];
+  for (int q = 0; q < BS_TPT; q++) {
+    const int t = blockIdx.y * 256 * BS_TPT + q * 256 + threadIdx.x;
+    tm[q] = t < nterm ? terms[t] : make_int4(-1, 0, 0, 0);
+    acc[q] = 0.0;
+  }
+  const size_t nch = (ng + ch - 1) / ch;
+  for (size_t chunk = blockIdx.x; chunk < nch; chunk += gridDim.x) {
+    const size_t x0 = chunk * ch;
+    const int nx = (int)min((size_t)ch, ng - x0);
+    __syncthreads();
+    for (int f = 0; f < nfield; f++)
+      for (int c = threadIdx.x; c < ch; c += 256)
+        lf[f * ch + c] = c < nx ? fields[(size_t)f * ng + x0 + c] : (T)0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BS_TPT; q++) {
+      if (tm[q].x < 0) continue;
+      const T* A = lf + tm[q].x * ch;
+      const T* B = lf + tm[q].y * ch;
+      double s = 0.0;
+      if (tm[q].z >= 0) {
+        const T* Cf = lf + tm[q].z * ch;
+        for (int c = 0; c < nx; c++) s += (double)A[c] * (double)B[c] * (double)Cf[c];
+      } else {
+        for (int c = 0; c < nx; c++) s += (double)A[c] * (double)B[c];
+      }
+      acc[q] += s;
+    }
+  }
+  for (int q = 0; q < BS_TPT; q++)
+    if (tm[q].x >= 0) atomicAdd(&sums[blockIdx.y * 256 * BS_TPT + q * 256 + threadIdx.x], acc[q]);
+}
+
+// Configuration order of bimodule.c:49-317.
+void bispectrum_configs(int Nk, std::vector<std::array<int, 3>>& tri) {
+  tri.clear();
+  for (int a = 0; a < Nk; a++) {
+    for (int b = 0; b < a; b++) {
+      if (b < a / 2) {
+        tri.push_back({a, a, b});
+        continue;
+      }
+      for (int c = std::max(a - b - 1, 0); c < b; c++) tri.push_back({a, b, c});
+      tri.push_back({a, b, b});
+      tri.push_back({a, a, b});
+    }
+    tri.push_back({a, a, a});
+  }
+}
+
+template <typename T>
+int run_bispectrum(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window, double R,
+                   int interlacing, int Nk, double k_min, double k_max, long long cap,
+                   double* kP, double* Pk, double* Nk_out, double* kB, double* Bk,
+                   double* Ntri_out) {
+  using C2 = typename cplx<T>::type;
+  EXH_REQUIRE(nd >= 2 && nd % 2 == 0, EXH_ERR_ARG, "nd must be even");
+  EXH_REQUIRE(ntype >= 1 && ntype <= 4, EXH_ERR_UNSUPPORTED, "1 to 4 tracer types supported");
+  EXH_REQUIRE(Nk >= 1 && Nk <= 4096, EXH_ERR_ARG, "Nk must lie in [1, 4096]");
+  EXH_REQUIRE(window >= 0 && window <= 5, EXH_ERR_UNSUPPORTED, "window %d", window);
+  EXH_REQUIRE(k_max > k_min, EXH_ERR_ARG, "k_max must exceed k_min");
+  std::vector<std::array<int, 3>> tri;
+  bispectrum_configs(Nk, tri);
+  const long long ntri = (long long)tri.size();
+  EXH_REQUIRE(cap >= ntri, EXH_ERR_ARG, "output capacity %lld below %lld configurations", cap,
+              ntri);
+  const int nsub = interlacing ? 2 : 1;
+  const int nz = nd / 2 + 1;
+  const size_t nh = (size_t)nd * nd * nz, ng = (size_t)nd * nd * nd;
+  const int nf = ntype + 1;  // fields per shell: tracers and the indicator
+  const size_t shells_bytes = (size_t)Nk * nf * ng * sizeof(T);
+  EXH_REQUIRE(shells_bytes <= (size_t)96 << 30, EXH_ERR_UNSUPPORTED,
+              "bispectrum: %zu bytes of shell fields exceed the 96 GiB budget", shells_bytes);
+  // 1. r2c of every grid, combined modes and shells
+  C2* raw = (C2*)ctx_buf(c, BUF_GRIDK, nh * ntype * nsub * sizeof(C2));
+  fft_r2c(c, nd, nd, nd, sizeof(T) == 4 ? 4 : 8, ntype * nsub, grid, raw);
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    size_t r = off;
+    off += (b + 255) & ~(size_t)255;
+    return r;
+  };
+  const size_t o_gk = take(nh * ntype * sizeof(C2)), o_sh = take(nh * sizeof(short)),
+               o_w = take(nh * nf * sizeof(C2)), o_ks = take((size_t)Nk * 8),
+               o_kc = take((size_t)Nk * 8);
+  char* kb = (char*)ctx_buf(c, BUF_SHELLK, off);
+  C2* gk = (C2*)(kb + o_gk);
+  short* shell = (short*)(kb + o_sh);
+  C2* work = (C2*)(kb + o_w);
+  double* ksum = (double*)(kb + o_ks);
+  unsigned long long* kcnt = (unsigned long long*)(kb + o_kc);
+  hipLaunchKernelGGL(k_zero<double>, dim3(grid_for(Nk, 256)), dim3(256), 0, c->stream, ksum,
+                     (size_t)Nk);
+  hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(grid_for(Nk, 256)), dim3(256), 0,
+                     c->stream, kcnt, (size_t)Nk);
+  ShellArgs sa{};
+  const T Lr = (T)L;
+  sa.nd = nd;
+  sa.nz = nz;
+  sa.ntype = ntype;
+  sa.nsub = nsub;
+  sa.Nk = Nk;
+  sa.window = window;
+  sa.kn = (double)(T)(2 * M_PI / (double)Lr);
+  sa.Lb = (double)(T)(Lr / (T)nd);
+  sa.Normk = (double)(T)std::pow((double)(Lr / (T)(nd * nd)), 3.0 / 2.0);
+  sa.R = (double)(T)R;
+  sa.kmin = (double)(T)k_min;
+  sa.dk = (double)(T)(((T)k_max - (T)k_min) / (T)Nk);
+  sa.dz = fast_div((unsigned)nz);
+  sa.dnd = fast_div((unsigned)nd);
+  EXH_REQUIRE(nh < (1ull << 31), EXH_ERR_UNSUPPORTED, "grid too large for the shell kernels");
+  {
+    TimedScope ts(c, "bispec_shells");
+    hipLaunchKernelGGL(k_shell_modes<T>, dim3(std::min(grid_for(nh, 256), 4096u)), dim3(256),
+                       (size_t)Nk * 12, c->stream, sa, raw, gk, shell, ksum, kcnt);
+    EXH_LAUNCH_CHECK();
+    // 2. the real field of every shell and tracer, and of its indicator
+    T* fields = (T*)ctx_buf(c, BUF_SHELLS, shells_bytes);
+    for (int s = 0; s < Nk; s++) {
+      hipLaunchKernelGGL(k_shell_fill<T>, dim3(std::min(grid_for(nh, 256), 8192u)), dim3(256), 0,
+                         c->stream, gk, shell, nh, ntype, s, work);
+      EXH_LAUNCH_CHECK();
+      fft_c2r(c, nd, nd, nd, sizeof(T) == 4 ? 4 : 8, nf, work, fields + (size_t)s * nf * ng);
+    }
+  }
+  // 3. the products: per configuration ntype^3 triples and the indicator's,
+  // per shell the ntype (ntype+1)/2 pairs and the indicator's square
+  const int NBs = ntype * ntype * ntype, NPs = ntype * (ntype + 1) / 2;
+  std::vector<int4> terms;
+  for (const auto& t : tri) {
+    for (int f1 = 0; f1 < ntype; f1++)
+      for (int f2 = 0; f2 < ntype; f2++)
+        for (int f3 = 0; f3 < ntype; f3++)
+          terms.push_back(make_int4(t[0] * nf + f1, t[1] * nf + f2, t[2] * nf + f3, 0));
+    terms.push_back(make_int4(t[0] * nf + ntype, t[1] * nf + ntype, t[2] * nf + ntype, 0));
+  }
+  const size_t tri_terms = terms.size();
+  for (int s = 0; s < Nk; s++) {
+    for (int f1 = 0; f1 < ntype; f1++)
+      for (int f2 = 0; f2 <= f1; f2++) terms.push_back(make_int4(s * nf + f1, s * nf + f2, -1, 0));
+    terms.push_back(make_int4(s * nf + ntype, s * nf + ntype, -1, 0));
+  }
+  const int nterm = (int)terms.size();
+  size_t toff = 0;
+  auto ttake = [&](size_t b) {
+    size_t r = toff;
+    toff += (b + 255) & ~(size_t)255;
+    return r;
+  };
+  const size_t o_terms = ttake((size_t)nterm * sizeof(int4)), o_sums = ttake((size_t)nterm * 8);
+  char* tb = (char*)ctx_buf(c, BUF_PART, toff);
+  int4* dterms = (int4*)(tb + o_terms);
+  double* dsums = (double*)(tb + o_sums);
+  EXH_CHECK_HIP(hipMemcpyAsync(dterms, terms.data(), (size_t)nterm * sizeof(int4),
+                               hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_zero<double>, dim3(grid_for(nterm, 256)), dim3(256), 0, c->stream, dsums,
+                     (size_t)nterm);
+  const int nfield = Nk * nf;
+  int ch = 256;
+  while (ch > 16 && (size_t)nfield * ch * sizeof(T) > 96 * 1024) ch >>= 1;
+  EXH_REQUIRE((size_t)nfield * ch * sizeof(T) <= 96 * 1024, EXH_ERR_UNSUPPORTED,
+              "bispectrum: %d shell fields exceed the LDS staging", nfield);
+  {
+    TimedScope ts(c, "bispec_products");
+    const T* fields = (const T*)c->buf[BUF_SHELLS];
+    const int lds = nfield * ch * (int)sizeof(T);
+    EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_shell_products<T>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    const unsigned ty = (unsigned)((nterm + 256 * BS_TPT - 1) / (256 * BS_TPT));
+    const size_t nch = (ng + ch - 1) / ch;
+    const unsigned tx = (unsigned)std::max<size_t>(1, std::min<size_t>(nch, 2048 / ty + 1));
+    hipLaunchKernelGGL(k_shell_products<T>, dim3(tx, ty), dim3(256), lds, c->stream, fields,
+                       nfield, ng, ch, dterms, nterm, dsums);
+    EXH_LAUNCH_CHECK();
+  }
+  std::vector<double> hs(nterm), hks(Nk);
+  std::vector<unsigned long long> hkc(Nk);
+  EXH_CHECK_HIP(hipMemcpyAsync(hs.data(), dsums, (size_t)nterm * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(hks.data(), ksum, (size_t)Nk * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+  EXH_CHECK_HIP(hipMemcpyAsync(hkc.data(), kcnt, (size_t)Nk * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  // 4. normalisation (bimodule.c:356-370) and outputs in T (spectrum.c:368-383)
+  std::vector<double> km(Nk);
+  for (int s = 0; s < Nk; s++) km[s] = (double)(T)(hkc[s] ? hks[s] / (double)hkc[s] : NAN);
+  const double L15 = std::pow((double)Lr, 1.5);
+  for (long long n = 0; n < ntri; n++) {
+    const double* v = hs.data() + (size_t)n * (NBs + 1);
+    const double IB = v[NBs];
+    for (int j = 0; j < NBs; j++) Bk[(size_t)j * ntri + n] = (double)(T)(v[j] / IB * L15);
+    Ntri_out[n] = (double)(T)(IB / (double)ng);
+    for (int q = 0; q < 3; q++) kB[(size_t)q * ntri + n] = km[tri[n][q]];
+  }
+  for (int s = 0; s < Nk; s++) {
+    const double* v = hs.data() + tri_terms + (size_t)s * (NPs + 1);
+    const double IP = v[NPs];
+    for (int j = 0; j < NPs; j++) Pk[(size_t)j * Nk + s] = (double)(T)(v[j] / IP);
+    Nk_out[s] = (double)(T)(IP / (double)ng);
+    kP[s] = km[s];
+  }
+  return (int)ntri;
+}
+
+template int run_bispectrum<float>(exh_ctx*, const float*, int, int, double, int, double, int,
+                                   int, double, double, long long, double*, double*, double*,
+                                   double*, double*, double*);
+template int run_bispectrum<double>(exh_ctx*, const double*, int, int, double, int, double, int,
+                                    int, double, double, long long, double*, double*, double*,
+                                    double*, double*, double*);
 
 template void run_grid<float>(exh_ctx*, const float*, const float*, const float*, const int*,
                               long long, int, int, double, double, double, int, int, double,

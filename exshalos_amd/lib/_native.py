@@ -97,6 +97,11 @@ _SIGS = {
     "exh_abundance_compute": (C.c_int, [_P, _P, C.c_longlong, C.c_double, C.c_double, C.c_int,
                                         C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, _P, _P,
                                         _P]),
+    "exh_bispectrum_size": (C.c_int, [C.c_int]),
+    "exh_bispectrum_compute": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_double, C.c_int,
+                                         C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         C.c_int, C.c_longlong, _P, _P, _P, _P, _P, _P,
+                                         C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -1,7 +1,7 @@
 """Mirror of the reference's ``pyexshalos.lib.spectrum`` extension module
 (hot-path methods of src/spectrum/spectrum.c:28-38): check_precision,
-grid_compute, power_compute, abundance_compute, with the same positional
-arguments.
+grid_compute, power_compute, bi_compute, abundance_compute, with the same
+positional arguments.
 
 Additions (keyword-only): ``device_out`` (return torch device tensors) and
 ``kbin_precision`` for power_compute (4: the reference's float bin index,
@@ -103,3 +103,35 @@ def abundance_compute(Mh_array, Mmin, Mmax, Nm, Lc, ndx, ndy, ndz, verbose):
             "abundance_compute")
     Rt = np_real(prec)
     return {"Mh": Mmean.astype(Rt), "dn": dn.astype(Rt), "dn_err": dn_err.astype(Rt)}
+
+
+def bi_compute(grid, ntype, nd, L, window, R, interlacing, Nk, k_min, k_max, verbose, nthreads):
+    """Bispectra of all tracer triples and the shell power spectra
+    (spectrum.c:291-406): dict kP, Pk [NPs][Nk], Nk, kB [3][Ntri],
+    Bk [ntype^3][Ntri], Ntri."""
+    prec = get_precision()
+    ctx = N.context()
+    g = to_device(grid, np_real(prec))
+    ntype, nd, Nk = int(ntype), int(nd), int(Nk)
+    il = int(bool(interlacing))
+    if g.numel() != ntype * (1 + il) * nd**3:
+        raise ValueError("grid size does not match ntype, interlacing and nd")
+    lib = N.load()
+    ntri = lib.exh_bispectrum_size(Nk)
+    N.check(min(ntri, 0), "bispectrum_size")
+    NPs, NBs = ntype * (ntype + 1) // 2, ntype**3
+    kP = np.zeros(Nk)
+    Pk = np.zeros((NPs, Nk))
+    cnt = np.zeros(Nk)
+    kB = np.zeros((3, ntri))
+    Bk = np.zeros((NBs, ntri))
+    ctri = np.zeros(ntri)
+    got = C.c_int()
+    N.check(lib.exh_bispectrum_compute(ctx, N.ptr(g), ntype, nd, float(L), int(window), float(R),
+                                       il, Nk, float(k_min), float(k_max), prec, ntri,
+                                       N.ptr(kP), N.ptr(Pk), N.ptr(cnt), N.ptr(kB), N.ptr(Bk),
+                                       N.ptr(ctri), C.byref(got)),
+            "bi_compute")
+    Rt = np_real(prec)
+    return {"kP": kP.astype(Rt), "Pk": Pk.astype(Rt), "Nk": cnt.astype(Rt), "kB": kB.astype(Rt),
+            "Bk": Bk.astype(Rt), "Ntri": ctri.astype(Rt)}

@@ -1,6 +1,6 @@
-"""Density grids, power spectra and halo abundances: the hot-path part of
-pyexshalos.simulation (simulation.py:9-311, 719-806), same signatures and
-defaults.
+"""Density grids, power spectra, bispectra and halo abundances: the hot-path
+part of pyexshalos.simulation (simulation.py:9-311, 467-588, 719-806), same
+signatures and defaults.
 
 Extra keyword-only arguments: ``device_out`` (torch device tensors) and,
 for Compute_Power_Spectrum, ``kbin_precision`` (4: float bin index as the
@@ -176,3 +176,48 @@ def Compute_Abundance(
     ndz = nd if ndz is None else ndz
     return abundance_compute(Mh, Rt(Mmin), Rt(Mmax), np.int32(Nm), Rt(Lc), np.int32(ndx),
                              np.int32(ndy), np.int32(ndz), np.int32(verbose))
+
+
+def Compute_BiSpectrum(
+    grid,
+    L: float = 1000.0,
+    window: Union[str, int] = "CIC",
+    R: float = 4.0,
+    Nk: int = 25,
+    k_min: Optional[float] = None,
+    k_max: Optional[float] = None,
+    folds: int = 1,
+    verbose: bool = False,
+    nthreads: int = 1,
+    ntypes: int = 1,
+) -> Dict[str, np.ndarray]:
+    """Bispectra and shell power spectra of density grids (reference
+    simulation.py:467-588): dict kP, Pk, Nk, kB, Bk, Ntri."""
+    shape = tuple(grid.shape)
+    if len(shape) == 3:
+        interlacing, ntypes = 0, 1
+    elif len(shape) == 5:
+        interlacing, ntypes = 1, shape[0]
+    elif len(shape) == 4 and ntypes == 1:
+        interlacing = 1
+    elif len(shape) == 4 and ntypes > 1:
+        interlacing = 0
+    else:
+        raise ValueError("grid must have 3, 4 or 5 dimensions")
+    nd = shape[-1]
+    L = L / folds
+    if k_min is None:
+        k_min = 2.0 * np.pi / L
+    if k_max is None:
+        k_max = np.pi / L * nd
+    from .lib.spectrum import bi_compute, check_precision
+
+    Rt = np.float32 if check_precision() == 4 else np.float64
+    wid = _window_id(window, allow_none=True)
+    x = bi_compute(grid, np.int32(ntypes), np.int32(nd), Rt(L), np.int32(wid), Rt(R),
+                   np.int32(interlacing), np.int32(Nk), Rt(k_min), Rt(k_max), np.int32(verbose),
+                   np.int32(nthreads))
+    if ntypes == 1:
+        x["Pk"] = x["Pk"].reshape([Nk])
+        x["Bk"] = x["Bk"].reshape([len(x["Ntri"])])
+    return x

@@ -247,6 +247,26 @@ int exh_abundance_compute(exh_ctx* ctx, const void* Mh, long long nh, double Mmi
                           int Nm, double Lc, int ndx, int ndy, int ndz, int precision,
                           double* Mmean_out, double* dn_out, double* dn_err_out);
 
+/* Number of bispectrum configurations for Nk shells (the order of
+ * bimodule.c:49-317), or EXH_ERR_ARG. */
+int exh_bispectrum_size(int Nk);
+
+/* Bispectra of all tracer triples and the shell power spectra: replaces
+ * bi_compute (spectrum.c:291-406) / Bi_Spectrum (bimodule.c:4-373).  grid:
+ * device [ntype][1+interlacing][nd^3] T (not modified); window and R as in
+ * exh_power_compute.  Every shell's real field is computed once (one c2r
+ * per shell and tracer, plus the shell indicator) and kept on the device.
+ * Outputs (host), Ntri = exh_bispectrum_size(Nk) <= cap: kP[Nk] mean |k|
+ * per shell, Pk[ntype(ntype+1)/2][Nk], Nk_out[Nk] (sum of the indicator
+ * squared / ng), kB[3][Ntri] mean |k| of the three shells, Bk[ntype^3][Ntri]
+ * (sum of the triple product / sum of the indicators' triple product x
+ * L^(3/2)), Ntri_out[Ntri] (that indicator sum / ng); *ntri = Ntri.  Sums
+ * in double; outputs rounded to T as the reference's arrays. */
+int exh_bispectrum_compute(exh_ctx* ctx, const void* grid, int ntype, int nd, double L,
+                           int window, double R, int interlacing, int Nk, double k_min,
+                           double k_max, int precision, long long cap, double* kP, double* Pk,
+                           double* Nk_out, double* kB, double* Bk, double* Ntri_out, int* ntri);
+
 #ifdef __cplusplus
 }
 #endif

@@ -464,3 +464,101 @@ def abundance(Mh, Mmin=-1.0, Mmax=-1.0, Nm=25, Lc=2.0, ndx=256, ndy=256, ndz=256
         _ptr(m), C.c_longlong(len(m)), RC(Mmin), RC(Mmax), C.c_int(Nm), RC(Lc), C.c_int(ndx),
         C.c_int(ndy), C.c_int(ndz), _ptr(Mmean), _ptr(dn), _ptr(dn_err))
     return {"Mh": Mmean, "dn": dn, "dn_err": dn_err}
+
+
+# --------------------------------------------------------------------------
+# bispectrum
+# --------------------------------------------------------------------------
+def bispectrum_triangles(Nk):
+    """The reference's configuration order (bimodule.c:49-317): for each bin
+    a, every b < a gives (a, a, b) when b < floor(a/2), else (a, b, c) for
+    max(a-b-1, 0) <= c < b, then (a, b, b) and (a, a, b); each a ends with
+    (a, a, a).  Returns a list of bin triples."""
+    tri = []
+    for a in range(Nk):
+        for b in range(a):
+            if b < a // 2:
+                tri.append((a, a, b))
+                continue
+            for c in range(max(a - b - 1, 0), b):
+                tri.append((a, b, c))
+            tri.append((a, b, b))
+            tri.append((a, a, b))
+        tri.append((a, a, a))
+    return tri
+
+
+def shell_index(nd, L, Nk, k_min, k_max, precision=8):
+    """Bin of every half-complex mode, Indice(|k|, k_min, dk) in the grid's
+    precision (bimodule.c:72-75, spectrum_h.c:39-48), and |k|."""
+    dt = _real(precision)
+    kn = dt(2 * np.pi / dt(L))
+    dk = dt((dt(k_max) - dt(k_min)) / dt(Nk))
+    nz = nd // 2 + 1
+    f = np.arange(nd)
+    f = np.where(2 * f < nd, f, f - nd).astype(dt) * kn
+    fz = np.arange(nz)
+    fz = np.where(2 * fz < nd, fz, fz - nd).astype(dt) * kn
+    kx, ky, kz = np.meshgrid(f, f, fz, indexing="ij")
+    kmod = np.sqrt(kx * kx + ky * ky + kz * kz).astype(dt)
+    ind = np.floor(((kmod - dt(k_min)) / dk).astype(dt)).astype(np.int64)
+    return ind, kmod
+
+
+def bispectrum(grid, ntype, nd, L, window=2, R=4.0, interlacing=False, Nk=25, k_min=None,
+               k_max=None, precision=8):
+    """bi_compute / Bi_Spectrum (spectrum.c:291-406, bimodule.c:4-373): every
+    shell's field by c2r of its modes, sums of triple (and pair) products over
+    the cells in double, B = sum / sum(I I I) L^(3/2).  Returns the
+    reference's keys kP, Pk [NPs][Nk], Nk, kB [3][Ntri], Bk [NBs][Ntri],
+    Ntri."""
+    dt = _real(precision)
+    RC = _rc(precision)
+    sfx = _sfx(precision)
+    il = bool(interlacing)
+    grid = np.asarray(grid, dtype=dt).reshape(ntype, 1 + int(il), nd, nd, nd)
+    L = dt(L)
+    k_min = 2.0 * np.pi / L if k_min is None else k_min
+    k_max = np.pi / L * nd if k_max is None else k_max
+    gks = []
+    for t in range(ntype):
+        f1 = np.ascontiguousarray(r2c(grid[t, 0]).astype(dt))
+        f2 = np.ascontiguousarray(r2c(grid[t, 1]).astype(dt)) if il else None
+        out = np.zeros_like(f1)
+        getattr(lib(), "ora_gridk" + sfx)(C.c_int(nd), RC(L), C.c_int(window), RC(R),
+                                          _ptr(f1), _ptr(f2), _ptr(out))
+        gks.append(out[..., 0].astype(np.float64) + 1j * out[..., 1].astype(np.float64))
+    ind, kmod = shell_index(nd, L, Nk, k_min, k_max, precision)
+    F, I, km = [], [], []
+    for a in range(Nk):
+        m = ind == a
+        F.append([(np.fft.irfftn(np.where(m, g, 0), s=(nd, nd, nd), axes=(0, 1, 2)) * nd**3)
+                  .astype(dt).astype(np.float64) for g in gks])
+        I.append((np.fft.irfftn(m.astype(np.complex128), s=(nd, nd, nd), axes=(0, 1, 2)) *
+                  nd**3).astype(dt).astype(np.float64))
+        cnt = int(m.sum())
+        km.append(np.float64(kmod[m].astype(np.float64).sum()) / cnt if cnt else np.nan)
+    tri = bispectrum_triangles(Nk)
+    NBs, NPs, ng = ntype**3, ntype * (ntype + 1) // 2, nd**3
+    B = np.zeros((NBs, len(tri)))
+    IB = np.zeros(len(tri))
+    for n, (i1, i2, i3) in enumerate(tri):
+        for f1 in range(ntype):
+            for f2 in range(ntype):
+                for f3 in range(ntype):
+                    B[(f1 * ntype + f2) * ntype + f3, n] = np.sum(F[i1][f1] * F[i2][f2] *
+                                                               F[i3][f3])
+        IB[n] = np.sum(I[i1] * I[i2] * I[i3])
+    P = np.zeros((NPs, Nk))
+    IP = np.zeros(Nk)
+    for a in range(Nk):
+        q = 0
+        for f1 in range(ntype):
+            for f2 in range(f1 + 1):
+                P[q, a] = np.sum(F[a][f1] * F[a][f2])
+                q += 1
+        IP[a] = np.sum(I[a] * I[a])
+    kB = np.array([[km[t[j]] for t in tri] for j in range(3)])
+    return {"kP": np.array(km).astype(dt), "Pk": (P / IP).astype(dt), "Nk": (IP / ng).astype(dt),
+            "kB": kB.astype(dt), "Bk": (B / IB * np.float64(L) ** 1.5).astype(dt),
+            "Ntri": (IB / ng).astype(dt), "B64": B, "IB64": IB}

@@ -476,3 +476,32 @@ def test_pk_truncation_field_and_halos(bbks, prec):
     assert _rel(out["grid"], ref["delta"]) < tol
     ref_h = O.find_halos(out["grid"], k, P, LC, seed=1, precision=prec)
     np.testing.assert_array_equal(out["Mh"], ref_h["Mh"])
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("ntype,il,window", [(1, 0, 2), (1, 1, 2), (2, 1, 1), (1, 0, 3)])
+def test_bispectrum(prec, ntype, il, window):
+    """bi_compute (spectrum.c:291-406, bimodule.c): configurations in the
+    reference's order, shell power spectra, bispectra of every tracer triple
+    and the triangle counts against the oracle (whose real-space sums are
+    pinned by Fourier-space triangle sums)."""
+    from exshalos_amd.lib import _native as N
+
+    set_precision(prec)
+    RT = np.float32 if prec == 4 else np.float64
+    rng = np.random.default_rng(40 + ntype + il + window)
+    nd, L, Nk = 24, 120.0, 7
+    grid = rng.normal(size=(ntype, 1 + il, nd, nd, nd)).astype(RT)
+    grid[:, :, :4] += 1.5  # some phase coherence, so that B is not noise only
+    assert N.load().exh_bispectrum_size(Nk) == len(O.bispectrum_triangles(Nk))
+    ref = O.bispectrum(grid, ntype, nd, L, window=window, R=6.0, interlacing=bool(il), Nk=Nk,
+                       precision=prec)
+    kmin, kmax = 2 * np.pi / L, np.pi / L * nd
+    out = SP.bi_compute(grid, ntype, nd, L, window, 6.0, il, Nk, kmin, kmax, 0, 1)
+    tol = 1e-4 if prec == 4 else 1e-10
+    for key in ("kP", "Nk", "kB", "Ntri"):
+        assert out[key].shape == ref[key].shape
+        np.testing.assert_allclose(out[key], ref[key], rtol=1e-6 if prec == 4 else 1e-12)
+    assert _rel(out["Pk"], ref["Pk"]) < tol
+    assert out["Bk"].shape == ref["Bk"].shape
+    assert _rel(out["Bk"], ref["Bk"]) < tol

@@ -289,3 +289,58 @@ def test_abundance(precision, given_range):
     lo = edges[:-1][full]
     assert np.all(lnm >= lo - 1e-6) and np.all(lnm < lo + dlnM + 1e-6)
     assert out["Mh"][0] == 0 and out["dn"][0] == 0
+
+
+def test_bispectrum_configurations():
+    """Configuration order of bimodule.c:49-317: bins non-increasing in each
+    triple, no triple twice, every b >= floor(a/2) triple closes in bin space
+    (c >= a - b - 1), and each a ends with (a, a, a)."""
+    tri = O.bispectrum_triangles(12)
+    assert len(set(tri)) == len(tri)
+    assert all(a >= b >= c for a, b, c in tri)
+    assert all(c >= a - b - 1 for a, b, c in tri if b >= a // 2 and (a, b, c) != (a, a, b))
+    ends = [i for i, t in enumerate(tri) if t[0] == t[1] == t[2]]
+    assert [tri[i][0] for i in ends] == list(range(12))
+    assert ends[-1] == len(tri) - 1
+
+
+@pytest.mark.parametrize("window", [0, 2])
+def test_bispectrum_equals_fourier_triangle_sums(window):
+    """Sum over cells of F_a F_b F_c, F_s the inverse transform of the modes of
+    shell s, equals ng times the sum over closed triangles k1 + k2 + k3 = 0
+    (k_i in shells a, b, c) of D(k1) D(k2) D(k3), D the deconvolved,
+    normalised modes on the full grid; the I fields count closed triangles;
+    P is Parseval's sum over the shell."""
+    nd, L, Nk = 8, 64.0, 4
+    rng = np.random.default_rng(5 + window)
+    g = rng.normal(size=(nd, nd, nd))
+    out = O.bispectrum(g[None, None], 1, nd, L, window=window, Nk=Nk, precision=8)
+    kn = 2 * np.pi / L
+    Lb = L / nd
+    f = np.fft.fftfreq(nd, 1.0 / nd)
+    kx, ky, kz = np.meshgrid(f * kn, f * kn, f * kn, indexing="ij")
+    W = 1.0
+    if window == 2:
+        s = lambda x: np.where(x == 0, 1.0, np.sin(x) / np.where(x == 0, 1.0, x))  # noqa: E731
+        W = (s(kx * Lb / 2) * s(ky * Lb / 2) * s(kz * Lb / 2)) ** 2
+    D = np.fft.fftn(g) * (L / nd**2) ** 1.5 / W
+    kmin, kmax = 2 * np.pi / L, np.pi / L * nd
+    b = np.floor((np.sqrt(kx**2 + ky**2 + kz**2) - kmin) / ((kmax - kmin) / Nk)).astype(int)
+    idx = np.indices((nd, nd, nd)).reshape(3, -1).T
+    bf, Df = b.reshape(-1), D.reshape(-1)
+    ng = nd**3
+    for n, (a, bb, c) in enumerate(O.bispectrum_triangles(Nk)):
+        i1, i2 = np.nonzero(bf == a)[0], np.nonzero(bf == bb)[0]
+        k3 = (-(idx[i1][:, None, :] + idx[i2][None, :, :])) % nd
+        j3 = (k3[..., 0] * nd + k3[..., 1]) * nd + k3[..., 2]
+        closed = bf[j3] == c
+        S = ng * np.sum(Df[i1][:, None] * Df[i2][None, :] * Df[j3] * closed)
+        assert abs(S.imag) < 1e-8 * max(abs(S.real), 1.0)
+        np.testing.assert_allclose(out["B64"][0, n], S.real, rtol=1e-9,
+                                   atol=1e-9 * np.max(np.abs(out["B64"])))
+        np.testing.assert_allclose(out["IB64"][n], ng * closed.sum(), rtol=1e-12)
+    for a in range(Nk):
+        m = bf == a
+        P = ng * np.sum(np.abs(Df[m]) ** 2) / (ng * m.sum())
+        np.testing.assert_allclose(out["Pk"][0, a], P, rtol=1e-9)
+        np.testing.assert_allclose(out["Nk"][a], m.sum(), rtol=1e-12)
