@@ -60,11 +60,13 @@ __device__ inline double spline_eval_v(const SplineView& s, double xv) {
   return s.y[lo] + t * (s.b[lo] + t * (s.c[lo] + t * s.d[lo]));
 }
 
-// The spline tables in LDS (when a.stage) or in global memory.
-template <typename T>
+// The spline tables in LDS (STAGE) or in global memory.  STAGE is a
+// template argument so that the staged kernel's loads are known to be LDS
+// loads (a pointer that may be either becomes a slower flat access).
+template <bool STAGE, typename T>
 __device__ inline SplineView stage_spline(const FieldArgs<T>& a, double* lds) {
   SplineView v{a.sx, a.sy, a.sb, a.sc, a.sd, a.nk};
-  if (!a.stage) return v;
+  if (!STAGE) return v;
   const double* src[5] = {a.sx, a.sy, a.sb, a.sc, a.sd};
   for (int q = 0; q < 5; q++)
     for (int e = threadIdx.x; e < a.nk; e += blockDim.x) lds[q * a.nk + e] = src[q][e];
@@ -165,12 +167,12 @@ __device__ inline void mode_coords(size_t id, size_t nh, int n1, int nz2, const 
 // Mode kernels: a flat grid-stride walk over the half-complex modes, so
 // consecutive lanes take consecutive modes (a row of nd/2+1 modes does not
 // fit a workgroup evenly).
-template <typename T>
+template <typename T, bool STAGE>
 __global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale, FastDiv dz, FastDiv dy,
                                                typename cplx<T>::type* out,
                                                typename cplx<T>::type* raw) {
   extern __shared__ double spl[];
-  const SplineView sp = stage_spline(a, spl);
+  const SplineView sp = stage_spline<STAGE>(a, spl);
   const size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
   for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
        id += (size_t)gridDim.x * blockDim.x) {
@@ -185,12 +187,12 @@ __global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale, FastDiv 
 
 // phi_k = delta_k / (-k^2) * scale with delta_k taken from the white-noise
 // modes (no stored copy of delta_k is needed).
-template <typename T>
+template <typename T, bool STAGE>
 __global__ __launch_bounds__(256) void k_poisson_modes(FieldArgs<T> a, T k_smooth, T scale,
                                                        FastDiv dz, FastDiv dy,
                                                        typename cplx<T>::type* out) {
   extern __shared__ double spl[];
-  const SplineView sp = stage_spline(a, spl);
+  const SplineView sp = stage_spline<STAGE>(a, spl);
   const size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
   for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
        id += (size_t)gridDim.x * blockDim.x) {
@@ -594,9 +596,9 @@ void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k
   C2* ck = (C2*)ctx_buf(c, BUF_CKF, nh * sizeof(C2));
   {
     TimedScope ts(c, "field_modes");
-    hipLaunchKernelGGL(k_field<T>, dim3(mode_grid((long long)nh)), dim3(256), spline_lds(a),
-                       c->stream, a, (T)b->Normx, fast_div(b->nz2), fast_div(b->nd[1]), ck,
-                       (C2*)deltak_raw);
+    auto kf = a.stage ? k_field<T, true> : k_field<T, false>;
+    hipLaunchKernelGGL(kf, dim3(mode_grid((long long)nh)), dim3(256), spline_lds(a), c->stream, a,
+                       (T)b->Normx, fast_div(b->nz2), fast_div(b->nd[1]), ck, (C2*)deltak_raw);
     EXH_LAUNCH_CHECK();
   }
   if (after_modes) after_modes();
@@ -642,9 +644,9 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
     {
       TimedScope ts(c, "lpt1_kspace");
       auto a = make_field_args<T>(c, b, s, 1e5);
-      hipLaunchKernelGGL(k_poisson_modes<T>, dim3(mode_grid((long long)nh)), dim3(256),
-                         spline_lds(a), c->stream, a, (T)k_smooth, Normx, fast_div(b->nz2),
-                         fast_div(n1), ck);
+      auto kp = a.stage ? k_poisson_modes<T, true> : k_poisson_modes<T, false>;
+      hipLaunchKernelGGL(kp, dim3(mode_grid((long long)nh)), dim3(256), spline_lds(a), c->stream,
+                         a, (T)k_smooth, Normx, fast_div(b->nz2), fast_div(n1), ck);
       EXH_LAUNCH_CHECK();
     }
     fft_c2r(c, n0, n1, n2, b->precision, 1, ck, phi);
