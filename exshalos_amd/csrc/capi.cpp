@@ -96,6 +96,12 @@ int exh_ctx_destroy(exh_ctx* c) {
     (void)hipStreamSynchronize(c->aux);
     (void)hipStreamDestroy(c->aux);
   }
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->ev_side_fork) (void)hipEventDestroy(c->ev_side_fork);
+  if (c->ev_side_join) (void)hipEventDestroy(c->ev_side_join);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->ev_spec) (void)hipEventDestroy(c->ev_spec);

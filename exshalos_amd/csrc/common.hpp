@@ -147,9 +147,20 @@ struct exh_ctx {
   // stream's, ordered by the two events; created on first use
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_spec = nullptr;
+  // third stream (default priority) for the peak sort's small segments,
+  // forked per level and joined at the end of the sort; created on first use
+  hipStream_t side = nullptr;
+  hipEvent_t ev_side_fork = nullptr, ev_side_join = nullptr;
   bool own_stream = false;
   void* buf[exh::BUF_NSLOTS] = {};
   size_t cap[exh::BUF_NSLOTS] = {};
+  // allocation number of each slot's current buffer (0: none yet)
+  unsigned long long buf_gen[exh::BUF_NSLOTS] = {};
+  unsigned long long n_allocs = 0;
+  // halo growth claims: the allocation they were cleared for and the last
+  // round tag (epoch) used in them
+  unsigned long long claim_gen = 0;
+  unsigned claim_epoch = 0;
   std::map<exh::FftKey, exh::FftPlan> plans;
   std::map<int, void*> fft_tw;  // twiddle tables of the strided FFT passes
   // per-kernel HIP-event timing (enabled by exh_ctx_set_timing)

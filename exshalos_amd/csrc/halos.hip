@@ -52,7 +52,7 @@ struct PeakGeom {
 };
 
 template <typename T>
-__device__ inline bool is_peak_at(const T* d, unsigned c, int i, int j, int k,
+__device__ inline bool is_peak_at(const T* d, unsigned c, T v, int i, int j, int k,
                                   const PeakGeom& g) {
   const unsigned xp = i == g.n0 - 1 ? c - (unsigned)(g.n0 - 1) * g.sx : c + g.sx;
   const unsigned xm = i == 0 ? c + (unsigned)(g.n0 - 1) * g.sx : c - g.sx;
@@ -60,23 +60,30 @@ __device__ inline bool is_peak_at(const T* d, unsigned c, int i, int j, int k,
   const unsigned ym = j == 0 ? c + (unsigned)(g.n1 - 1) * g.sy : c - g.sy;
   const unsigned zp = k == g.n2 - 1 ? c - (unsigned)(g.n2 - 1) : c + 1;
   const unsigned zm = k == 0 ? c + (unsigned)(g.n2 - 1) : c - 1;
-  const T v = d[c];
   const T a = d[xp], b = d[xm], e = d[yp], f = d[ym], h = d[zp], l = d[zm];
   return (v > a) & (v > b) & (v > e) & (v > f) & (v > h) & (v > l);
 }
 
 // Peak flags of the thread's PK_ITEMS cells (cells past the grid: false).
-template <typename T>
-__device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const PeakGeom& g) {
+// The pass also initialises the growth rounds' per-cell state of those
+// cells: SIDE 1 copies delta to eff, SIDE 2 sets the flags free.
+template <int SIDE, typename T>
+__device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const PeakGeom& g,
+                                     T* eff, int* flag) {
   unsigned r = fdiv(c0, g.dz);
   int k = (int)(c0 - r * (unsigned)g.n2);
   int i = (int)fdiv(r, g.dy);
   int j = (int)(r - (unsigned)i * (unsigned)g.n1);
   unsigned bits = 0;
+  T cv[PK_ITEMS];
 #pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
     const unsigned c = c0 + (unsigned)q * 256u;
-    if (c < ng && is_peak_at(d, c, i, j, k, g)) bits |= 1u << q;
+    cv[q] = 0;
+    if (c < ng) {
+      cv[q] = d[c];
+      if (is_peak_at(d, c, cv[q], i, j, k, g)) bits |= 1u << q;
+    }
     k += 256;
     while (k >= g.n2) {
       k -= g.n2;
@@ -84,6 +91,15 @@ __device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const
         j = 0;
         ++i;
       }
+    }
+  }
+  // stores after every load, so that no load waits behind a store
+#pragma unroll
+  for (int q = 0; q < PK_ITEMS; q++) {
+    const unsigned c = c0 + (unsigned)q * 256u;
+    if (c < ng) {
+      if (SIDE == 1) eff[c] = cv[q];
+      if (SIDE == 2) flag[c] = FLAG_FREE;
     }
   }
   return bits;
@@ -94,11 +110,12 @@ __device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const
 // coalesced.  The write pass ranks peaks in cell order from per-row wave
 // ballots, which keeps the reference's lexicographic peak order.
 template <typename T>
-__global__ __launch_bounds__(256) void k_peak_count(const T* d, PeakGeom g, long long* bcnt) {
+__global__ __launch_bounds__(256) void k_peak_count(const T* d, PeakGeom g, long long* bcnt,
+                                                    T* eff) {
   __shared__ int sh[4];
   const unsigned ng = (unsigned)g.n0 * g.sx;
   const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
-  int cnt = __popc(peak_bits(d, base, ng, g));
+  int cnt = __popc(peak_bits<1>(d, base, ng, g, eff, (int*)nullptr));
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
   __syncthreads();
@@ -107,12 +124,12 @@ __global__ __launch_bounds__(256) void k_peak_count(const T* d, PeakGeom g, long
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_peak_write(const T* d, PeakGeom g, const long long* boff,
-                                                    T* key, int* cell) {
+                                                    T* key, int* cell, int* flag) {
   __shared__ int wcnt[PK_ITEMS][4];
   const unsigned ng = (unsigned)g.n0 * g.sx;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
-  const unsigned bits = peak_bits(d, base, ng, g);
+  const unsigned bits = peak_bits<2>(d, base, ng, g, (T*)nullptr, flag);
   unsigned long long bal[PK_ITEMS];
 #pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
@@ -737,10 +754,25 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   const PeakGeom pg{n0, n1, n2, (unsigned)((long long)n1 * n2), (unsigned)n2, fast_div(n2),
                     fast_div(n1)};
   long long* bc = (long long*)ctx_buf(c, BUF_HALO, (size_t)(2 * nblk + 2) * 8);
+  int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
+  T* eff = (T*)ctx_buf(c, BUF_EFF, (size_t)ng * sizeof(T));
+  unsigned long long* claim = (unsigned long long*)ctx_buf(c, BUF_CLAIM, (size_t)ng * 8);
+  // Claims carry their round's tag (an epoch counted over the context's
+  // calls) in the high half, so the claims of earlier rounds and calls never
+  // match a later round: the array is cleared only when it is newly
+  // allocated or the epochs are about to run out.
+  if (c->claim_gen != c->buf_gen[BUF_CLAIM] || c->claim_epoch > 0xffffffffu - 2000000u) {
+    const long long ncap = (long long)(c->cap[BUF_CLAIM] / 8);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncap, 256 * 8)), dim3(256), 0, c->stream, claim,
+                       ncap, 0ull);
+    EXH_LAUNCH_CHECK();
+    c->claim_gen = c->buf_gen[BUF_CLAIM];
+    c->claim_epoch = 0;
+  }
   {
     TimedScope ts(c, "peaks");
     hipLaunchKernelGGL(k_peak_count<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, pg,
-                       bc);
+                       bc, eff);
     EXH_LAUNCH_CHECK();
     exclusive_scan<long long>(c, bc, bc + nblk + 1, nblk);
   }
@@ -753,7 +785,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   {
     TimedScope ts(c, "peaks_write");
     hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, pg,
-                       bc + nblk + 1, pkey, pcell);
+                       bc + nblk + 1, pkey, pcell, flag);
     EXH_LAUNCH_CHECK();
   }
   // 2. sort (densities in lexicographic peak order; each peak's cell rides
@@ -803,16 +835,8 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   char* cand = (char*)ctx_buf(c, BUF_CAND, coff);
   int* ccell = (int*)(cand + o_cc);
   T* cden = (T*)(cand + o_cd);
-  int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
-  T* eff = (T*)ctx_buf(c, BUF_EFF, (size_t)ng * sizeof(T));
-  EXH_CHECK_HIP(hipMemcpyAsync(eff, delta, (size_t)ng * sizeof(T), hipMemcpyDeviceToDevice,
-                               c->stream));
-  unsigned long long* claim = (unsigned long long*)ctx_buf(c, BUF_CLAIM, (size_t)ng * 8);
+  // eff (delta) and the free flags were written by the peak passes
   S.flag = flag;
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ng, 256 * 8)), dim3(256), 0, c->stream, flag, ng,
-                     FLAG_FREE);
-  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ng, 256 * 8)), dim3(256), 0, c->stream, claim, ng,
-                     0ull);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, derr, 1, 0);
   EXH_LAUNCH_CHECK();
 
@@ -869,7 +893,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     TimedScope ts(c, "halo_rounds");
     while (npend > 0) {
       round++;
-      unsigned long long tag_hi = (unsigned long long)round << 32;
+      const unsigned long long tag_hi = (unsigned long long)(++c->claim_epoch) << 32;
       ga.prank = pend;
       ga.pcell = qcell;
       ga.pden = qden;

@@ -36,6 +36,7 @@ void* ctx_buf(exh_ctx* c, Slot s, size_t bytes) {
     size_t want = (bytes + 4095) & ~(size_t)4095;
     EXH_CHECK_HIP(hipMalloc(&c->buf[s], want));
     c->cap[s] = want;
+    c->buf_gen[s] = ++c->n_allocs;
   }
   return c->buf[s];
 }

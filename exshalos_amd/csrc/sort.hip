@@ -905,6 +905,35 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
   b.floor = floor;
   int cur = 0;
   long long nsmall_host = 0;
+  // Small segments are final once their level has placed them: each level's
+  // new ones are sorted on the context's side stream (forked after the
+  // level, joined when the sort ends), beside the next levels.
+  if (!c->side) {
+    EXH_CHECK_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    EXH_CHECK_HIP(hipEventCreateWithFlags(&c->ev_side_fork, hipEventDisableTiming));
+    EXH_CHECK_HIP(hipEventCreateWithFlags(&c->ev_side_join, hipEventDisableTiming));
+  }
+  struct SideJoin {
+    exh_ctx* c;
+    bool used = false;
+    ~SideJoin() {
+      if (!used) return;
+      (void)hipEventRecord(c->ev_side_join, c->side);
+      (void)hipStreamWaitEvent(c->stream, c->ev_side_join, 0);
+    }
+  } join{c};
+  long long nsmall_done = 0;
+  auto sort_small = [&]() {
+    if (nsmall_host <= nsmall_done) return;
+    EXH_CHECK_HIP(hipEventRecord(c->ev_side_fork, c->stream));
+    EXH_CHECK_HIP(hipStreamWaitEvent(c->side, c->ev_side_fork, 0));
+    join.used = true;
+    hipLaunchKernelGGL((k_small_sort<K, SMALL>), dim3((unsigned)(nsmall_host - nsmall_done)),
+                       dim3(64 * SMALL_WAVES), 0, c->side, keys, vals, kB, vB, kC, vC,
+                       b.slo + nsmall_done, b.shi + nsmall_done, b.sbuf + nsmall_done);
+    EXH_LAUNCH_CHECK();
+    nsmall_done = nsmall_host;
+  };
   if (n <= SMALL) {
     long long init[2] = {0, n - 1};
     int zero = 0;
@@ -988,17 +1017,13 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
     nsmall_host = (long long)hs->nsmall;
     EXH_REQUIRE(nseg_next <= maxseg && nsmall_host <= maxsmall, EXH_ERR_INTERNAL,
                 "peak sort: segment lists overflow (%d, %lld)", nseg_next, nsmall_host);
+    sort_small();
     b.nseg = nseg_next;
     b.ntiles = ntiles_next;
     cur = 1 - cur;
     src_id = dst_id;
   }
-  if (nsmall_host > 0) {
-    hipLaunchKernelGGL((k_small_sort<K, SMALL>), dim3((unsigned)nsmall_host),
-                       dim3(64 * SMALL_WAVES), 0, c->stream, keys, vals, kB, vB, kC, vC, b.slo,
-                       b.shi, b.sbuf);
-    EXH_LAUNCH_CHECK();
-  }
+  sort_small();
   if (c->stat_sort_async_open > 0) {
     EXH_CHECK_HIP(hipMemcpyAsync(&hs->err, &st->err, 8, hipMemcpyDeviceToHost, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));

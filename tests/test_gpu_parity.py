@@ -63,6 +63,24 @@ def test_find_halos_bit_exact(bbks, prec, nd, seed):
     np.testing.assert_array_equal(out["posh"], lag)
 
 
+def test_find_halos_repeated_calls_mixed_sizes(bbks):
+    """The growth claims persist across calls and are not cleared between
+    them (round tags keep counting): a large box, a small one in the same
+    buffers, then the large one again must each match the oracle."""
+    k, P = bbks
+    set_precision(4)
+    cases = []
+    for nd, seed in [(64, 2), (32, 4), (64, 2)]:
+        delta, _, _ = O.field_from_modes(k, P, nd, LC, _noise(nd, seed, 4), precision=4)
+        cases.append((delta, O.find_halos(delta, k, P, LC, seed=12345, precision=4)))
+    for delta, ref in cases:
+        out = X.find_halos(delta, k, P, LC, 0.31, 0.0, -1.0, 1, 1.0, 0.0, 0.0, 1, 0, 0)
+        cat = X.halo_catalogue()
+        np.testing.assert_array_equal(cat["x"], ref["x"])
+        np.testing.assert_array_equal(cat["count"], ref["count"])
+        np.testing.assert_array_equal(out["flag"].reshape(-1), ref["flag"].reshape(-1))
+
+
 @pytest.mark.parametrize("prec", [4, 8])
 def test_find_halos_ties_and_barrier(bbks, prec):
     """Quantised density values force many ties (Lomuto tie order) and an
