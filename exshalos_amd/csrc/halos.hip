@@ -109,13 +109,19 @@ __device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const
 // block tests cells tile + q * 256 + x (q < PK_ITEMS), so every load row is
 // coalesced.  The write pass ranks peaks in cell order from per-row wave
 // ballots, which keeps the reference's lexicographic peak order.
+// The count pass keeps each thread's PK_ITEMS peak flags as one byte
+// (ng / 8 bytes), so the write pass reads them instead of testing the
+// neighbours again.
 template <typename T>
 __global__ __launch_bounds__(256) void k_peak_count(const T* d, PeakGeom g, long long* bcnt,
-                                                    T* eff) {
+                                                    T* eff, unsigned char* pbits) {
+  static_assert(PK_ITEMS == 8, "one byte of peak flags per thread");
   __shared__ int sh[4];
   const unsigned ng = (unsigned)g.n0 * g.sx;
   const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
-  int cnt = __popc(peak_bits<1>(d, base, ng, g, eff, (int*)nullptr));
+  const unsigned bits = peak_bits<1>(d, base, ng, g, eff, (int*)nullptr);
+  pbits[blockIdx.x * 256u + threadIdx.x] = (unsigned char)bits;
+  int cnt = __popc(bits);
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cnt;
   __syncthreads();
@@ -124,12 +130,18 @@ __global__ __launch_bounds__(256) void k_peak_count(const T* d, PeakGeom g, long
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_peak_write(const T* d, PeakGeom g, const long long* boff,
-                                                    T* key, int* cell, int* flag) {
+                                                    const unsigned char* pbits, T* key, int* cell,
+                                                    int* flag) {
   __shared__ int wcnt[PK_ITEMS][4];
   const unsigned ng = (unsigned)g.n0 * g.sx;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
-  const unsigned bits = peak_bits<2>(d, base, ng, g, (T*)nullptr, flag);
+  const unsigned bits = pbits[blockIdx.x * 256u + threadIdx.x];
+#pragma unroll
+  for (int q = 0; q < PK_ITEMS; q++) {
+    const unsigned c = base + (unsigned)q * 256u;
+    if (c < ng) flag[c] = FLAG_FREE;
+  }
   unsigned long long bal[PK_ITEMS];
 #pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
@@ -753,7 +765,11 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   long long nblk = (ng + PK_TILE - 1) / PK_TILE;
   const PeakGeom pg{n0, n1, n2, (unsigned)((long long)n1 * n2), (unsigned)n2, fast_div(n2),
                     fast_div(n1)};
-  long long* bc = (long long*)ctx_buf(c, BUF_HALO, (size_t)(2 * nblk + 2) * 8);
+  // block counts and offsets, then the peak flag bytes
+  const size_t o_pb = ((size_t)(2 * nblk + 2) * 8 + 255) & ~(size_t)255;
+  char* hb = (char*)ctx_buf(c, BUF_HALO, o_pb + (size_t)nblk * 256);
+  long long* bc = (long long*)hb;
+  unsigned char* pbits = (unsigned char*)(hb + o_pb);
   int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
   T* eff = (T*)ctx_buf(c, BUF_EFF, (size_t)ng * sizeof(T));
   unsigned long long* claim = (unsigned long long*)ctx_buf(c, BUF_CLAIM, (size_t)ng * 8);
@@ -772,7 +788,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   {
     TimedScope ts(c, "peaks");
     hipLaunchKernelGGL(k_peak_count<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, pg,
-                       bc, eff);
+                       bc, eff, pbits);
     EXH_LAUNCH_CHECK();
     exclusive_scan<long long>(c, bc, bc + nblk + 1, nblk);
   }
@@ -785,7 +801,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   {
     TimedScope ts(c, "peaks_write");
     hipLaunchKernelGGL(k_peak_write<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, pg,
-                       bc + nblk + 1, pkey, pcell, flag);
+                       bc + nblk + 1, pbits, pkey, pcell, flag);
     EXH_LAUNCH_CHECK();
   }
   // 2. sort (densities in lexicographic peak order; each peak's cell rides
