@@ -878,13 +878,16 @@ __device__ inline T window_r(const PowerArgs& a, int fi, int fj, int fk) {
   return (T)exp((double)(-kmod * R * R) / 2.0);
 }
 
-template <typename T>
+// ONE: one tracer and the monopole only (two values per mode: P, |k|), with
+// the counts fixed at compile time so the per-tracer arrays stay in
+// registers; the arithmetic is the general path's.
+template <typename T, bool ONE>
 __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, double* part,
                              unsigned* pcnt) {
   extern __shared__ double lds[];
   using C2 = typename cplx<T>::type;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int NV = a.NV, Nk = a.Nk;
+  const int NV = ONE ? 2 : a.NV, Nk = a.Nk, LS = ONE ? 1 : a.ls;
   double* acc = lds + (size_t)wave * Nk * NV;
   unsigned* cacc = (unsigned*)(lds + (size_t)nw * Nk * NV) + (size_t)wave * Nk;
   for (int q = lane; q < Nk * NV; q += 64) acc[q] = 0.0;
@@ -964,7 +967,7 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
       const bool tail = valid && (lane == 63 || bnext != b);
       // combined modes of every tracer
       double dr[8], di[8];
-      const int nt = a.ntype;
+      const int nt = ONE ? 1 : a.ntype;
       if (valid) {
         const double2 e1 = a.ex[i], e2 = a.ex[j], e3 = a.ez[k];
         const double wk = a.rwin ? a.Normk / (double)window_r<T>(a, fi, fj, (2 * k < nd) ? k : k - nd)
@@ -991,7 +994,7 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
           if (v == NV - 1) {
             x = kmod;
           } else {
-            int pair = v / a.ls, l2 = 2 * (v % a.ls);
+            int pair = v / LS, l2 = 2 * (v % LS);
             // pair index -> (l, m <= l) in the reference's order
             int l = 0;
             while ((l + 1) * (l + 2) / 2 <= pair) l++;
@@ -1180,8 +1183,12 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
   unsigned long long* cnt = (unsigned long long*)(pbuf + o_cnt);
   {
     TimedScope ts(c, "power_bins");
-    hipLaunchKernelGGL(k_power_bins<T>, dim3(nblocks), dim3(64 * waves), per_wave * waves,
-                       c->stream, a, gk, part, pcnt);
+    if (NV == 2)
+      hipLaunchKernelGGL((k_power_bins<T, true>), dim3(nblocks), dim3(64 * waves),
+                         per_wave * waves, c->stream, a, gk, part, pcnt);
+    else
+      hipLaunchKernelGGL((k_power_bins<T, false>), dim3(nblocks), dim3(64 * waves),
+                         per_wave * waves, c->stream, a, gk, part, pcnt);
     EXH_LAUNCH_CHECK();
   }
   {
