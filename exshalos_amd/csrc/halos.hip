@@ -181,8 +181,9 @@ __global__ void k_cand_setup(const T* skey, const int* scell, long long np, long
 
 // Block counts of the non-negative entries of v (PK_TILE entries a block),
 // for the in-order compaction of the retry list.
-__global__ void k_nonneg_count(const int* v, long long n, long long* bcnt) {
+__global__ void k_nonneg_count(const int* v, const long long* dn, long long* bcnt) {
   __shared__ long long sh[4];
+  const long long n = *dn;
   long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
   long long cnt = 0;
   for (int q = 0; q < PK_ITEMS; q++)
@@ -198,6 +199,8 @@ __global__ void k_nonneg_count(const int* v, long long n, long long* bcnt) {
 // wave).  A counting sort: per rank block, a histogram over bricks; one
 // exclusive scan over (brick, block); then each block scatters its ranks.
 static constexpr int ORD_RB = 4096;
+static constexpr int RT_RING = 16;   // device ring of per-round pending counts
+static constexpr int RT_BATCH = 4;   // rounds per host read after the first
 static constexpr int ORD_MAXB = 8192;  // bricks (LDS ints)
 
 __device__ inline int brick_of(int cell, int n1, int n2, int bsh, int nby) {
@@ -242,10 +245,12 @@ __global__ void k_brick_scatter(const int* ccell, const T* cden, long long nc, i
 // In-order compaction of the candidates to retry (retry[t] >= 0), carrying
 // their cell and density along.
 template <typename T>
-__global__ void k_retry_write(const int* retry, long long n, const long long* boff,
-                              const int* cell_in, const T* den_in, int* rank_out, int* cell_out,
-                              T* den_out) {
+__global__ void k_retry_write(const int* retry, const long long* dn, const long long* boff,
+                              long long nbk, const int* cell_in, const T* den_in, int* rank_out,
+                              int* cell_out, T* den_out, long long* dnext) {
   __shared__ long long sh[4];
+  const long long n = *dn;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dnext = boff[nbk];  // next round's count
   long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
   long long cnt = 0;
   for (int q = 0; q < PK_ITEMS; q++)
@@ -337,15 +342,20 @@ __device__ inline void decode_cell(int cell, int n1, int n2, int& i, int& j, int
 // Workgroup w runs on XCD w mod 8: give each XCD a contiguous range of
 // blocks, so that a round's consecutive candidates (neighbours in space)
 // share one L2.
-__device__ inline int xcd_block() {
-  const unsigned w = blockIdx.x, nb = gridDim.x;
+__device__ inline int xcd_block(unsigned nb) {
+  const unsigned w = blockIdx.x;
   const unsigned q = nb / 8, rem = nb % 8, x = w % 8;
   return (int)(x * q + (x < rem ? x : rem) + w / 8);
 }
 
+// The round's count is read on the device (*dn): the grid may be larger
+// (sized by an earlier round's count) and its extra blocks return.
 template <typename T>
-__global__ void k_grow(GrowArgs<T> a, int npend, unsigned long long tag_hi) {
-  int t = xcd_block() * blockDim.x + threadIdx.x;
+__global__ void k_grow(GrowArgs<T> a, const long long* dn, unsigned long long tag_hi) {
+  const int npend = (int)*dn;
+  const unsigned nba = (unsigned)((npend + 255) / 256);
+  if (blockIdx.x >= nba) return;
+  int t = xcd_block(nba) * blockDim.x + threadIdx.x;
   if (t >= npend) return;
   const int rank = a.prank[t];
   const int cell = a.pcell[t];
@@ -432,9 +442,12 @@ __global__ void k_grow(GrowArgs<T> a, int npend, unsigned long long tag_hi) {
 // retry[t]: the rank again when candidate t must retry next round, else -1
 // (compacted in order, so the next round keeps this round's order).
 template <typename T>
-__global__ void k_commit(GrowArgs<T> a, int npend, unsigned long long tag_hi, int* retry,
-                         int tmax) {
-  int t = xcd_block() * blockDim.x + threadIdx.x;
+__global__ void k_commit(GrowArgs<T> a, const long long* dn, unsigned long long tag_hi,
+                         int* retry, int tmax) {
+  const int npend = (int)*dn;
+  const unsigned nba = (unsigned)((npend + 255) / 256);
+  if (blockIdx.x >= nba) return;
+  int t = xcd_block(nba) * blockDim.x + threadIdx.x;
   if (t >= npend) return;
   const int rank = a.prank[t];
   const int G = a.pG[t];
@@ -847,7 +860,8 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
          o_bk = ctake(((size_t)nbr * nblk_r + 1) * 4), o_rf = ctake(ncs * 4),
          o_pb2 = ctake((size_t)(2 * std::max(nblk_c, 1LL) + 2) * 8), o_pc0 = ctake(ncs * 4),
          o_pc1 = ctake(ncs * 4), o_pd0 = ctake(ncs * sizeof(T)), o_pd1 = ctake(ncs * sizeof(T)),
-         o_pg = ctake(ncs * 4), o_pn = ctake(ncs * 4), o_pr = ctake(ncs * 4);
+         o_pg = ctake(ncs * 4), o_pn = ctake(ncs * 4), o_pr = ctake(ncs * 4),
+         o_rt = ctake(RT_RING * 8);
   char* cand = (char*)ctx_buf(c, BUF_CAND, coff);
   int* ccell = (int*)(cand + o_cc);
   T* cden = (T*)(cand + o_cd);
@@ -903,33 +917,52 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     EXH_LAUNCH_CHECK();
   }
   if (phase) phase(2);
-  long long npend = nc;
-  int round = 0;
+  // Rounds run in batches without a host read between them: each round's
+  // pending count stays on the device (a ring of RT_RING counts), grids are
+  // sized by the last count the host read (counts only shrink), and the
+  // host reads the counts and the error flag once per batch.  A round with
+  // no pending candidate launches blocks that return at once.
+  long long* rt = (long long*)(cand + o_rt);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, (unsigned long long*)rt, 1,
+                     (unsigned long long)nc);
+  EXH_LAUNCH_CHECK();
+  long long ub = nc;
+  int round = 0, launched = 0;
   {
     TimedScope ts(c, "halo_rounds");
-    while (npend > 0) {
-      round++;
-      const unsigned long long tag_hi = (unsigned long long)(++c->claim_epoch) << 32;
-      ga.prank = pend;
-      ga.pcell = qcell;
-      ga.pden = qden;
-      hipLaunchKernelGGL(k_grow<T>, dim3(grid_for(npend, 256)), dim3(256), 0, c->stream, ga,
-                         (int)npend, tag_hi);
-      EXH_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_commit<T>, dim3(grid_for(npend, 256)), dim3(256), 0, c->stream, ga,
-                         (int)npend, tag_hi, retry, tmax);
-      EXH_LAUNCH_CHECK();
-      // candidates to retry, in this round's order
-      const long long nbk = (npend + PK_TILE - 1) / PK_TILE;
-      hipLaunchKernelGGL(k_nonneg_count, dim3((unsigned)nbk), dim3(256), 0, c->stream, retry,
-                         npend, pb2);
-      EXH_LAUNCH_CHECK();
-      exclusive_scan<long long>(c, pb2, pb2 + nbk + 1, nbk);
-      hipLaunchKernelGGL(k_retry_write<T>, dim3((unsigned)nbk), dim3(256), 0, c->stream, retry,
-                         npend, pb2 + nbk + 1, qcell, qden, pnext, qcell_next, qden_next);
-      EXH_LAUNCH_CHECK();
-      EXH_CHECK_HIP(hipMemcpyAsync(&((long long*)hbuf)[2], pb2 + nbk + 1 + nbk, 8,
-                                   hipMemcpyDeviceToHost, c->stream));
+    while (ub > 0) {
+      const int batch = launched == 0 ? 1 : RT_BATCH;
+      const int first = launched;
+      for (int q = 0; q < batch; q++) {
+        const long long* dn = rt + launched % RT_RING;
+        long long* dnext = rt + (launched + 1) % RT_RING;
+        launched++;
+        const unsigned long long tag_hi = (unsigned long long)(++c->claim_epoch) << 32;
+        ga.prank = pend;
+        ga.pcell = qcell;
+        ga.pden = qden;
+        hipLaunchKernelGGL(k_grow<T>, dim3(grid_for(ub, 256)), dim3(256), 0, c->stream, ga, dn,
+                           tag_hi);
+        EXH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_commit<T>, dim3(grid_for(ub, 256)), dim3(256), 0, c->stream, ga, dn,
+                           tag_hi, retry, tmax);
+        EXH_LAUNCH_CHECK();
+        // candidates to retry, in this round's order
+        const long long nbk = (ub + PK_TILE - 1) / PK_TILE;
+        hipLaunchKernelGGL(k_nonneg_count, dim3((unsigned)nbk), dim3(256), 0, c->stream, retry,
+                           dn, pb2);
+        EXH_LAUNCH_CHECK();
+        exclusive_scan<long long>(c, pb2, pb2 + nbk + 1, nbk);
+        hipLaunchKernelGGL(k_retry_write<T>, dim3((unsigned)nbk), dim3(256), 0, c->stream, retry,
+                           dn, pb2 + nbk + 1, nbk, qcell, qden, pnext, qcell_next, qden_next,
+                           dnext);
+        EXH_LAUNCH_CHECK();
+        std::swap(pend, pnext);
+        std::swap(qcell, qcell_next);
+        std::swap(qden, qden_next);
+      }
+      long long* hr = (long long*)hbuf + 4;  // the ring after the error word
+      EXH_CHECK_HIP(hipMemcpyAsync(hr, rt, RT_RING * 8, hipMemcpyDeviceToHost, c->stream));
       EXH_CHECK_HIP(hipMemcpyAsync((int*)hbuf + 1, derr, 4, hipMemcpyDeviceToHost, c->stream));
       EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
       int e = ((int*)hbuf)[1];
@@ -937,11 +970,10 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
                   "halo growth left the supported range (code %d): radius^2 beyond %d, count "
                   "beyond the barrier table, or a sphere wider than half the box",
                   e, stab - 1);
-      npend = hbuf[2];
-      std::swap(pend, pnext);
-      std::swap(qcell, qcell_next);
-      std::swap(qden, qden_next);
-      EXH_REQUIRE(round < 1000000, EXH_ERR_INTERNAL, "halo rounds did not converge");
+      for (int r = first; r < launched; r++)
+        if (hr[r % RT_RING] > 0) round++;
+      ub = hr[launched % RT_RING];
+      EXH_REQUIRE(launched < 1000000, EXH_ERR_INTERNAL, "halo rounds did not converge");
     }
   }
   c->stat_rounds = round;
