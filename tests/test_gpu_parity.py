@@ -164,14 +164,15 @@ def test_grid_compute(prec, window, il, nd):
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("window", [1, 2, 5])
 @pytest.mark.parametrize("il", [0, 1])
-@pytest.mark.parametrize("nL,nd", [(32, 32), (32, 64), (64, 64), (64, 128)])
+@pytest.mark.parametrize("nL,nd", [(32, 32), (32, 64), (64, 64), (64, 128), (64, 32)])
 def test_grid_compute_lattice_tiles(prec, window, il, nL, nd):
     """Particles in Lagrangian lattice order (np = nL^3, as LPT writes
     them) take the LDS-tiled deposits.  (32, 32) and (64, 64): fixed
-    windows at each tile's mean displacement; the antipodal outlier in the
-    first tile (also the tile's sampled particle) leaves its window and
-    deposits directly.  (32, 64) and (64, 128): 16-particle tiles span 32
-    cells a side, the window exceeds the LDS and extent-sized boxes take
+    windows of 24 cells at each tile's mean displacement; the antipodal
+    outlier in the first tile (also the tile's sampled particle) leaves its
+    window and deposits directly.  (64, 32): 8-cell tiles in 16-cell
+    windows.  (32, 64) and (64, 128): 16-particle tiles span 32 cells a
+    side, the window exceeds the LDS and extent-sized boxes take
     over (boxes that do not fit deposit directly).  Weighted and
     redshift-space variants run through the same tiles."""
     set_precision(prec)
