@@ -31,6 +31,7 @@ namespace exh {
 namespace {
 
 constexpr int FFT_THREADS = 256;
+constexpr int AXIS_THREADS = 256;  // threads of a strided-axis workgroup
 
 template <typename T>
 struct FftCfg;
@@ -193,8 +194,8 @@ template <typename C2, int N, int R, int NS, int BZ, int LD, bool INV, bool FROM
 __device__ __forceinline__ void fft_stage(const C2* gs, C2* gd, long long stride, bool ok,
                                           C2* __restrict__ lds, const C2* __restrict__ tw,
                                           int c, int grp, const Op& op) {
-  constexpr int G = FFT_THREADS / BZ;  // threads per column
-  constexpr int NBF = N / R;           // butterflies per column
+  constexpr int G = AXIS_THREADS / BZ;  // threads per column
+  constexpr int NBF = N / R;            // butterflies per column
   constexpr int PER = (NBF + G - 1) / G;
   C2 v[PER][R];
 #pragma unroll
@@ -265,7 +266,7 @@ constexpr size_t fft_lds_bytes() {
 // (line, column block) pairs so that neighbouring blocks, which share
 // 128-byte lines when a row is not line-aligned, meet in one L2.
 template <typename T, int N, bool INV, typename Op>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(const typename cplx<T>::type* src,
+__global__ __launch_bounds__(AXIS_THREADS) void k_fft_axis(const typename cplx<T>::type* src,
                                                           typename cplx<T>::type* dst,
                                                           long long stride, long long ncol,
                                                           long long line_stride, int nbx,
@@ -285,7 +286,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(const typename cplx<T>
   const long long line = logical / nbx;
   const long long xb = logical - line * nbx;
   const int t = threadIdx.x, c = t % BZ, grp = t / BZ;
-  for (int i = t; i < N; i += FFT_THREADS) tws[i] = tw[i];
+  for (int i = t; i < N; i += AXIS_THREADS) tws[i] = tw[i];
   __syncthreads();
   const long long col = xb * BZ + c;
   const bool ok = col < ncol;
@@ -327,7 +328,7 @@ void launch_axis(exh_ctx* c, const void* src, void* dst, long long stride, long 
   auto kern = k_fft_axis<T, N, INV, Op>;
   EXH_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-  hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(FFT_THREADS), lds, c->stream,
+  hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AXIS_THREADS), lds, c->stream,
                      (const typename cplx<T>::type*)src, (typename cplx<T>::type*)dst, stride,
                      ncol, line_stride, nbx, nblocks, twiddles<T>(c, N), op);
   EXH_LAUNCH_CHECK();
