@@ -500,8 +500,8 @@ static constexpr int WIN_HALO = 4;  // default margin h
 static constexpr int WIN_PPT = TILE_L * TILE_L * TILE_L / WIN_THREADS;
 
 template <typename T, int WIN>
-__global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
-    const T* pos, const T* vel, const T* mass, int nL, int nd, int W, int wh, T L, T Lb,
+__device__ __forceinline__ void deposit_window_tile(
+    int tile, const T* pos, const T* vel, const T* mass, int nL, int nd, int W, int wh, T L, T Lb,
     int direction, T rsd, int nsub, T* grid, unsigned long long* counts) {
   constexpr int NW = WIN == 1 ? 1 : (WIN == 2 ? 2 : 3);
   constexpr int NWAVE = WIN_THREADS / 64;
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
   const T Ld = L / (T)nd;
   const int nt = nL / TILE_L;
   const int E = W + 2 * wh, sx = E * E;
-  const int tz = blockIdx.x % nt, ty = (blockIdx.x / nt) % nt, tx = blockIdx.x / (nt * nt);
+  const int tz = tile % nt, ty = (tile / nt) % nt, tx = tile / (nt * nt);
   // tile centre in grid cells: the middle of the grid tile of the same index
   const int qc[3] = {tx * W + W / 2, ty * W + W / 2, tz * W + W / 2};
   if (threadIdx.x == 0) nvalid = 0;
@@ -638,6 +638,19 @@ __global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
   }
 }
 
+// One workgroup per tile, or (EXH_DEPOSIT_WG) fewer workgroups that walk the
+// tiles, leaving CUs to the kernels of other streams.
+template <typename T, int WIN>
+__global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
+    const T* pos, const T* vel, const T* mass, int nL, int nd, int W, int wh, T L, T Lb,
+    int direction, T rsd, int nsub, T* grid, unsigned long long* counts, int ntiles) {
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    deposit_window_tile<T, WIN>(tile, pos, vel, mass, nL, nd, W, wh, L, Lb, direction, rsd, nsub,
+                                grid, counts);
+    __syncthreads();
+  }
+}
+
 template <typename T>
 __global__ void k_zero(T* g, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -696,6 +709,16 @@ static bool deposit_window_enabled() {
     return !(e && std::atoi(e) == 0);
   }();
   return on;
+}
+
+// Workgroups of the window deposit (EXH_DEPOSIT_WG; 0 = one per tile).
+static int deposit_workgroups() {
+  static const int v = [] {
+    const char* e = std::getenv("EXH_DEPOSIT_WG");
+    const int n = e ? std::atoi(e) : 0;
+    return n < 0 ? 0 : n;
+  }();
+  return v;
 }
 
 // Window margin h in cells (EXH_DEPOSIT_MARGIN, 1..8, default WIN_HALO).
@@ -779,13 +802,15 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
     // single-step wrap is exact
     if (tiles && nd >= E && win_lds <= deposit_lds_bytes() && deposit_window_enabled()) {
       const unsigned nblk = (unsigned)nt * nt * nt;
+      const unsigned nwg = deposit_workgroups() > 0 ? std::min(nblk, (unsigned)deposit_workgroups())
+                                                    : nblk;
 #define EXH_DEPW(WN)                                                                             \
   do {                                                                                           \
     EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_deposit_window<T, WN>,                      \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, win_lds));     \
-    hipLaunchKernelGGL((k_deposit_window<T, WN>), dim3(nblk), dim3(WIN_THREADS), win_lds,        \
+    hipLaunchKernelGGL((k_deposit_window<T, WN>), dim3(nwg), dim3(WIN_THREADS), win_lds,         \
                        c->stream, pos, vel, mass, nL, nd, W, wh, Lf, Lb, direction, rsd, nsub,   \
-                       grid, counts);                                                            \
+                       grid, counts, (int)nblk);                                                 \
   } while (0)
       if (window == 1) EXH_DEPW(1);
       if (window == 2) EXH_DEPW(2);
