@@ -65,11 +65,11 @@ __device__ inline bool is_peak_at(const T* d, unsigned c, T v, int i, int j, int
 }
 
 // Peak flags of the thread's PK_ITEMS cells (cells past the grid: false).
-// The pass also initialises the growth rounds' per-cell state of those
-// cells: SIDE 1 copies delta to eff, SIDE 2 sets the flags free.
-template <int SIDE, typename T>
+// The pass also copies those cells of delta to eff, the growth rounds'
+// effective density.
+template <typename T>
 __device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const PeakGeom& g,
-                                     T* eff, int* flag) {
+                                     T* eff) {
   unsigned r = fdiv(c0, g.dz);
   int k = (int)(c0 - r * (unsigned)g.n2);
   int i = (int)fdiv(r, g.dy);
@@ -97,10 +97,7 @@ __device__ inline unsigned peak_bits(const T* d, unsigned c0, unsigned ng, const
 #pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
     const unsigned c = c0 + (unsigned)q * 256u;
-    if (c < ng) {
-      if (SIDE == 1) eff[c] = cv[q];
-      if (SIDE == 2) flag[c] = FLAG_FREE;
-    }
+    if (c < ng) eff[c] = cv[q];
   }
   return bits;
 }
@@ -119,7 +116,7 @@ __global__ __launch_bounds__(256) void k_peak_count(const T* d, PeakGeom g, long
   __shared__ int sh[4];
   const unsigned ng = (unsigned)g.n0 * g.sx;
   const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
-  const unsigned bits = peak_bits<1>(d, base, ng, g, eff, (int*)nullptr);
+  const unsigned bits = peak_bits(d, base, ng, g, eff);
   pbits[blockIdx.x * 256u + threadIdx.x] = (unsigned char)bits;
   int cnt = __popc(bits);
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
