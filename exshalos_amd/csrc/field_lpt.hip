@@ -35,6 +35,10 @@ struct FieldArgs {
   const T* noise;
   int fixed;
   T phase;
+  // truncated P(k) (R_max < 1e5): the DC mode takes the amplitude at
+  // |k| = (kl0 kl1 kl2)^(1/3) / 4 (density_grid.c:158-161)
+  int dc;
+  T dc_kmod;
 };
 
 // Natural cubic spline evaluated from per-interval coefficients
@@ -96,8 +100,12 @@ __device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& s
   if (k == a.n2 / 2) kz = (T)(-a.n2 / 2.0 * (double)a.kl2);
   T kmod = sqrt_t<T>(kx * kx + ky * ky + kz * kz);
   T A = 0;
-  if (kmod <= a.k_smooth && kmod > (T)0)
-    A = (T)sqrt(spline_eval_v(sp, (double)kmod) / 2.0);
+  if (kmod <= a.k_smooth) {
+    if (kmod > (T)0)
+      A = (T)sqrt(spline_eval_v(sp, (double)kmod) / 2.0);
+    else if (a.dc)
+      A = (T)sqrt(spline_eval_v(sp, (double)a.dc_kmod) / 2.0);
+  }
   size_t id = ((size_t)i * a.n1 + j) * (size_t)a.nz2 + k;
   T g0 = a.noise[2 * id], g1 = a.noise[2 * id + 1];
   bool selfconj = (i == 0 || i == a.n0 / 2) && (j == 0 || j == a.n1 / 2) &&
@@ -567,6 +575,14 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
   a.sd = d + 4 * nk;
   a.nk = nk;
   a.stage = 5 * (size_t)nk * sizeof(double) <= 48 * 1024 ? 1 : 0;
+  if (trunc) {
+    // the reference's roundings: the product of the three T frequencies in
+    // T, pow in double rounded to T, the division by 4.0 in double, stored
+    // as T (density_grid.c:159)
+    const T prod = (T)((T)((T)b->kl[0] * (T)b->kl[1]) * (T)b->kl[2]);
+    a.dc = 1;
+    a.dc_kmod = (T)((double)(T)std::pow((double)prod, 1.0 / 3.0) / 4.0);
+  }
   // the reference aborts (GSL EDOM) when |k| of the grid leaves the table
   double kmax = 0;
   for (int ax = 0; ax < 3; ax++) {
@@ -575,6 +591,7 @@ static FieldArgs<T> make_field_args(exh_ctx* c, const exh_box* b, const exh_fiel
   }
   kmax = std::sqrt(kmax);
   double kmin = (double)std::min({(T)b->kl[0], (T)b->kl[1], (T)b->kl[2]});
+  if (a.dc) kmin = std::min(kmin, (double)a.dc_kmod);
   if (kmin > k_smooth) kmin = 0;
   const double x0 = c->spline_x0, x1 = c->spline_x1;
   EXH_REQUIRE(x0 <= kmin * (1 + 1e-6) && std::min(kmax, k_smooth) <= x1 * (1 - 1e-7) + 1e-300,

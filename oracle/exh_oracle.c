@@ -205,9 +205,12 @@ int FN(ora_spline_eval_many)(const double *x, const double *y, int n,
 /* Output: deltak[nd0][nd1][nz2][2], unnormalised (the reference applies      */
 /* Normx after the c2r transform, density_grid.c:213-216).                    */
 /* ------------------------------------------------------------------------ */
+/* R_max < 1e5 (the truncated-P(k) case of Inter_Power) also gives the DC   */
+/* mode an amplitude: |k| is replaced by (kl0 kl1 kl2)^(1/3) / 4 and the      */
+/* mode is self-conjugate, sqrt(2) A g (density_grid.c:158-161, 169-175).     */
 int FN(ora_field_modes)(const FN(ora_params) * p, const double *Kin, const double *Pin,
                         int Nk, const REAL *noise, REAL k_smooth, int fixed,
-                        REAL phase, REAL *deltak) {
+                        REAL phase, REAL R_max, REAL *deltak) {
   ora_spline sp;
   if (spline_init(&sp, Kin, Pin, Nk)) return -1;
   const int n0 = p->nd[0], n1 = p->nd[1], n2 = p->nd[2], nz2 = p->nz2;
@@ -245,11 +248,18 @@ int FN(ora_field_modes)(const FN(ora_params) * p, const double *Kin, const doubl
           }
         }
         REAL kmod = (REAL)sqrt(kx * kx + ky * ky + kz * kz);
-        REAL A;
-        if (kmod <= k_smooth && kmod > 0.0)
-          A = (REAL)sqrt(spline_eval(&sp, (double)kmod, &err) / 2.0);
-        else
-          A = 0.0;
+        REAL A = 0.0;
+        if (kmod <= k_smooth) {
+          if (kmod > 0.0) {
+            A = (REAL)sqrt(spline_eval(&sp, (double)kmod, &err) / 2.0);
+          } else if (kmod == 0.0 && R_max < 100000.0) {
+            /* the product of the three REAL frequencies is rounded to REAL,
+             * pow runs in double, its result is rounded to REAL before the
+             * division by 4.0 (in double), then stored as REAL */
+            kmod = (REAL)pow(p->kl[0] * p->kl[1] * p->kl[2], 1.0 / 3.0) / 4.0;
+            A = (REAL)sqrt(spline_eval(&sp, (double)kmod, &err) / 2.0);
+          }
+        }
         const REAL *g = noise + 2 * id;
         int selfconj = (i == 0 || i == n0 / 2) && (j == 0 || j == n1 / 2) &&
                        (k == 0 || k == n2 / 2);
@@ -757,6 +767,21 @@ void FN(ora_lpt2_disp)(const FN(ora_params) * p, const REAL *phi2, REAL *S2) {
  * final halo position (lpt.c:245-255): posh = wrap(x Lc + Lc/2 + sum/count).
  * S1 and S2 (S2 may be NULL) are full displacement fields.  Velocities
  * follow with the growth-rate factors of lpt.c:76-78 and 206-208. */
+/* Compute_Posh (lpt.c:245-255): posh = wrap(x Lc + Lc/2 + posh / count),
+ * velh /= count. */
+static void halo_means(const FN(ora_params) * p, long long nh, const int *hx, const int *hcount,
+                       REAL *posh, REAL *velh) {
+  for (long long h = 0; h < nh; h++)
+    for (int a = 0; a < 3; a++) {
+      REAL base = hx[3 * h + a] * p->Lc + p->Lc / 2.0;
+      REAL r = base + posh[3 * h + a] / hcount[h];
+      if (r >= p->L[a]) r -= p->L[a];
+      if (r < 0) r += p->L[a];
+      posh[3 * h + a] = r;
+      if (velh) velh[3 * h + a] = velh[3 * h + a] / hcount[h];
+    }
+}
+
 void FN(ora_halo_disp)(const FN(ora_params) * p, const long long *flag, const REAL *S1,
                        const REAL *S2, long long nh, const int *hx, const int *hcount,
                        REAL *posh, REAL *velh) {
@@ -782,15 +807,29 @@ void FN(ora_halo_disp)(const FN(ora_params) * p, const long long *flag, const RE
       }
     }
   }
-  for (long long h = 0; h < nh; h++)
+  halo_means(p, nh, hx, hcount, posh, velh);
+}
+
+/* Halo sums of given displacements and velocities (the IN_disp branch of
+ * Generate_Halos_Box_from_Grid, box.c:96-110): posh += S and velh += V over
+ * the cells with |flag| < ng in cell order, then Compute_Posh. */
+void FN(ora_halo_disp_given)(const FN(ora_params) * p, const long long *flag, const REAL *S,
+                             const REAL *V, long long nh, const int *hx, const int *hcount,
+                             REAL *posh, REAL *velh) {
+  const long long ng = p->ng;
+  for (long long h = 0; h < 3 * nh; h++) {
+    posh[h] = 0.0;
+    if (velh) velh[h] = 0.0;
+  }
+  for (long long c = 0; c < ng; c++) {
+    long long h = flag[c] < 0 ? -flag[c] : flag[c];
+    if (h >= ng) continue;
     for (int a = 0; a < 3; a++) {
-      REAL base = hx[3 * h + a] * p->Lc + p->Lc / 2.0;
-      REAL r = base + posh[3 * h + a] / hcount[h];
-      if (r >= p->L[a]) r -= p->L[a];
-      if (r < 0) r += p->L[a];
-      posh[3 * h + a] = r;
-      if (velh) velh[3 * h + a] = velh[3 * h + a] / hcount[h];
+      posh[3 * h + a] += S[3 * c + a];
+      if (velh) velh[3 * h + a] += V[3 * c + a];
     }
+  }
+  halo_means(p, nh, hx, hcount, posh, velh);
 }
 
 /* Particle positions q + S with one periodic wrap (lpt.c:229-242). */

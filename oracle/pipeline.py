@@ -190,7 +190,8 @@ def field_modes(p, k, P, noise, precision=8, k_smooth=1e5, fixed=False, phase=0.
     deltak = np.zeros_like(noise)
     rc = getattr(lib(), "ora_field_modes" + _sfx(precision))(
         C.byref(p), _ptr(k64), _ptr(P64), C.c_int(len(k64)), _ptr(noise),
-        _rc(precision)(k_smooth), C.c_int(int(fixed)), _rc(precision)(phase), _ptr(deltak))
+        _rc(precision)(k_smooth), C.c_int(int(fixed)), _rc(precision)(phase),
+        _rc(precision)(R(R_max)), _ptr(deltak))
     if rc < 0:
         raise RuntimeError("oracle field_modes failed")
     if rc == 1:
@@ -198,17 +199,35 @@ def field_modes(p, k, P, noise, precision=8, k_smooth=1e5, fixed=False, phase=0.
     return deltak
 
 
+def _fft_workers() -> int:
+    """Threads of the oracle's FFTs (pocketfft through scipy.fft): at most 16,
+    the CPU share of one GPU box."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def c2r(ck: np.ndarray, nd) -> np.ndarray:
-    """Unnormalised complex-to-real 3D transform (FFTW c2r convention)."""
+    """Unnormalised complex-to-real 3D transform (FFTW c2r convention), in
+    float64."""
+    import scipy.fft as sf
+
     z = ck[..., 0].astype(np.float64) + 1j * ck[..., 1].astype(np.float64)
     n = int(np.prod(nd))
-    return np.fft.irfftn(z, s=tuple(nd), axes=(0, 1, 2)) * n
+    out = sf.irfftn(z, s=tuple(nd), axes=(0, 1, 2), workers=_fft_workers())
+    del z
+    out *= n
+    return out
 
 
 def r2c(x: np.ndarray) -> np.ndarray:
-    """Unnormalised real-to-complex 3D transform, returned as [..., 2]."""
-    z = np.fft.rfftn(x.astype(np.float64), axes=(0, 1, 2))
-    return np.stack([z.real, z.imag], axis=-1)
+    """Unnormalised real-to-complex 3D transform in float64, returned as
+    [..., 2]."""
+    import scipy.fft as sf
+
+    z = sf.rfftn(x.astype(np.float64), axes=(0, 1, 2), workers=_fft_workers())
+    out = np.empty(z.shape + (2,), dtype=np.float64)
+    out[..., 0] = z.real
+    out[..., 1] = z.imag
+    return out
 
 
 def field_from_modes(k, P, nd, Lc, noise, Om0=0.31, z=0.0, precision=8, k_smooth=1e5,
@@ -373,6 +392,62 @@ def halo_positions(p, flag, S1, S2, halos, precision=8, with_vel=False):
         _ptr(np.ascontiguousarray(halos["x"], dtype=np.int32)),
         _ptr(np.ascontiguousarray(halos["count"], dtype=np.int32)), _ptr(posh), _ptr(velh))
     return posh[:nh], (velh[:nh] if with_vel else None)
+
+
+def particle_velocities(S1, S2, p, precision=8):
+    """Particle velocities of Compute_1LPT / Compute_2LPT (lpt.c:82-86,
+    210-216): V = pow(Omz, 0.5454) Hz / (1 + z) S1 evaluated in double and
+    rounded to the float type, then V += 2 pow(Omz, 6/11) Hz / (1 + z) S2."""
+    R = _real(precision)
+    f64 = np.float64
+    Omz, Hz, z = f64(R(p.Omz)), f64(R(p.Hz)), f64(R(p.redshift))
+    V = (Omz**0.5454 * Hz / (1.0 + z) * np.asarray(S1, dtype=R).astype(f64)).astype(R)
+    if S2 is not None:
+        V = (V.astype(f64) + 2.0 * Omz ** (6.0 / 11.0) * Hz / (1.0 + z) *
+             np.asarray(S2, dtype=R).astype(f64)).astype(R)
+    return V
+
+
+def halo_positions_given(p, flag, S, V, halos, precision=8):
+    """Halo positions (and velocities when V is given) from given
+    displacements: the IN_disp branch of Generate_Halos_Box_from_Grid
+    (box.c:96-110) and Compute_Posh (lpt.c:245-255)."""
+    R = _real(precision)
+    nh = len(halos["count"])
+    posh = np.zeros((max(nh, 1), 3), dtype=R)
+    velh = np.zeros((max(nh, 1), 3), dtype=R) if V is not None else None
+    getattr(lib(), "ora_halo_disp_given" + _sfx(precision))(
+        C.byref(p), _ptr(np.ascontiguousarray(flag.reshape(-1), dtype=np.int64)),
+        _ptr(np.ascontiguousarray(S, dtype=R)),
+        _ptr(None if V is None else np.ascontiguousarray(V, dtype=R)), C.c_longlong(nh),
+        _ptr(np.ascontiguousarray(halos["x"], dtype=np.int32)),
+        _ptr(np.ascontiguousarray(halos["count"], dtype=np.int32)), _ptr(posh), _ptr(velh))
+    return posh[:nh], (velh[:nh] if V is not None else None)
+
+
+def halos_box_from_grid(delta, k, P, Lc, S=None, V=None, Om0=0.31, z=0.0, k_smooth=1e4,
+                        dc=-1.0, Nmin=1, a=1.0, beta=0.0, alpha=0.0, do_2lpt=False,
+                        precision=8):
+    """Generate_Halos_Box_from_Grid (box.c:71-150) with the mass-draw seed
+    1234 (exshalos.c:987).  With S given (IN_disp) the halos take the means
+    of S (and V) over their cells; otherwise the LPT of the grid
+    (Compute_Denk, then Compute_1LPT/2LPT) gives the displacements.
+    Returns posh, velh (V given, or the LPT velocities), the halo dict, and
+    S1/S2 when the LPT ran."""
+    R = _real(precision)
+    delta = np.ascontiguousarray(delta, dtype=R)
+    nd = delta.shape
+    p = params(nd, Lc, Om0, z, dc, precision)
+    hal = find_halos(delta, k, P, Lc, Om0, z, dc, Nmin, a, beta, alpha, 1234, precision, p)
+    out = dict(halos=hal, params=p, Mh=hal["Mh"])
+    if S is not None:
+        out["posh"], out["velh"] = halo_positions_given(p, hal["flag"], S, V, hal, precision)
+        return out
+    dk = (r2c(delta) * R(p.Normk)).astype(R)
+    S1, S2 = lpt(dk, p, k_smooth, do_2lpt, precision)
+    out["posh"], out["velh"] = halo_positions(p, hal["flag"], S1, S2, hal, precision, True)
+    out["S1"], out["S2"] = S1, S2
+    return out
 
 
 def halos_box_from_pk(k, P, nd, Lc, noise, Om0=0.31, z=0.0, k_smooth=1e4, dc=-1.0,

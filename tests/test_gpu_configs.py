@@ -1,0 +1,292 @@
+"""Parity at the workload sizes, through the named Python API.
+
+The chain is the one pyexshalos users run: ``mock.Generate_Halos_Box_from_Pk``
+(mock.py:11-184) -> ``simulation.Compute_Density_Grid`` (simulation.py:9-166)
+-> ``simulation.Compute_Power_Spectrum`` (simulation.py:170-311), here on the
+HIP library, against the CPU oracle fed the same white-noise modes.
+
+Workloads:
+
+* nd=64, L=256 Mpc/h, 1LPT, default barrier, seed 1; CIC of the halo
+  positions on a 64^3 grid; P(k) in 32 linear bins.
+* nd=256, L=1024 Mpc/h, 2LPT; CIC of the 256^3 displaced particles on a
+  256^3 grid; P(k) in 128 bins.  FP64: halos bit-exact on the GPU's grid,
+  P(k) within 1e-5 with exact mode counts.  FP32: P(k) within 1e-4 of the
+  FP64 oracle.
+* nd=512, L=2048 Mpc/h, 2LPT, FP32 fields with an FP64 bin index; interlaced
+  CIC of 512^3 particles, 256 bins: halos bit-exact against the oracle's
+  finder on the GPU's grid, catalogue invariants, mass conservation of the
+  grid, and P(k) within 1e-4 of the FP64 oracle's.
+
+Halos are compared bit for bit with the oracle's finder run on the density
+grid the GPU produced (the grids agree to FFT rounding, and the finder is
+exact on a given grid).  Float outputs carry the tolerance written at each
+comparison.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from exshalos_amd import mock as M
+from exshalos_amd import simulation as SIM
+from exshalos_amd import synth
+from exshalos_amd.lib import exshalos as X
+from exshalos_amd.lib import set_precision
+from oracle import pipeline as O
+
+pytestmark = pytest.mark.gpu
+
+LC = 4.0
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _noise(nd, seed, prec):
+    return synth.white_noise(nd, seed, np.float32 if prec == 4 else np.float64)
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _pdist(a, b, L):
+    d = np.abs(np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64))
+    return float(np.max(np.minimum(d, L - d)))
+
+
+def _pos_tol(prec, L):
+    # FP32: a few units in the last place of coordinates up to L, plus the
+    # float FFT/stencil rounding of displacements of ~10 Mpc/h
+    return 8 * np.spacing(np.float32(L)) + 2e-4 if prec == 4 else 1e-8
+
+
+def _halos_bit_exact(out, ref):
+    cat = X.halo_catalogue()
+    assert len(cat["count"]) == len(ref["count"]) > 0
+    np.testing.assert_array_equal(cat["x"], ref["x"])
+    np.testing.assert_array_equal(cat["count"], ref["count"])
+    np.testing.assert_array_equal(out["Mh"], ref["Mh"])
+    if "flag" in out:
+        np.testing.assert_array_equal(out["flag"].reshape(-1), ref["flag"].reshape(-1))
+    return cat
+
+
+# ------------------------------------------------------------------ nd = 64
+@pytest.mark.parametrize("prec", [8, 4])
+def test_nd64_halos_cic_power_chain(bbks, prec):
+    """Generate_Halos_Box_from_Pk (1LPT, seed 1) -> CIC of the halo
+    positions -> 32 linear bins, all through the named API."""
+    k, P = bbks
+    set_precision(prec)
+    nd, L = 64, 64 * LC
+    noise = _noise(nd, 1, prec)
+    out = M.Generate_Halos_Box_from_Pk(k, P, nd=nd, Lc=LC, Om0=0.31, z=0.0, k_smooth=1e4,
+                                       seed=1, OUT_DEN=True, OUT_FLAG=True, OUT_VEL=True,
+                                       noise=noise)
+    ref = O.find_halos(out["grid"], k, P, LC, seed=1, precision=prec)
+    _halos_bit_exact(out, ref)
+    p = ref["params"]
+    delta, dk, _ = O.field_from_modes(k, P, nd, LC, noise, precision=prec)
+    assert _rel(out["grid"], delta) < (2e-6 if prec == 4 else 1e-12)
+    S1, _ = O.lpt(dk, p, 1e4, False, prec)
+    posh, velh = O.halo_positions(p, ref["flag"], S1, None, ref, precision=prec,
+                                  with_vel=True)
+    assert _pdist(out["posh"], posh, L) < _pos_tol(prec, L)
+    assert _rel(out["velh"], velh) < (1e-4 if prec == 4 else 1e-9)
+    # the halo positions differ by the rounding of their displacement sums
+    # (double on the GPU, float in cell order in the reference), hence the
+    # FP32 bars of the halo grid and its spectrum
+    g = SIM.Compute_Density_Grid(out["posh"], nd=nd, L=L, window="CIC")
+    gr = O.grid_compute(posh, nd, L, window=2, precision=prec)[0, 0]
+    assert g.shape == (nd, nd, nd)
+    assert _rel(g, gr) < (3e-4 if prec == 4 else 1e-9)
+    pk = SIM.Compute_Power_Spectrum(g, L=L, window="CIC", Nk=32)
+    pr = O.power_compute(gr, 1, nd, L, window=2, Nk=32, precision=prec)
+    np.testing.assert_array_equal(pk["Nk"], pr["Nk"])
+    assert _rel(pk["Pk"], pr["Pk64"][0, 0]) < (3e-4 if prec == 4 else 1e-5)
+    assert _rel(pk["k"], pr["k64"]) < (1e-6 if prec == 4 else 1e-12)
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("given", [False, True])
+def test_nd64_halos_box_from_grid(bbks, prec, given):
+    """Generate_Halos_Box_from_Grid (box.c:71-150, mass-draw seed 1234):
+    with given displacements and velocities the halos take their means over
+    the member cells (IN_disp); without them the LPT of the grid does
+    (Compute_Denk, then 2LPT here)."""
+    k, P = bbks
+    set_precision(prec)
+    R = np.float32 if prec == 4 else np.float64
+    nd, L = 64, 64 * LC
+    delta, dk, p = O.field_from_modes(k, P, nd, LC, _noise(nd, 6, prec), precision=prec)
+    if given:
+        rng = np.random.default_rng(3)
+        S = rng.normal(0, 3.0, (nd**3, 3)).astype(R)
+        V = rng.normal(0, 300.0, (nd**3, 3)).astype(R)
+        out = M.Generate_Halos_Box_from_Grid(delta, k, P, S=S, V=V, Lc=LC, OUT_VEL=True,
+                                             OUT_FLAG=True)
+        ref = O.halos_box_from_grid(delta, k, P, LC, S=S, V=V, precision=prec)
+    else:
+        out = M.Generate_Halos_Box_from_Grid(delta, k, P, Lc=LC, OUT_VEL=True, OUT_FLAG=True,
+                                             DO_2LPT=True, OUT_LPT=False)
+        ref = O.halos_box_from_grid(delta, k, P, LC, do_2lpt=True, precision=prec)
+    _halos_bit_exact(out, ref["halos"])
+    if given:  # plain means of the given arrays
+        assert _pdist(out["posh"], ref["posh"], L) < _pos_tol(prec, L)
+        assert _rel(out["velh"], ref["velh"]) < (2e-5 if prec == 4 else 1e-9)
+    else:  # float r2c of the real grid (Compute_Denk) before the LPT
+        assert _pdist(out["posh"], ref["posh"], L) < (1e-2 if prec == 4 else 1e-8)
+        assert _rel(out["velh"], ref["velh"]) < (1e-3 if prec == 4 else 1e-9)
+
+
+# ----------------------------------------------------------------- nd = 256
+@pytest.fixture(scope="module")
+def oracle_nd256(bbks):
+    """The FP64 oracle's 2LPT particles, their CIC grid and 128-bin P(k) at
+    nd=256, L=1024 from seed-1 modes."""
+    k, P = bbks
+    nd, L = 256, 256 * LC
+    noise = _noise(nd, 1, 8)
+    _say("oracle nd=256: field")
+    delta, dk, p = O.field_from_modes(k, P, nd, LC, noise, precision=8)
+    _say("oracle nd=256: 2LPT")
+    S1, S2 = O.lpt(dk, p, 1e4, True, 8)
+    pos = O.positions(S1 + S2, p, 8)
+    vel = O.particle_velocities(S1, S2, p, 8)
+    _say("oracle nd=256: CIC")
+    g = O.grid_compute(pos, nd, L, window=2, precision=8)[0, 0]
+    _say("oracle nd=256: P(k)")
+    pk = O.power_compute(g, 1, nd, L, window=2, Nk=128, precision=8)
+    return dict(noise=noise, delta=delta, dk=dk, p=p, S1=S1, S2=S2, pos=pos, vel=vel, grid=g,
+                pk=pk)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("prec", [8, 4])
+def test_nd256_2lpt_halos_cic_power(bbks, oracle_nd256, prec):
+    """nd=256, L=1024, 2LPT through the named API.  Halos, flags and masses
+    bit-exact against the oracle's finder on the GPU's grid; halo and
+    particle positions and velocities against the oracle's 2LPT; CIC grid
+    and 128-bin P(k) with exact mode counts, within 1e-5 of the FP64
+    oracle in FP64 and within 1e-4 in FP32."""
+    k, P = bbks
+    o = oracle_nd256
+    set_precision(prec)
+    R = np.float32 if prec == 4 else np.float64
+    nd, L = 256, 256 * LC
+    _say(f"nd=256 prec={prec}: GPU pipeline")
+    out = M.Generate_Halos_Box_from_Pk(k, P, nd=nd, Lc=LC, Om0=0.31, z=0.0, k_smooth=1e4,
+                                       seed=1, OUT_DEN=True, OUT_LPT=True, OUT_VEL=True,
+                                       DO_2LPT=True, OUT_FLAG=True, noise=o["noise"].astype(R))
+    _say(f"nd=256 prec={prec}: oracle halos on the GPU's grid")
+    ref = O.find_halos(out["grid"], k, P, LC, seed=1, precision=prec)
+    _halos_bit_exact(out, ref)
+    assert _rel(out["grid"], o["delta"]) < (2e-6 if prec == 4 else 1e-12)
+    posh, velh = O.halo_positions(o["p"], ref["flag"], o["S1"], o["S2"], ref, precision=8,
+                                  with_vel=True)
+    assert _pdist(out["posh"], posh, L) < _pos_tol(prec, L)
+    assert _rel(out["velh"], velh) < (1e-4 if prec == 4 else 1e-9)
+    assert _pdist(out["pos"], o["pos"], L) < _pos_tol(prec, L)
+    assert _rel(out["vel"], o["vel"]) < (1e-4 if prec == 4 else 1e-9)
+    _say(f"nd=256 prec={prec}: grid and P(k)")
+    g = SIM.Compute_Density_Grid(out["pos"], nd=nd, L=L, window="CIC")
+    assert _rel(g, o["grid"]) < (2e-4 if prec == 4 else 1e-9)
+    pk = SIM.Compute_Power_Spectrum(g, L=L, window="CIC", Nk=128, kbin_precision=8)
+    np.testing.assert_array_equal(pk["Nk"], o["pk"]["Nk"])
+    assert _rel(pk["Pk"], o["pk"]["Pk64"][0, 0]) < (1e-4 if prec == 4 else 1e-5)
+    assert _rel(pk["k"], o["pk"]["k64"]) < (1e-6 if prec == 4 else 1e-12)
+    if prec == 8:
+        # the reference's float bin index as well: the same counts here
+        pk4 = SIM.Compute_Power_Spectrum(g, L=L, window="CIC", Nk=128, kbin_precision=4)
+        pr4 = O.power_compute(o["grid"].astype(np.float32), 1, nd, L, window=2, Nk=128,
+                              precision=4)
+        np.testing.assert_array_equal(pk4["Nk"], pr4["Nk"])
+
+
+# ----------------------------------------------------------------- nd = 512
+def _catalogue_invariants(delta, flag, cat, dc):
+    """Every catalogue's invariants: the members of halo h are the cells
+    flagged h or -h (its centre), their number is its count, the counts are
+    sphere sizes, the mean density over the members reaches the barrier,
+    and free/rejected cells carry ng / ng+1."""
+    ng = delta.size
+    f = flag.reshape(-1)
+    a = np.abs(f)
+    nh = len(cat["count"])
+    assert np.all((a < nh) | (a == ng) | (a == ng + 1))
+    members = np.bincount(a[a < ng], minlength=nh)
+    np.testing.assert_array_equal(members, cat["count"])
+    cidx = (cat["x"][:, 0].astype(np.int64) * delta.shape[1] + cat["x"][:, 1]) * \
+        delta.shape[2] + cat["x"][:, 2]
+    np.testing.assert_array_equal(f[cidx], -np.arange(nh))
+    sph = set(O.cells_in_spheres(20000).tolist())
+    assert set(np.unique(cat["count"]).tolist()) <= sph
+    sums = np.bincount(a[a < ng], weights=delta.reshape(-1)[a < ng].astype(np.float64),
+                       minlength=nh)
+    assert np.all(sums / cat["count"] >= dc * (1 - 1e-6))
+
+
+@pytest.mark.timeout(900)
+def test_nd512_precision_form(bbks):
+    """nd=512, L=2048, 2LPT.  FP32 pipeline through the named API with an
+    FP64 bin index and interlaced CIC of 512^3 particles, 256 bins:
+
+    * halos, flags and masses bit-exact against the oracle's FP32 finder
+      on the GPU's grid, and the catalogue invariants;
+    * the CIC grid conserves mass (mean of rho/rho_bar - 1 is zero);
+    * P(k) within 1e-4 of the FP64 oracle's P(k) of the same modes, mode
+      counts exact; the FP64 GPU pipeline within 1e-5 of it."""
+    k, P = bbks
+    nd, L, Nk = 512, 512 * LC, 256
+    noise = _noise(nd, 1, 8)
+    set_precision(4)
+    _say("nd=512: FP32 GPU pipeline")
+    out = M.Generate_Halos_Box_from_Pk(k, P, nd=nd, Lc=LC, Om0=0.31, z=0.0, k_smooth=1e4,
+                                       seed=1, OUT_DEN=True, OUT_LPT=True, DO_2LPT=True,
+                                       OUT_FLAG=True, noise=noise.astype(np.float32))
+    cat = X.halo_catalogue()
+    g32 = SIM.Compute_Density_Grid(out["pos"], nd=nd, L=L, window="CIC", interlacing=True)
+    assert g32.shape == (2, nd, nd, nd)
+    assert abs(float(g32[0].astype(np.float64).mean())) < 1e-5
+    assert abs(float(g32[1].astype(np.float64).mean())) < 1e-5
+    pk32 = SIM.Compute_Power_Spectrum(g32, L=L, window="CIC", Nk=Nk, kbin_precision=8)
+    del g32
+    _say("nd=512: oracle FP32 halo finder on the GPU's grid")
+    ref = O.find_halos(out["grid"], k, P, LC, seed=1, precision=4)
+    _halos_bit_exact(out, ref)
+    _catalogue_invariants(out["grid"], out["flag"], cat, float(ref["params"].dc))
+    del ref, out
+    set_precision(8)
+    _say("nd=512: FP64 GPU pipeline")
+    out64 = M.Generate_Halos_Box_from_Pk(k, P, nd=nd, Lc=LC, Om0=0.31, z=0.0, k_smooth=1e4,
+                                         seed=1, OUT_LPT=True, DO_2LPT=True, noise=noise)
+    g64 = SIM.Compute_Density_Grid(out64["pos"], nd=nd, L=L, window="CIC", interlacing=True)
+    del out64
+    pk64 = SIM.Compute_Power_Spectrum(g64, L=L, window="CIC", Nk=Nk)
+    del g64
+    _say("nd=512: FP64 oracle field and 2LPT")
+    p = O.params(nd, LC, precision=8)
+    dk = O.field_modes(p, k, P, noise, 8)
+    S1, S2 = O.lpt(dk, p, 1e4, True, 8)
+    del dk
+    S1 += S2
+    del S2
+    pos = O.positions(S1, p, 8)
+    del S1
+    _say("nd=512: FP64 oracle CIC and P(k)")
+    gr = O.grid_compute(pos, nd, L, window=2, interlacing=True, precision=8)
+    del pos
+    pr = O.power_compute(gr, 1, nd, L, window=2, interlacing=True, Nk=Nk, precision=8)
+    np.testing.assert_array_equal(pk32["Nk"], pr["Nk"])
+    np.testing.assert_array_equal(pk64["Nk"], pr["Nk"])
+    e32 = _rel(pk32["Pk"], pr["Pk64"][0, 0])
+    e64 = _rel(pk64["Pk"], pr["Pk64"][0, 0])
+    _say(f"nd=512: P(k) error vs the FP64 oracle: FP32 {e32:.2e}, FP64 {e64:.2e}")
+    assert e32 < 1e-4
+    assert e64 < 1e-5

@@ -105,9 +105,39 @@ def test_self_conjugate_mode_is_real_with_sqrt2():
     n0 = np.indices((nd, nd, nd))[0]
     expect = np.sqrt(2.0) * A * 0.9 * (-1.0) ** n0 / np.sqrt(L**3)
     np.testing.assert_allclose(delta, expect, atol=1e-13)
-    # the DC mode is always zero
+    # without the R_max truncation the DC mode is zero
     d2, dk2, _ = O.field_from_modes(k, P, nd, Lc, _one_mode(nd, (0, 0, 0), (3.0, 0.0)))
     assert np.all(dk2 == 0.0) and np.all(d2 == 0.0)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_dc_mode_with_truncated_spectrum_is_uniform(prec):
+    """With R_max < 1e5 the DC mode is not zero: |k| is replaced by
+    k0 = (kl0 kl1 kl2)^(1/3) / 4, A = sqrt(P_trunc(k0) / 2), and the
+    self-conjugate rule writes sqrt(2) A g (density_grid.c:158-161,
+    169-175).  A single DC mode is then a uniform field
+    delta = sqrt(2) A g Normx.  P_trunc is the FFTLog-truncated spectrum
+    (pinned against scipy.fft.fht below), splined here by scipy's natural
+    cubic spline."""
+    from exshalos_amd.synth import bbks_spectrum
+
+    nd, Lc, R_max, g = 16, 4.0, 60.0, 3.0
+    k, P = bbks_spectrum()
+    Rt = np.float32 if prec == 4 else np.float64
+    d, dk, p = O.field_from_modes(k, P, nd, Lc, _one_mode(nd, (0, 0, 0), (g, 0.7)),
+                                  precision=prec, R_max=R_max)
+    k2, P2, _, _ = O.pk_truncate(np.asarray(k, Rt).astype(np.float64),
+                                 np.asarray(P, Rt).astype(np.float64), R_max)
+    kf = 2 * np.pi / (nd * Lc)
+    k0 = kf / 4.0
+    A = np.sqrt(CubicSpline(k2, P2, bc_type="natural")(k0) / 2.0)
+    expect = np.sqrt(2.0) * A * g / np.sqrt((nd * Lc) ** 3)
+    tol = 2e-6 if prec == 4 else 1e-12
+    assert dk[0, 0, 0, 1] == 0.0 and np.count_nonzero(dk) == 1
+    np.testing.assert_allclose(d, np.full(d.shape, expect), rtol=tol, atol=0)
+    # the DC amplitude follows the truncated spectrum (not the given one)
+    A_full = np.sqrt(CubicSpline(k, P, bc_type="natural")(k0) / 2.0)
+    assert abs(A_full / A - 1.0) > 1e-3
 
 
 @pytest.mark.parametrize("nd", [8, 16])

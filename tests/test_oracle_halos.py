@@ -340,3 +340,55 @@ def test_mass_function_integral_is_a_cumulative_trapezoid(bbks):
     expect = -cumulative_trapezoid(f / R**3, np.log(sig), initial=0.0)
     np.testing.assert_allclose(tabs["Int"], expect, rtol=1e-5, atol=1e-12 * expect.max())
     assert np.all(np.diff(tabs["Int"]) > 0)
+
+
+# ------------------------------------------------------------- mass draw
+@pytest.mark.parametrize("prec", [4, 8])
+def test_mass_draw_uses_the_halo_index_uniform(bbks, prec):
+    """Compute_Mass (find_halos.c:453-476): halo i (catalogue order, densest
+    first) takes the i-th uniform of taus seeded with the barrier seed and
+    Mh = InvI(I(count Mcell) + (I(next Mcell) - I(count Mcell)) u_i), with
+    the products count*Mcell in the grid's float type and the interpolated
+    integral rounded to it (den_tmp is an fft_real).  Recomputed here from
+    the taus stream (pinned by GSL's known answer), scipy's natural cubic
+    splines of the Tinker table (pinned above) and the sphere table (pinned
+    by OEIS A000605), for four isolated spikes of different heights, so
+    their counts differ and a permuted or shifted uniform fails."""
+    from scipy.interpolate import CubicSpline
+
+    k, P = bbks
+    R = np.float32 if prec == 4 else np.float64
+    nd = 48
+    p = O.params(nd, LC, precision=prec)
+    delta = np.zeros((nd, nd, nd))
+    spots = [(6, 6, 6), (30, 8, 10), (8, 30, 34), (34, 34, 20), (20, 44, 44)]
+    heights = [300.0, 120.0, 40.0, 12.0, 0.05]  # the last is the unvisited peak 0
+    for s, hgt in zip(spots, heights):
+        delta[s] = hgt * p.dc
+    seed = 4321
+    res = O.find_halos(delta.astype(R), k, P, LC, seed=seed, precision=prec, p=p)
+    cnt = res["count"]
+    assert len(cnt) == 4 and len(set(cnt.tolist())) == 4
+    np.testing.assert_array_equal(res["x"], np.array(spots[:4], dtype=np.int32))
+    tabs = res["tables"]
+    I = CubicSpline(tabs["Mknots"], tabs["Int"], bc_type="natural")
+    InvI = CubicSpline(tabs["Int"], tabs["Mknots"], bc_type="natural")
+    sph = list(O.cells_in_spheres(2000))
+    u = O.taus_uniforms(seed, len(cnt))
+
+    def draw(uu):
+        out = []
+        for c, ui in zip(cnt, uu):
+            nxt = sph[sph.index(int(c)) + 1]
+            m0 = float(R(c) * R(p.Mcell))
+            m1 = float(R(nxt) * R(p.Mcell))
+            v = R(I(m0) + (I(m1) - I(m0)) * ui)
+            out.append(R(InvI(float(v))))
+        return np.array(out, dtype=np.float64)
+
+    tol = 2e-6 if prec == 4 else 1e-9
+    got = res["Mh"].astype(np.float64)
+    np.testing.assert_allclose(got, draw(u), rtol=tol, atol=0)
+    # the bar separates the right draw from plausible wrong ones
+    for wrong in (u[::-1], np.roll(u, 1), O.taus_uniforms(seed + 1, len(cnt))):
+        assert np.max(np.abs(got / draw(wrong) - 1.0)) > 100 * tol

@@ -96,3 +96,78 @@ def test_positions_are_lattice_plus_displacement_wrapped():
     q = (np.indices((nd, nd, nd)).reshape(3, -1).T + 0.5) * LC
     np.testing.assert_allclose(pos, np.mod(q + S, L), atol=1e-12)
     assert (pos >= 0).all() and (pos < L).all()
+
+
+def _numpy_halo_disp(p, flag, S1, S2, x, count, R):
+    """Compute_1LPT/Compute_2LPT halo sums and Compute_Posh restated with
+    numpy (lpt.c:56-80, 182-210, 245-255): each cell whose |flag| < ng adds
+    its displacement to halo |flag| in cell order, in the grid's float type
+    (posh += kx); velocities add f * kx in double, rounded to the float type
+    each time (velh += pow(Omz, 0.5454) Hz / (1 + z) * kx, and
+    2 pow(Omz, 6/11) Hz / (1 + z) * kx for the second order); the 1LPT pass
+    over all cells precedes the 2LPT pass.  Then posh = wrap(x Lc + Lc/2 +
+    posh / count) with the centre rounded to the float type, velh /= count.
+    The order of additions per halo is kept by walking the members by their
+    rank within their halo."""
+    ng = flag.size
+    h = np.abs(flag.reshape(-1))
+    cells = np.nonzero(h < ng)[0]
+    hh = h[cells]
+    order = np.argsort(hh, kind="stable")  # members of each halo in cell order
+    first = np.searchsorted(hh[order], hh[order])
+    rank = np.empty(len(cells), dtype=np.int64)
+    rank[order] = np.arange(len(cells)) - first
+    nh = len(count)
+    posh = np.zeros((nh, 3), dtype=R)
+    velh = np.zeros((nh, 3), dtype=R)
+    f64 = np.float64
+    Omz, Hz, z = f64(R(p.Omz)), f64(R(p.Hz)), f64(R(p.redshift))
+    facs = (Omz**0.5454 * Hz / (1.0 + z), 2.0 * Omz ** (6.0 / 11.0) * Hz / (1.0 + z))
+    for S, fac in ((S1, facs[0]), (S2, facs[1])):
+        if S is None:
+            continue
+        S = np.asarray(S, dtype=R).reshape(-1, 3)
+        for r in range(int(rank.max()) + 1 if len(rank) else 0):
+            sel = rank == r
+            c, t = cells[sel], hh[sel]
+            posh[t] = posh[t] + S[c]
+            velh[t] = (velh[t].astype(f64) + fac * S[c].astype(f64)).astype(R)
+    Lc, L = R(p.Lc), np.array(p.L, dtype=R)
+    base = (x.astype(R) * Lc).astype(f64) + f64(Lc) / 2.0
+    base = base.astype(R)
+    out = base + posh / count.astype(R)[:, None]
+    out = np.where(out >= L, out - L, out)
+    out = np.where(out < 0, out + L, out).astype(R)
+    return out, (velh / count.astype(R)[:, None]).astype(R)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("do2", [False, True])
+def test_halo_positions_and_velocities_numpy_restatement(bbks, prec, do2):
+    """The oracle's per-halo displacement means (ora_halo_disp) equal the
+    numpy restatement bit for bit, on a random field's catalogue (halos of
+    many sizes, centres flagged -h, rejected cells ng + 1 with Nmin = 3)
+    with 1LPT and 2LPT displacements.  Dropping the centre cells (flag
+    -h) or the second-order velocity term breaks the equality."""
+    from exshalos_amd import synth
+
+    k, P = bbks
+    R = np.float32 if prec == 4 else np.float64
+    nd = 32
+    noise = synth.white_noise(nd, 8, R)
+    delta, dk, p = O.field_from_modes(k, P, nd, LC, noise, precision=prec)
+    res = O.find_halos(delta, k, P, LC, Nmin=3, seed=5, precision=prec, p=p)
+    assert len(res["count"]) > 50 and (res["flag"] == nd**3 + 1).any()
+    S1, S2 = O.lpt(dk, p, 1e4, do2, prec)
+    posh, velh = O.halo_positions(p, res["flag"], S1, S2, res, precision=prec, with_vel=True)
+    ep, ev = _numpy_halo_disp(p, res["flag"], S1, S2, res["x"], res["count"], R)
+    np.testing.assert_array_equal(posh, ep)
+    np.testing.assert_array_equal(velh, ev)
+    assert (posh >= 0).all() and (posh < nd * LC).all()
+    # plausible mistakes are caught
+    wrong, _ = _numpy_halo_disp(p, np.where(res["flag"] < 0, nd**3, res["flag"]), S1, S2,
+                                res["x"], res["count"], R)
+    assert not np.array_equal(wrong, posh)  # centre cells dropped
+    if do2:
+        _, wv = _numpy_halo_disp(p, res["flag"], S1, None, res["x"], res["count"], R)
+        assert not np.array_equal(wv, velh)
