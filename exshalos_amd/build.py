@@ -33,6 +33,7 @@ SOURCES = [
     "sort.hip",
     "halos.hip",
     "spectrum.hip",
+    "adjoint.hip",
 ]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

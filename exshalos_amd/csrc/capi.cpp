@@ -308,6 +308,28 @@ int exh_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smoot
   EXH_API_END
 }
 
+int exh_pk_adjoint(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
+                   int do_2lpt, int window, int Nk, double k_min, double k_max,
+                   int kbin_precision, const double* gP, void* g_noise, double* P_out,
+                   long long* count_out) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  check_box(b);
+  check_src(s);
+  EXH_REQUIRE(gP != nullptr && g_noise != nullptr, EXH_ERR_ARG, "null gradient array");
+  EXH_REQUIRE(Nk >= 1 && k_max > k_min, EXH_ERR_ARG, "bad binning");
+  EXH_REQUIRE(kbin_precision == 4 || kbin_precision == 8, EXH_ERR_ARG,
+              "kbin_precision must be 4 or 8");
+  if (b->precision == 4)
+    run_pk_adjoint<float>(c, b, s, k_smooth, do_2lpt, window, Nk, k_min, k_max, kbin_precision,
+                          gP, (float*)g_noise, P_out, count_out);
+  else
+    run_pk_adjoint<double>(c, b, s, k_smooth, do_2lpt, window, Nk, k_min, k_max, kbin_precision,
+                           gP, (double*)g_noise, P_out, count_out);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  EXH_API_END
+}
+
 }  // extern "C"
 
 // Runs the caller's launches on another stream of the context until the

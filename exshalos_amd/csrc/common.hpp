@@ -82,6 +82,13 @@ enum Slot {
   BUF_SHELLS,   // bispectrum: real fields of every k shell and tracer
   BUF_SHELLK,   // bispectrum: combined modes, shell index per mode, shell work grids
   BUF_EFF,      // halo growth: delta with flagged cells set to -Mtot
+  BUF_ADJ_TAB,  // adjoint: per-bin factors
+  BUF_ADJ_S1,   // adjoint: first-order displacements [ng][3]
+  BUF_ADJ_POS,  // adjoint: recomputed particle positions [ng][3]
+  BUF_ADJ_GRID, // adjoint: recomputed density grid
+  BUF_ADJ_G,    // adjoint: dL/dgrid, then dL/dphi
+  BUF_ADJ_GS,   // adjoint: dL/dpos, then dL/dS1 [ng][3]
+  BUF_ADJ_G6,   // adjoint: dL/dS_ab of the 2LPT source [6][ng]
   BUF_NSLOTS
 };
 

@@ -89,11 +89,9 @@ __device__ inline double sqrt_t<double>(double v) {
   return sqrt(v);
 }
 
-// Generated (non-copied) mode at (i, j, k): A(|k|) * noise, with the
-// reference's float/double placement (density_grid.c:152-198).
+// Amplitude A = sqrt(P(|k|) / 2) of mode (i, j, k) (density_grid.c:94-166).
 template <typename T>
-__device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j,
-                                      int k, T& re, T& im) {
+__device__ inline T mode_amp(const FieldArgs<T>& a, const SplineView& sp, int i, int j, int k) {
   T kx = (i < a.n0 / 2) ? (T)i * a.kl0 : (T)(i - a.n0) * a.kl0;
   T ky = (j < a.n1 / 2) ? (T)j * a.kl1 : (T)(j - a.n1) * a.kl1;
   T kz = (T)k * a.kl2;
@@ -106,6 +104,15 @@ __device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& s
     else if (a.dc)
       A = (T)sqrt(spline_eval_v(sp, (double)a.dc_kmod) / 2.0);
   }
+  return A;
+}
+
+// Generated (non-copied) mode at (i, j, k): A(|k|) * noise, with the
+// reference's float/double placement (density_grid.c:152-198).
+template <typename T>
+__device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j,
+                                      int k, T& re, T& im) {
+  const T A = mode_amp(a, sp, i, j, k);
   size_t id = ((size_t)i * a.n1 + j) * (size_t)a.nz2 + k;
   T g0 = a.noise[2 * id], g1 = a.noise[2 * id + 1];
   bool selfconj = (i == 0 || i == a.n0 / 2) && (j == 0 || j == a.n1 / 2) &&
@@ -125,6 +132,15 @@ __device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& s
     re = (T)((double)re * cos((double)a.phase) - (double)im * sin((double)a.phase));
     im = (T)((double)re * sin((double)a.phase) + (double)im * cos((double)a.phase));
   }
+}
+
+// True where the kz = 0, nd/2 planes copy the conjugate of the mirror mode
+// (-i, -j) instead of drawing one (density_grid.c:109-150).
+template <typename T>
+__device__ inline bool copied_mode(const FieldArgs<T>& a, int i, int j, int k) {
+  if (!((k == 0 || k == a.n2 / 2) && (i > a.n0 / 2 || j > a.n1 / 2))) return false;
+  return (i > a.n0 / 2 && j > a.n1 / 2) || (i == 0 || i == a.n0 / 2) ||
+         (j == 0 || j == a.n1 / 2) || (i > a.n0 / 2 && j < a.n1 / 2);
 }
 
 // Mode (i, j, k) of the field including the Hermitian copy rule of the
@@ -214,6 +230,56 @@ __global__ __launch_bounds__(256) void k_poisson_modes(FieldArgs<T> a, T k_smoot
     } else {
       out[id] = {(T)0, (T)0};
     }
+  }
+}
+
+// Adjoint of noise -> delta_k (field_mode) -> phi_k = delta_k / (-k^2) *
+// scale -> phi = c2r(phi_k): from gk = r2c(dL/dphi), dL/dphi_k = w gk
+// (w = 2 for 0 < kz < nd/2, 1 in the kz = 0, nd/2 planes: the transpose of
+// the c2r), dL/ddelta_k = dL/dphi_k scale / (-k^2), and dL/dnoise through
+// the amplitudes and the copy rule: a drawn mode collects its own gradient
+// and the conjugate of its copy's (the mirror mode in those planes); copied
+// and zero-amplitude modes get zero.  Unit noise without `fixed` or a
+// phase.
+template <typename T, bool STAGE>
+__global__ __launch_bounds__(256) void k_field_adjoint(FieldArgs<T> a, T k_smooth, T scale,
+                                                       FastDiv dz, FastDiv dy,
+                                                       const typename cplx<T>::type* gk,
+                                                       T* gnoise) {
+  extern __shared__ double spl[];
+  const SplineView sp = stage_spline<STAGE>(a, spl);
+  const size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
+       id += (size_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    mode_coords(id, nh, a.n1, a.nz2, dz, dy, i, j, k);
+    double g0 = 0.0, g1 = 0.0;
+    T fact;
+    if (!copied_mode(a, i, j, k) &&
+        lpt_fact<T>(a.n0, a.n1, a.n2, a.kl0, a.kl1, a.kl2, k_smooth, i, j, k, fact)) {
+      const double A = (double)mode_amp(a, sp, i, j, k);
+      const double w = (k == 0 || 2 * k == a.n2) ? 1.0 : 2.0;
+      const double f = A * w * (double)scale / (double)fact;
+      const bool selfconj = (i == 0 || i == a.n0 / 2) && (j == 0 || j == a.n1 / 2) &&
+                            (k == 0 || k == a.n2 / 2);
+      const auto v = gk[id];
+      if (selfconj) {
+        g0 = 1.4142135623730951 * f * (double)v.x;
+      } else {
+        g0 = f * (double)v.x;
+        g1 = f * (double)v.y;
+        if (k == 0 || k == a.n2 / 2) {
+          const int mi = i == 0 ? 0 : a.n0 - i, mj = j == 0 ? 0 : a.n1 - j;
+          if (copied_mode(a, mi, mj, k)) {
+            const auto u = gk[((size_t)mi * a.n1 + mj) * (size_t)a.nz2 + k];
+            g0 += f * (double)u.x;
+            g1 -= f * (double)u.y;
+          }
+        }
+      }
+    }
+    gnoise[2 * id] = (T)g0;
+    gnoise[2 * id + 1] = (T)g1;
   }
 }
 
@@ -708,6 +774,27 @@ void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_s
 }
 
 
+// dL/dnoise of the white-noise modes from gk = r2c(dL/dphi1), phi1 the
+// first-order potential the LPT computes from those modes (k_poisson_modes).
+template <typename T>
+void launch_field_adjoint(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
+                          const void* gk, T* gnoise) {
+  EXH_REQUIRE(!s->fixed && s->phase == 0.0, EXH_ERR_UNSUPPORTED,
+              "the adjoint takes unit white noise without fixed amplitudes or a phase");
+  auto a = make_field_args<T>(c, b, s, 1e5);
+  const size_t nh = (size_t)b->nd[0] * b->nd[1] * b->nz2;
+  TimedScope ts(c, "field_adjoint");
+  auto kf = a.stage ? k_field_adjoint<T, true> : k_field_adjoint<T, false>;
+  hipLaunchKernelGGL(kf, dim3(mode_grid((long long)nh)), dim3(256), spline_lds(a), c->stream, a,
+                     (T)k_smooth, (T)b->Normx, fast_div(b->nz2), fast_div(b->nd[1]),
+                     (const typename cplx<T>::type*)gk, gnoise);
+  EXH_LAUNCH_CHECK();
+}
+
+template void launch_field_adjoint<float>(exh_ctx*, const exh_box*, const exh_field_src*, double,
+                                          const void*, float*);
+template void launch_field_adjoint<double>(exh_ctx*, const exh_box*, const exh_field_src*,
+                                           double, const void*, double*);
 template void launch_field<float>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*,
                                   void*, const std::function<void()>&);
 template void launch_field<double>(exh_ctx*, const exh_box*, const exh_field_src*, double, void*,

@@ -18,6 +18,16 @@ void launch_field(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k
 template <typename T>
 void launch_lpt(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
                 int do_2lpt, int out_pos, T* S, T* V, T* P = nullptr);
+template <typename T>
+void launch_field_adjoint(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
+                          const void* gk, T* gnoise);
+// adjoint.hip: dL/dnoise of the monopole P(k) of the LPT particles of the
+// modes (gP: host [Nk] dL/dP_b)
+template <typename T>
+void run_pk_adjoint(exh_ctx* c, const exh_box* b, const exh_field_src* s, double k_smooth,
+                    int do_2lpt, int window, int Nk, double k_min, double k_max,
+                    int kbin_precision, const double* gP, T* gnoise, double* P_out,
+                    long long* count_out);
 
 // halos.hip
 struct HaloResult {
@@ -61,6 +71,15 @@ int run_bispectrum(exh_ctx* c, const T* grid, int ntype, int nd, double L, int w
                    double* kP, double* Pk, double* Nk_out, double* kB, double* Bk,
                    double* Ntri_out);
 void bispectrum_configs(int Nk, std::vector<std::array<int, 3>>& tri);
+// adjoints: dL/dgrid of the monopole P(k) of one grid (gP: host [Nk]), and
+// dL/dpos of the particles of a normalised unweighted grid from dL/dgrid
+template <typename T>
+void run_power_adjoint(exh_ctx* c, const T* grid, int nd, double L, int window, double R,
+                       int Nk, double k_min, double k_max, int kbin_precision, const double* gP,
+                       T* g, double* P_out, long long* count_out);
+template <typename T>
+void run_deposit_adjoint(exh_ctx* c, const T* pos, long long np, int nd, double L, int window,
+                         const T* g, T* gpos);
 // host outputs of a deferred run_power (waits for the context's stream)
 void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_out);
 

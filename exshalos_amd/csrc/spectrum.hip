@@ -886,6 +886,40 @@ __device__ inline double shfl_up_d(double v, int o) {
 
 static constexpr int PW_MAXV = 64;  // max values per mode (pairs x multipoles + 1)
 
+// Bin of half-complex mode (i, j, k), or -1: the reference's mode mask
+// (powermodule.c:47-55) and bin index floor((|k| - k_min) / dk) in float
+// (kbin 4, spectrum_h.c:39-48) or double (kbin 8); |k| and mu along the
+// line of sight as the binning evaluates them.
+__device__ inline int mode_bin(const PowerArgs& a, int i, int j, int k, double& kmod, double& mu) {
+  const int nd = a.nd;
+  const bool masked = (k == 0 || k == nd / 2) &&
+                      ((i > nd / 2 && j > nd / 2) || (i == 0 || i == nd / 2) ||
+                       (j == 0 || j == nd / 2) || (i > nd / 2 && j < nd / 2));
+  if (masked) return -1;
+  const int fi = (2 * i < nd) ? i : i - nd, fj = (2 * j < nd) ? j : j - nd;
+  const int fk = (2 * k < nd) ? k : k - nd;
+  int b;
+  if (a.kbin == 4) {
+    float kx = (float)fi * a.knf, ky = (float)fj * a.knf, kz = (float)fk * a.knf;
+    float km = sqrtf(kx * kx + ky * ky + kz * kz);
+    float tmp = (km - a.kminf) / a.dkf;
+    b = (int)floorf(tmp);
+    kmod = km;
+    if (a.direction == 0) mu = (double)(kx / km);
+    if (a.direction == 1) mu = (double)(ky / km);
+    if (a.direction == 2) mu = (double)(kz / km);
+  } else {
+    double kx = fi * a.kn, ky = fj * a.kn, kz = fk * a.kn;
+    double km = sqrt(kx * kx + ky * ky + kz * kz);
+    b = (int)floor((km - a.kmin) / a.dk);
+    kmod = km;
+    if (a.direction == 0) mu = kx / km;
+    if (a.direction == 1) mu = ky / km;
+    if (a.direction == 2) mu = kz / km;
+  }
+  return (b < 0 || b >= a.Nk) ? -1 : b;
+}
+
 // W of the sphere (3) and exponential (4) windows at mode (fi, fj, fk), in
 // the grid's precision with the reference's double intermediates
 // (spectrum_h.c:60-67).
@@ -952,35 +986,8 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
       const int part = (int)(row - row0);
       const int nparts = (int)(__shfl(row, 63, 64) - row0) + 1;
       const int fi = (2 * i < nd) ? i : i - nd, fj = (2 * j < nd) ? j : j - nd;
-      int b = -1;
       double kmod = 0, mu = 0;
-      if (k < nz) {
-        const int fk = (2 * k < nd) ? k : k - nd;
-        bool masked = (k == 0 || k == nd / 2) &&
-                      ((i > nd / 2 && j > nd / 2) || (i == 0 || i == nd / 2) ||
-                       (j == 0 || j == nd / 2) || (i > nd / 2 && j < nd / 2));
-        if (!masked) {
-          if (a.kbin == 4) {
-            float kx = (float)fi * a.knf, ky = (float)fj * a.knf, kz = (float)fk * a.knf;
-            float km = sqrtf(kx * kx + ky * ky + kz * kz);
-            float tmp = (km - a.kminf) / a.dkf;
-            b = (int)floorf(tmp);
-            kmod = km;
-            if (a.direction == 0) mu = (double)(kx / km);
-            if (a.direction == 1) mu = (double)(ky / km);
-            if (a.direction == 2) mu = (double)(kz / km);
-          } else {
-            double kx = fi * a.kn, ky = fj * a.kn, kz = fk * a.kn;
-            double km = sqrt(kx * kx + ky * ky + kz * kz);
-            b = (int)floor((km - a.kmin) / a.dk);
-            kmod = km;
-            if (a.direction == 0) mu = kx / km;
-            if (a.direction == 1) mu = ky / km;
-            if (a.direction == 2) mu = kz / km;
-          }
-          if (b < 0 || b >= Nk) b = -1;
-        }
-      }
+      const int b = k < nz ? mode_bin(a, i, j, k, kmod, mu) : -1;
       const bool valid = b >= 0;
       // segment bookkeeping
       const int bprev = __shfl_up(b, 1, 64);
@@ -1099,25 +1106,17 @@ static double sinc_t(T x) {
   return r;
 }
 
+// Binning arguments of the P(k) passes: per-axis tables with the
+// reference's fft_real rounding of kn and Lb (uploaded to the context's
+// table slot), window, Normk and the bin-edge arithmetic.
 template <typename T>
-void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window, double R,
-               int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
-               int kbin_precision, double* k_out, double* P_out, long long* count_out,
-               bool deferred) {
-  using C2 = typename cplx<T>::type;
-  EXH_REQUIRE(nd >= 2 && nd % 2 == 0, EXH_ERR_ARG, "nd must be even");
-  EXH_REQUIRE(ntype >= 1 && ntype <= 8, EXH_ERR_UNSUPPORTED, "1 to 8 tracer types supported");
-  EXH_REQUIRE(Nk >= 1 && l_max >= 0, EXH_ERR_ARG, "bad binning");
-  EXH_REQUIRE(window >= 0 && window <= 5, EXH_ERR_UNSUPPORTED, "window %d cannot be deconvolved",
-              window);
+static PowerArgs make_power_args(exh_ctx* c, int ntype, int nd, double L, int window, double R,
+                                 int interlacing, int Nk, double k_min, double k_max, int l_max,
+                                 int direction, int kbin_precision) {
   const int nsub = interlacing ? 2 : 1;
   const int nz = nd / 2 + 1;
-  const size_t nh = (size_t)nd * nd * nz;
   const int NPs = ntype * (ntype + 1) / 2, ls = l_max / 2 + 1;
   const int NV = NPs * ls + 1;
-  EXH_REQUIRE(NV <= PW_MAXV, EXH_ERR_UNSUPPORTED, "too many spectra x multipoles");
-  C2* gk = (C2*)ctx_buf(c, BUF_GRIDK, nh * ntype * nsub * sizeof(C2));
-  fft_r2c(c, nd, nd, nd, sizeof(T) == 4 ? 4 : 8, ntype * nsub, grid, gk);
 
   // per-axis tables with the reference's fft_real rounding of kn and Lb
   T Lr = (T)L;
@@ -1184,6 +1183,30 @@ void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int windo
     // the FP32 build rounds kn to float before use
     a.kn = (double)(float)a.kn;
   }
+  return a;
+}
+
+template <typename T>
+void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window, double R,
+               int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,
+               int kbin_precision, double* k_out, double* P_out, long long* count_out,
+               bool deferred) {
+  using C2 = typename cplx<T>::type;
+  EXH_REQUIRE(nd >= 2 && nd % 2 == 0, EXH_ERR_ARG, "nd must be even");
+  EXH_REQUIRE(ntype >= 1 && ntype <= 8, EXH_ERR_UNSUPPORTED, "1 to 8 tracer types supported");
+  EXH_REQUIRE(Nk >= 1 && l_max >= 0, EXH_ERR_ARG, "bad binning");
+  EXH_REQUIRE(window >= 0 && window <= 5, EXH_ERR_UNSUPPORTED, "window %d cannot be deconvolved",
+              window);
+  const int nsub = interlacing ? 2 : 1;
+  const int nz = nd / 2 + 1;
+  const size_t nh = (size_t)nd * nd * nz;
+  const int NPs = ntype * (ntype + 1) / 2, ls = l_max / 2 + 1;
+  const int NV = NPs * ls + 1;
+  EXH_REQUIRE(NV <= PW_MAXV, EXH_ERR_UNSUPPORTED, "too many spectra x multipoles");
+  C2* gk = (C2*)ctx_buf(c, BUF_GRIDK, nh * ntype * nsub * sizeof(C2));
+  fft_r2c(c, nd, nd, nd, sizeof(T) == 4 ? 4 : 8, ntype * nsub, grid, gk);
+  PowerArgs a = make_power_args<T>(c, ntype, nd, L, window, R, interlacing, Nk, k_min, k_max,
+                                   l_max, direction, kbin_precision);
 
   // LDS: per wave Nk*NV doubles + Nk counters
   size_t per_wave = (size_t)Nk * NV * 8 + (size_t)Nk * 4;
@@ -1258,6 +1281,211 @@ void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_o
       }
   }
 }
+
+// ---------------------------------------------------------------------------
+// adjoints of the monopole P(k) and of the mass assignment
+// ---------------------------------------------------------------------------
+// In place over the modes F = r2c(delta) of one grid (no interlacing):
+// G = (dL/dF) / w with dL/dF = gPn[b] wk^2 F for every mode the forward
+// pass bins (same mask and bin index), 0 elsewhere.  With
+// P_b = sum_b |F wk|^2 / N_b, gPn[b] = 2 dL/dP_b / N_b.  w is the weight
+// of a stored mode in the c2r (2 for 0 < kz < nd/2, 1 in the kz = 0, nd/2
+// planes), so that c2r(G) is the gradient with respect to the real grid:
+// the transpose of the r2c.
+template <typename T>
+__global__ void k_power_adjoint(PowerArgs a, const double* gPn, typename cplx<T>::type* gk) {
+  using C2 = typename cplx<T>::type;
+  const int nd = a.nd, nz = a.nz;
+  const size_t nh = (size_t)nd * nd * nz;
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
+       id += (size_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    if (nh < (1ull << 31)) {
+      const unsigned r = fdiv((unsigned)id, a.dz);
+      k = (int)((unsigned)id - r * (unsigned)nz);
+      i = (int)fdiv(r, a.dnd);
+      j = (int)(r - (unsigned)i * (unsigned)nd);
+    } else {
+      k = (int)(id % nz);
+      const size_t r = id / nz;
+      j = (int)(r % nd);
+      i = (int)(r / nd);
+    }
+    double kmod, mu;
+    const int b = mode_bin(a, i, j, k, kmod, mu);
+    C2 out{(T)0, (T)0};
+    if (b >= 0) {
+      const int fi = (2 * i < nd) ? i : i - nd, fj = (2 * j < nd) ? j : j - nd;
+      const double wk = a.rwin ? a.Normk / (double)window_r<T>(a, fi, fj, (2 * k < nd) ? k : k - nd)
+                               : a.wx[i] * a.wx[j] * a.wz[k] * a.Normk;
+      const double w = (k == 0 || 2 * k == nd) ? 1.0 : 2.0;
+      const double f = gPn[b] * wk * wk / w;
+      const C2 F = gk[id];
+      out = C2{(T)(f * (double)F.x), (T)(f * (double)F.y)};
+    }
+    gk[id] = out;
+  }
+}
+
+// CIC weights of one axis and their derivatives with respect to the
+// coordinate, for the cell xt and its neighbour o toward the particle
+// (dep_cic's arithmetic).
+template <typename T>
+__device__ inline void cic_axis(T x, T Ld, int nd, int& xt, int& o, T& t, T& d, double& dt) {
+  T q = x / Ld;
+  xt = (int)floor(q);
+  if (xt == nd) xt -= 1;
+  T dx = (T)((double)q - ((double)xt + 0.5));
+  double sgn = 1.0;
+  if (dx >= (T)0) {
+    o = xt + 1 == nd ? 0 : xt + 1;
+  } else {
+    o = xt == 0 ? nd - 1 : xt - 1;
+    dx = -dx;
+    sgn = -1.0;
+  }
+  t = (T)(1.0 - (double)dx);
+  d = dx;
+  dt = -sgn / (double)Ld;  // d t / dx; d d / dx = -dt
+}
+
+// dL/dpos of every particle from g = dL/drho on the grid: the transpose of
+// the deposit, the weights' derivatives at the particle times g at the
+// cells it feeds, times `scale` (1 / rho_bar for normalised grids).
+// Positions are wrapped as the deposit wraps them.
+template <typename T, int WIN>
+__global__ void k_deposit_adjoint(const T* pos, long long np, int nd, T L, const T* g,
+                                  double scale, T* gpos) {
+  const T Ld = L / (T)nd;
+  const size_t n2 = (size_t)nd * nd;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < np;
+       p += (long long)gridDim.x * blockDim.x) {
+    T x[3];
+    bool ok = true;
+    for (int a = 0; a < 3; a++) {
+      x[a] = wrap_while<T>(pos[3 * p + a], L);
+      ok = ok && x[a] >= (T)0;
+    }
+    double gx[3] = {0.0, 0.0, 0.0};
+    if (ok && WIN == 2) {
+      int xt[3], o[3];
+      T t[3], d[3];
+      double dt[3];
+      for (int a = 0; a < 3; a++) cic_axis<T>(x[a], Ld, nd, xt[a], o[a], t[a], d[a], dt[a]);
+      // the eight cells: c[ax] selects xt (0) or o (1) per axis
+      for (int c = 0; c < 8; c++) {
+        const int s0 = c & 1, s1 = (c >> 1) & 1, s2 = (c >> 2) & 1;
+        const size_t id = (size_t)(s0 ? o[0] : xt[0]) * n2 + (size_t)(s1 ? o[1] : xt[1]) * nd +
+                          (size_t)(s2 ? o[2] : xt[2]);
+        const double gv = (double)g[id];
+        const double w0 = s0 ? d[0] : t[0], w1 = s1 ? d[1] : t[1], w2 = s2 ? d[2] : t[2];
+        const double e0 = s0 ? -dt[0] : dt[0], e1 = s1 ? -dt[1] : dt[1], e2 = s2 ? -dt[2] : dt[2];
+        gx[0] += gv * e0 * w1 * w2;
+        gx[1] += gv * w0 * e1 * w2;
+        gx[2] += gv * w0 * w1 * e2;
+      }
+    } else if (ok && WIN == 5) {
+      int cc[3];
+      double w[3][3], dw[3][3];
+      for (int a = 0; a < 3; a++) {
+        const double u = (double)x[a] / (double)Ld - 0.5;
+        const double cf = floor(u + 0.5);
+        const double dd = u - cf;
+        cc[a] = (int)cf;
+        w[a][0] = (double)(T)(0.5 * (0.5 - dd) * (0.5 - dd));
+        w[a][1] = (double)(T)(0.75 - dd * dd);
+        w[a][2] = (double)(T)(0.5 * (0.5 + dd) * (0.5 + dd));
+        const double du = 1.0 / (double)Ld;
+        dw[a][0] = -(0.5 - dd) * du;
+        dw[a][1] = -2.0 * dd * du;
+        dw[a][2] = (0.5 + dd) * du;
+      }
+      for (int ia = 0; ia < 3; ia++) {
+        int ix = cc[0] + ia - 1;
+        ix = ix < 0 ? ix + nd : (ix >= nd ? ix - nd : ix);
+        for (int ib = 0; ib < 3; ib++) {
+          int iy = cc[1] + ib - 1;
+          iy = iy < 0 ? iy + nd : (iy >= nd ? iy - nd : iy);
+          for (int ie = 0; ie < 3; ie++) {
+            int iz = cc[2] + ie - 1;
+            iz = iz < 0 ? iz + nd : (iz >= nd ? iz - nd : iz);
+            const double gv = (double)g[(size_t)ix * n2 + (size_t)iy * nd + iz];
+            gx[0] += gv * dw[0][ia] * w[1][ib] * w[2][ie];
+            gx[1] += gv * w[0][ia] * dw[1][ib] * w[2][ie];
+            gx[2] += gv * w[0][ia] * w[1][ib] * dw[2][ie];
+          }
+        }
+      }
+    }
+    for (int a = 0; a < 3; a++) gpos[3 * p + a] = (T)(gx[a] * scale);
+  }
+}
+
+// Gradient of the monopole P(k) of `grid` with respect to the grid values:
+// g = c2r(G) with G from k_power_adjoint; gP: host [Nk] dL/dP_b.  Runs
+// the forward binning first (the counts N_b, and P_out / count_out when
+// given).  The modes are left in the context's BUF_GRIDK and overwritten.
+template <typename T>
+void run_power_adjoint(exh_ctx* c, const T* grid, int nd, double L, int window, double R,
+                       int Nk, double k_min, double k_max, int kbin_precision, const double* gP,
+                       T* g, double* P_out, long long* count_out) {
+  using C2 = typename cplx<T>::type;
+  std::vector<long long> cnt(Nk);
+  std::vector<double> Pk(Nk);
+  run_power<T>(c, grid, 1, nd, L, window, R, 0, Nk, k_min, k_max, 0, -1, kbin_precision, nullptr,
+               Pk.data(), cnt.data(), false);
+  if (P_out) std::copy(Pk.begin(), Pk.end(), P_out);
+  if (count_out) std::copy(cnt.begin(), cnt.end(), count_out);
+  std::vector<double> gPn(Nk);
+  for (int b = 0; b < Nk; b++) gPn[b] = cnt[b] > 0 ? 2.0 * gP[b] / (double)cnt[b] : 0.0;
+  PowerArgs a = make_power_args<T>(c, 1, nd, L, window, R, 0, Nk, k_min, k_max, 0, -1,
+                                   kbin_precision);
+  double* dg = (double*)ctx_pinned(c, (size_t)Nk * sizeof(double));
+  std::copy(gPn.begin(), gPn.end(), dg);
+  double* dgd = (double*)ctx_buf(c, BUF_ADJ_TAB, (size_t)Nk * sizeof(double));
+  EXH_CHECK_HIP(hipMemcpyAsync(dgd, dg, (size_t)Nk * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+  const size_t nh = (size_t)nd * nd * (nd / 2 + 1);
+  C2* gk = (C2*)c->buf[BUF_GRIDK];
+  {
+    TimedScope ts(c, "power_adjoint");
+    hipLaunchKernelGGL(k_power_adjoint<T>, dim3(std::min<size_t>((nh + 255) / 256, 65536)),
+                       dim3(256), 0, c->stream, a, dgd, gk);
+    EXH_LAUNCH_CHECK();
+  }
+  fft_c2r(c, nd, nd, nd, sizeof(T) == 4 ? 4 : 8, 1, gk, g);
+  EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // dg is pinned host staging
+}
+
+// dL/dpos (gpos [np][3]) from dL/ddelta (g) of an unweighted, normalised
+// grid of np particles: delta = rho ng / np - 1.
+template <typename T>
+void run_deposit_adjoint(exh_ctx* c, const T* pos, long long np, int nd, double L, int window,
+                         const T* g, T* gpos) {
+  EXH_REQUIRE(window == EXH_WINDOW_CIC || window == EXH_WINDOW_TSC, EXH_ERR_UNSUPPORTED,
+              "the adjoint supports the CIC (2) and TSC (5) windows");
+  const double scale = (double)nd * nd * nd / (double)np;
+  const unsigned gs = (unsigned)std::min<long long>((np + 255) / 256, 65536);
+  TimedScope ts(c, "deposit_adjoint");
+  if (window == EXH_WINDOW_CIC)
+    hipLaunchKernelGGL((k_deposit_adjoint<T, 2>), dim3(gs), dim3(256), 0, c->stream, pos, np, nd,
+                       (T)L, g, scale, gpos);
+  else
+    hipLaunchKernelGGL((k_deposit_adjoint<T, 5>), dim3(gs), dim3(256), 0, c->stream, pos, np, nd,
+                       (T)L, g, scale, gpos);
+  EXH_LAUNCH_CHECK();
+}
+
+template void run_power_adjoint<float>(exh_ctx*, const float*, int, double, int, double, int,
+                                       double, double, int, const double*, float*, double*,
+                                       long long*);
+template void run_power_adjoint<double>(exh_ctx*, const double*, int, double, int, double, int,
+                                        double, double, int, const double*, double*, double*,
+                                        long long*);
+template void run_deposit_adjoint<float>(exh_ctx*, const float*, long long, int, double, int,
+                                         const float*, float*);
+template void run_deposit_adjoint<double>(exh_ctx*, const double*, long long, int, double, int,
+                                          const double*, double*);
 
 // ---------------------------------------------------------------------------
 // halo abundance: abundance_compute (spectrum.c:535-600) and

@@ -78,6 +78,9 @@ _SIGS = {
     "exh_halos_flag": (C.c_int, [_P, _P, C.c_int]),
     "exh_lpt": (C.c_int, [_P, C.POINTER(Box), C.POINTER(FieldSrc), C.c_double, C.c_int, C.c_int,
                           _P, _P]),
+    "exh_pk_adjoint": (C.c_int, [_P, C.POINTER(Box), C.POINTER(FieldSrc), C.c_double, C.c_int,
+                                 C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P,
+                                 _P]),
     "exh_halos_box_from_pk": (C.c_int, [_P, C.POINTER(Box), C.POINTER(FieldSrc),
                                         C.POINTER(HaloOpts), C.c_double, C.c_int, C.c_int, _P,
                                         _P, _P, C.POINTER(C.c_longlong)]),

@@ -169,6 +169,26 @@ int exh_halos_flag(exh_ctx* ctx, long long* flag, int to_host);
 int exh_lpt(exh_ctx* ctx, const exh_box* box, const exh_field_src* src, double k_smooth,
             int do_2lpt, int out_pos, void* S, void* V);
 
+/* Gradient of a loss of the monopole power spectrum of the LPT particles
+ * with respect to the white-noise modes: the adjoint of modes -> delta_k
+ * (Compute_Den, density_grid.c:94-198) -> 1LPT (+ 2LPT) displacements and
+ * positions (lpt.c:4-242) -> unweighted, normalised mass assignment of the
+ * ng particles on the box's own grid (Tracer_Grid, gridmodule.c:146-270;
+ * window CIC or TSC, no interlacing) -> P(k) monopole in Nk bins
+ * (Compute_gridk + Power_Spectrum, spectrum_h.c:83-167,
+ * powermodule.c:4-108).  The reference has no adjoint: this entry point is
+ * the transpose of the forward steps the library runs for exh_lpt,
+ * exh_grid_compute and exh_power_compute, recomputing the positions instead
+ * of storing them.  modes: EXH_SRC_MODES (unit noise; fixed and phase are
+ * not supported).  gP: host [Nk] dL/dP_b.  g_noise: device
+ * [nd0][nd1][nz2][2] T, overwritten with dL/dnoise (zero at the copied
+ * modes of the kz = 0, nd/2 planes, whose noise the field ignores).  P_out,
+ * count_out (host [Nk], optional): the forward P(k) and mode counts. */
+int exh_pk_adjoint(exh_ctx* ctx, const exh_box* box, const exh_field_src* modes,
+                   double k_smooth, int do_2lpt, int window, int Nk, double k_min,
+                   double k_max, int kbin_precision, const double* gP, void* g_noise,
+                   double* P_out, long long* count_out);
+
 /* --------------------------------------------------------------- pipeline */
 /* Halos and particles from a linear P(k): replaces halos_box_from_pk
  * (exshalos.c:710-940) / Generate_Halos_Box_from_Pk (box.c:4-68).
