@@ -74,24 +74,95 @@ def algorithmic_bytes(name: str, nd: int, npk: int, interlacing: int, prec: int)
     return table.get(name)
 
 
-def run_cpu_baseline(k, P, nd=128, reps=1):
-    """The CPU oracle (a C + numpy restatement of the reference's CPU path,
-    single-threaded C loops and numpy FFTs) on a bounded sample: the same
-    pipeline at nd=128, FP32 mode."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(k, P, nd=256, threads=None):
+    """The reference's CPU path restated (oracle/exh_oracle.c) and timed on
+    the host: the OpenMP build (deposit, deconvolution, binning and
+    normalisations threaded as the reference threads them; field modes,
+    peaks, sort, growth, masses and LPT stencils sequential as in the
+    reference) with threaded pocketfft transforms standing in for FFTW's
+    threads, on `threads` cores (OMP_NUM_THREADS, else all cores, at most
+    16: the host share of one GPU).  The pipeline is the benchmark's at
+    nd (FP32 mode): field -> halos -> 1LPT + 2LPT -> interlaced CIC -> P(k)
+    in nd/2 bins.  Also times the sequential parity build on the same
+    sample (kind "port", 1 core) for comparison."""
     from oracle import pipeline as O
     from exshalos_amd import synth
 
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(16, os.cpu_count() or 1)
     noise = synth.white_noise(nd, 1, np.float32)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        out = O.halos_box_from_pk(k, P, nd, 4.0, noise, k_smooth=1e4, seed=1, do_2lpt=True,
-                                  out_lpt=True, precision=4)
-        g = O.grid_compute(out["pos"], nd, 4.0 * nd, window=2, interlacing=True, precision=4)
-        O.power_compute(g, 1, nd, 4.0 * nd, window=2, interlacing=True, Nk=nd // 2, precision=4)
-    dt = (time.perf_counter() - t0) / reps
-    return {"value": nd**3 / dt / 1e6, "unit": "Mcells/s", "cores": 1, "kind": "port",
-            "sample": f"oracle pipeline field->halos->2LPT->CIC(interlaced)->P(k) at nd={nd}, "
-                      f"Lc=4, FP32 mode, {reps} run(s), {dt:.2f} s each"}
+    L = 4.0 * nd
+    fft_s = [0.0]
+    c2r0, r2c0, workers0 = O.c2r, O.r2c, O._fft_workers
+
+    def timed(f):
+        def g(*a, **kw):
+            t = time.perf_counter()
+            try:
+                return f(*a, **kw)
+            finally:
+                fft_s[0] += time.perf_counter() - t
+        return g
+
+    def pipeline(omp, workers):
+        O.use_openmp(omp)
+        if omp:
+            got = O.lib().ora_omp_threads(workers)
+            assert got == workers, f"OpenMP runs {got} threads, not {workers}"
+        O._fft_workers = lambda: workers
+        O.c2r, O.r2c = timed(c2r0), timed(r2c0)
+        fft_s[0] = 0.0
+        st = {}
+        try:
+            t = time.perf_counter()
+            p = O.params(nd, 4.0, precision=4)
+            dk = O.field_modes(p, k, P, noise, 4, 1e5)
+            delta = (O.c2r(dk, (nd,) * 3).astype(np.float32) * np.float32(p.Normx))
+            st["field"] = time.perf_counter() - t
+            t = time.perf_counter()
+            hal = O.find_halos(delta, k, P, 4.0, seed=1, precision=4, p=p)
+            st["halos"] = time.perf_counter() - t
+            t = time.perf_counter()
+            S1, S2 = O.lpt(dk, p, 1e4, True, 4)
+            st["lpt"] = time.perf_counter() - t
+            t = time.perf_counter()
+            O.halo_positions(p, hal["flag"], S1, S2, hal, precision=4)
+            pos = O.positions((S1 + S2).astype(np.float32), p, 4)
+            del S1, S2
+            st["positions"] = time.perf_counter() - t
+            t = time.perf_counter()
+            g = O.grid_compute(pos, nd, L, window=2, interlacing=True, precision=4)
+            st["cic"] = time.perf_counter() - t
+            t = time.perf_counter()
+            O.power_compute(g, 1, nd, L, window=2, interlacing=True, Nk=nd // 2, precision=4)
+            st["power"] = time.perf_counter() - t
+        finally:
+            O.use_openmp(False)
+            O.c2r, O.r2c, O._fft_workers = c2r0, r2c0, workers0
+        st["fft_part"] = fft_s[0]
+        return sum(v for n, v in st.items() if n != "fft_part"), st
+
+    tot, st = pipeline(True, threads)
+    tot1, _ = pipeline(False, 1)
+    return {"value": round(nd**3 / tot / 1e6, 3), "unit": "Mcells/s", "cores": threads,
+            "kind": "port",
+            "sample": f"field->halos->2LPT->interlaced CIC->P(k) (Nk={nd // 2}) at nd={nd}, "
+                      f"Lc=4, FP32, one run: {tot:.2f} s on {threads} threads "
+                      f"(OpenMP where the reference uses it, threaded FFTs)",
+            "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count()},
+            "stages_s": {n: round(v, 3) for n, v in st.items()},
+            "port_1core": {"value": round(nd**3 / tot1 / 1e6, 3), "seconds": round(tot1, 2)}}
 
 
 def main():
@@ -226,7 +297,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        print("[bench] timing the CPU oracle sample", flush=True)
+        print(f"[bench] timing the CPU baseline at nd={args.cpu_nd}", flush=True)
         cpu = run_cpu_baseline(k, P, nd=args.cpu_nd)
 
     ms = r32["ms"]

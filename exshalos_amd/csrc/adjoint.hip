@@ -10,7 +10,8 @@
 // this is the transpose of each step, in reverse:
 //
 //   dL/dP_b -> dL/dF (k_power_adjoint) -> dL/ddelta = c2r(dL/dF / w)
-//   -> dL/dpos (k_deposit_adjoint, positions recomputed, not stored)
+//   -> dL/dpos (k_deposit_adjoint, positions recomputed from the lattice
+//      and S in double, not stored)
 //   -> dL/dphi2 = -c D^T... = -c/(60 Lc) sum_a D_a(dL/dS_a)   (D^T = -D)
 //   -> dL/dsrc = c2r(K2 r2c(dL/dphi2))   (K2 real and even: self-adjoint)
 //   -> dL/dS_ab (k_src_adjoint_a) -> dL/dS1 += -1/(60 Lc) D_b(dL/dS_ab)
@@ -159,7 +160,7 @@ void run_pk_adjoint(exh_ctx* c, const exh_box* b, const exh_field_src* s, double
   run_power_adjoint<T>(c, grid, n, L, window, 0.0, Nk, k_min, k_max, kbin_precision, gP, g, P_out,
                        count_out);
   T* gS = (T*)ctx_buf(c, BUF_ADJ_GS, 3 * ng * sizeof(T));
-  run_deposit_adjoint<T>(c, pos, (long long)ng, n, L, window, g, gS);
+  run_deposit_adjoint<T>(c, S, n, (double)(T)b->Lc, n, L, window, g, gS);
 
   using C2 = typename cplx<T>::type;
   const size_t nh = (size_t)n * n * (n / 2 + 1);

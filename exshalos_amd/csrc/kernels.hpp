@@ -72,14 +72,15 @@ int run_bispectrum(exh_ctx* c, const T* grid, int ntype, int nd, double L, int w
                    double* Ntri_out);
 void bispectrum_configs(int Nk, std::vector<std::array<int, 3>>& tri);
 // adjoints: dL/dgrid of the monopole P(k) of one grid (gP: host [Nk]), and
-// dL/dpos of the particles of a normalised unweighted grid from dL/dgrid
+// dL/dS of the LPT particles (nL^3 lattice, spacing Lc, displacements S)
+// of a normalised unweighted grid from dL/dgrid
 template <typename T>
 void run_power_adjoint(exh_ctx* c, const T* grid, int nd, double L, int window, double R,
                        int Nk, double k_min, double k_max, int kbin_precision, const double* gP,
                        T* g, double* P_out, long long* count_out);
 template <typename T>
-void run_deposit_adjoint(exh_ctx* c, const T* pos, long long np, int nd, double L, int window,
-                         const T* g, T* gpos);
+void run_deposit_adjoint(exh_ctx* c, const T* S, int nL, double Lc, int nd, double L, int window,
+                         const T* g, T* gS);
 // host outputs of a deferred run_power (waits for the context's stream)
 void power_copy_out(exh_ctx* c, double* k_out, double* P_out, long long* count_out);
 

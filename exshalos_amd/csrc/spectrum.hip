@@ -1349,29 +1349,38 @@ __device__ inline void cic_axis(T x, T Ld, int nd, int& xt, int& o, T& t, T& d, 
   dt = -sgn / (double)Ld;  // d t / dx; d d / dx = -dt
 }
 
-// dL/dpos of every particle from g = dL/drho on the grid: the transpose of
-// the deposit, the weights' derivatives at the particle times g at the
-// cells it feeds, times `scale` (1 / rho_bar for normalised grids).
-// Positions are wrapped as the deposit wraps them.
+// dL/dS of every particle of the nL^3 lattice (the LPT particles, lattice
+// order) from g = dL/drho on the grid: the transpose of the deposit, the
+// weights' derivatives at the particle times g at the cells it feeds, times
+// `scale` (1 / rho_bar for normalised grids).  The positions are recomputed
+// from the displacements, q + S with q = (i + 1/2) Lc, in double: for a
+// float field this places each particle relative to the weights' kinks (the
+// cell centres) to the precision of S rather than to that of a float
+// coordinate up to L, so the gradient is that of the exact positions.
 template <typename T, int WIN>
-__global__ void k_deposit_adjoint(const T* pos, long long np, int nd, T L, const T* g,
+__global__ void k_deposit_adjoint(const T* S, int nL, double Lc, int nd, double L, const T* g,
                                   double scale, T* gpos) {
-  const T Ld = L / (T)nd;
+  using D = double;
+  const D Ld = L / (D)nd;
   const size_t n2 = (size_t)nd * nd;
+  const long long np = (long long)nL * nL * nL;
   for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < np;
        p += (long long)gridDim.x * blockDim.x) {
-    T x[3];
+    const long long r = p / nL;
+    const int q[3] = {(int)(r / nL), (int)(r % nL), (int)(p % nL)};
+    D x[3];
     bool ok = true;
     for (int a = 0; a < 3; a++) {
-      x[a] = wrap_while<T>(pos[3 * p + a], L);
-      ok = ok && x[a] >= (T)0;
+      D v = ((D)q[a] * Lc + Lc / 2.0) + (D)S[3 * p + a];
+      x[a] = wrap_while<D>(v, L);
+      ok = ok && x[a] >= 0.0;
     }
     double gx[3] = {0.0, 0.0, 0.0};
     if (ok && WIN == 2) {
       int xt[3], o[3];
-      T t[3], d[3];
+      D t[3], d[3];
       double dt[3];
-      for (int a = 0; a < 3; a++) cic_axis<T>(x[a], Ld, nd, xt[a], o[a], t[a], d[a], dt[a]);
+      for (int a = 0; a < 3; a++) cic_axis<D>(x[a], Ld, nd, xt[a], o[a], t[a], d[a], dt[a]);
       // the eight cells: c[ax] selects xt (0) or o (1) per axis
       for (int c = 0; c < 8; c++) {
         const int s0 = c & 1, s1 = (c >> 1) & 1, s2 = (c >> 2) & 1;
@@ -1388,13 +1397,13 @@ __global__ void k_deposit_adjoint(const T* pos, long long np, int nd, T L, const
       int cc[3];
       double w[3][3], dw[3][3];
       for (int a = 0; a < 3; a++) {
-        const double u = (double)x[a] / (double)Ld - 0.5;
+        const double u = x[a] / Ld - 0.5;
         const double cf = floor(u + 0.5);
         const double dd = u - cf;
         cc[a] = (int)cf;
-        w[a][0] = (double)(T)(0.5 * (0.5 - dd) * (0.5 - dd));
-        w[a][1] = (double)(T)(0.75 - dd * dd);
-        w[a][2] = (double)(T)(0.5 * (0.5 + dd) * (0.5 + dd));
+        w[a][0] = 0.5 * (0.5 - dd) * (0.5 - dd);
+        w[a][1] = 0.75 - dd * dd;
+        w[a][2] = 0.5 * (0.5 + dd) * (0.5 + dd);
         const double du = 1.0 / (double)Ld;
         dw[a][0] = -(0.5 - dd) * du;
         dw[a][1] = -2.0 * dd * du;
@@ -1457,22 +1466,24 @@ void run_power_adjoint(exh_ctx* c, const T* grid, int nd, double L, int window, 
   EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // dg is pinned host staging
 }
 
-// dL/dpos (gpos [np][3]) from dL/ddelta (g) of an unweighted, normalised
-// grid of np particles: delta = rho ng / np - 1.
+// dL/dS (gS [nL^3][3]) of the LPT particles from dL/ddelta (g) of their
+// unweighted, normalised grid: delta = rho ng / np - 1.  Lc: the lattice
+// spacing (the box's cell), L the box side.
 template <typename T>
-void run_deposit_adjoint(exh_ctx* c, const T* pos, long long np, int nd, double L, int window,
-                         const T* g, T* gpos) {
+void run_deposit_adjoint(exh_ctx* c, const T* S, int nL, double Lc, int nd, double L, int window,
+                         const T* g, T* gS) {
   EXH_REQUIRE(window == EXH_WINDOW_CIC || window == EXH_WINDOW_TSC, EXH_ERR_UNSUPPORTED,
               "the adjoint supports the CIC (2) and TSC (5) windows");
+  const long long np = (long long)nL * nL * nL;
   const double scale = (double)nd * nd * nd / (double)np;
   const unsigned gs = (unsigned)std::min<long long>((np + 255) / 256, 65536);
   TimedScope ts(c, "deposit_adjoint");
   if (window == EXH_WINDOW_CIC)
-    hipLaunchKernelGGL((k_deposit_adjoint<T, 2>), dim3(gs), dim3(256), 0, c->stream, pos, np, nd,
-                       (T)L, g, scale, gpos);
+    hipLaunchKernelGGL((k_deposit_adjoint<T, 2>), dim3(gs), dim3(256), 0, c->stream, S, nL, Lc,
+                       nd, L, g, scale, gS);
   else
-    hipLaunchKernelGGL((k_deposit_adjoint<T, 5>), dim3(gs), dim3(256), 0, c->stream, pos, np, nd,
-                       (T)L, g, scale, gpos);
+    hipLaunchKernelGGL((k_deposit_adjoint<T, 5>), dim3(gs), dim3(256), 0, c->stream, S, nL, Lc,
+                       nd, L, g, scale, gS);
   EXH_LAUNCH_CHECK();
 }
 
@@ -1482,9 +1493,9 @@ template void run_power_adjoint<float>(exh_ctx*, const float*, int, double, int,
 template void run_power_adjoint<double>(exh_ctx*, const double*, int, double, int, double, int,
                                         double, double, int, const double*, double*, double*,
                                         long long*);
-template void run_deposit_adjoint<float>(exh_ctx*, const float*, long long, int, double, int,
+template void run_deposit_adjoint<float>(exh_ctx*, const float*, int, double, int, double, int,
                                          const float*, float*);
-template void run_deposit_adjoint<double>(exh_ctx*, const double*, long long, int, double, int,
+template void run_deposit_adjoint<double>(exh_ctx*, const double*, int, double, int, double, int,
                                           const double*, double*);
 
 // ---------------------------------------------------------------------------

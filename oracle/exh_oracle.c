@@ -33,6 +33,28 @@
 #define REAL double
 #endif
 
+/* -DORACLE_OMP builds the CPU baseline: the loops the reference runs under
+ * OpenMP (the deposit, gridmodule.c:173-366; the deconvolution,
+ * spectrum_h.c:116-157; the binning, powermodule.c:23-81; the
+ * normalisations) run threaded, the deposit with atomic adds instead of
+ * the reference's racy ones; everything the reference runs sequentially
+ * stays sequential.  The parity oracle is built without it. */
+#ifdef ORACLE_OMP
+#include <omp.h>
+#define ORA_ATOMIC _Pragma("omp atomic")
+#define ORA_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#ifdef ORACLE_SHARED
+/* Threads of the OpenMP build's parallel loops; returns the count in use. */
+int ora_omp_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+#endif
+#else
+#define ORA_ATOMIC
+#define ORA_PAR_FOR
+#endif
+
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 #ifdef ORACLE_F32
@@ -878,7 +900,7 @@ static void deposit_ngp(REAL *g, const REAL *x, int nd, REAL L, REAL m) {
   REAL Ld = L / nd;
   int xt[3];
   cell_index(x, xt, Ld, nd);
-  g[xt[0] * nd * nd + xt[1] * nd + xt[2]] += m;
+  ORA_ATOMIC g[xt[0] * nd * nd + xt[1] * nd + xt[2]] += m;
 }
 
 static void deposit_cic(REAL *g, const REAL *x, int nd, REAL L, REAL m) {
@@ -899,14 +921,14 @@ static void deposit_cic(REAL *g, const REAL *x, int nd, REAL L, REAL m) {
   int x0 = xt[0], x1 = wrap_int(xt[0], sg[0], nd);
   int y0 = xt[1], y1 = wrap_int(xt[1], sg[1], nd);
   int z0 = xt[2], z1 = wrap_int(xt[2], sg[2], nd);
-  g[x0 * nd * nd + y0 * nd + z0] += m * t[0] * t[1] * t[2];
-  g[x1 * nd * nd + y0 * nd + z0] += m * dx[0] * t[1] * t[2];
-  g[x0 * nd * nd + y1 * nd + z0] += m * t[0] * dx[1] * t[2];
-  g[x0 * nd * nd + y0 * nd + z1] += m * t[0] * t[1] * dx[2];
-  g[x1 * nd * nd + y1 * nd + z0] += m * dx[0] * dx[1] * t[2];
-  g[x1 * nd * nd + y0 * nd + z1] += m * dx[0] * t[1] * dx[2];
-  g[x0 * nd * nd + y1 * nd + z1] += m * t[0] * dx[1] * dx[2];
-  g[x1 * nd * nd + y1 * nd + z1] += m * dx[0] * dx[1] * dx[2];
+  ORA_ATOMIC g[x0 * nd * nd + y0 * nd + z0] += m * t[0] * t[1] * t[2];
+  ORA_ATOMIC g[x1 * nd * nd + y0 * nd + z0] += m * dx[0] * t[1] * t[2];
+  ORA_ATOMIC g[x0 * nd * nd + y1 * nd + z0] += m * t[0] * dx[1] * t[2];
+  ORA_ATOMIC g[x0 * nd * nd + y0 * nd + z1] += m * t[0] * t[1] * dx[2];
+  ORA_ATOMIC g[x1 * nd * nd + y1 * nd + z0] += m * dx[0] * dx[1] * t[2];
+  ORA_ATOMIC g[x1 * nd * nd + y0 * nd + z1] += m * dx[0] * t[1] * dx[2];
+  ORA_ATOMIC g[x0 * nd * nd + y1 * nd + z1] += m * t[0] * dx[1] * dx[2];
+  ORA_ATOMIC g[x1 * nd * nd + y1 * nd + z1] += m * dx[0] * dx[1] * dx[2];
 }
 
 /* Triangular-shaped cloud (not in the reference; textbook definition,
@@ -931,7 +953,7 @@ static void deposit_tsc(REAL *g, const REAL *x, int nd, REAL L, REAL m) {
       for (int e = 0; e < 3; e++) {
         int ix = wrap_int(c[0], a - 1, nd), iy = wrap_int(c[1], b - 1, nd),
             iz = wrap_int(c[2], e - 1, nd);
-        g[(size_t)ix * nd * nd + (size_t)iy * nd + iz] += m * w[0][a] * w[1][b] * w[2][e];
+        ORA_ATOMIC g[(size_t)ix * nd * nd + (size_t)iy * nd + iz] += m * w[0][a] * w[1][b] * w[2][e];
       }
 }
 
@@ -964,12 +986,12 @@ static long double deposit_sphere(REAL *g, const REAL *x, int nd, REAL L, REAL R
         size_t at = (size_t)c[0] * nd * nd + (size_t)c[1] * nd + c[2];
         if (!expw) {
           if (r2 <= R * R) {
-            g[at] += m;
+            ORA_ATOMIC g[at] += m;
             M += (long double)m;
           }
         } else if (r2 <= R * R * R_times * R_times) {
           REAL w = exp(-0.5 * r2 / R / R);
-          g[at] += m * w;
+          ORA_ATOMIC g[at] += m * w;
           M += (long double)m * w;
         }
       }
@@ -1008,11 +1030,15 @@ int FN(ora_tracer_grid)(REAL *grid, int nd, REAL L, int direction, REAL *pos,
   for (int t = 0; t < ntype; t++) M1[t] = M2[t] = 0.0;
   if (direction != -1) {
     REAL Hz = 100.0 * sqrt(Om0 * pow(1.0 + z, 3) + (1.0 - Om0));
+    ORA_PAR_FOR
     for (long long i = 0; i < np; i++)
       pos[3 * i + direction] = wrap_while(
           pos[3 * i + direction], vel[3 * i + direction] * pow(1.0 + z, 3.0) / Hz, L);
   }
   const int nsub = 1 + (interlacing ? 1 : 0);
+#ifdef ORACLE_OMP
+#pragma omp parallel for schedule(static) reduction(+ : M1[:64], M2[:64])
+#endif
   for (long long j = 0; j < np; j++) {
     int t = type ? type[j] : 0;
     REAL m = mass ? mass[j] : 1.0;
@@ -1030,6 +1056,7 @@ int FN(ora_tracer_grid)(REAL *grid, int nd, REAL L, int direction, REAL *pos,
       for (int s = 0; s < nsub; s++) {
         REAL *g = grid + ((size_t)t * nsub + s) * ng;
         long double M = s == 0 ? M1[t] : M2[t];
+        ORA_PAR_FOR
         for (size_t c = 0; c < ng; c++) g[c] = g[c] / (M / ng) - 1.0;
       }
   }
@@ -1090,6 +1117,7 @@ void FN(ora_gridk)(int nd, REAL L, int window, REAL R, const REAL *f1, const REA
   REAL kn = 2 * M_PI / L;
   REAL Lb = L / nd;
   const int nz = nd / 2 + 1;
+  ORA_PAR_FOR
   for (int i = 0; i < nd; i++) {
     REAL kx = (2 * i < nd) ? i * kn : (i - nd) * kn;
     for (int j = 0; j < nd; j++) {
@@ -1128,6 +1156,10 @@ void FN(ora_power_bins)(const REAL *const *gk, int ntype, int nd, REAL L, int Nk
   const int nz = nd / 2 + 1;
   long double *P = (long double *)calloc((size_t)NPs * ls * Nk, sizeof(long double));
   long double *Km = (long double *)calloc((size_t)Nk, sizeof(long double));
+#ifdef ORACLE_OMP
+  const size_t nP = (size_t)NPs * ls * Nk;
+#pragma omp parallel for schedule(static) reduction(+ : P[:nP], Km[:Nk], count[:Nk])
+#endif
   for (int i = 0; i < nd; i++) {
     REAL kx = (2 * i < nd) ? i * kn : (i - nd) * kn;
     for (int j = 0; j < nd; j++) {

@@ -37,7 +37,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_OMP_PATH = os.path.join(_HERE, "liboracle_omp.so")
 _lib = None
+_lib_omp = None
+_use_omp = False
 
 M_MAX = 1e16
 
@@ -49,12 +52,27 @@ def build() -> str:
 
 
 def lib():
-    global _lib
+    global _lib, _lib_omp
+    if _use_omp:
+        if _lib_omp is None:
+            if not os.path.exists(_OMP_PATH):
+                build()
+            _lib_omp = C.CDLL(_OMP_PATH)
+        return _lib_omp
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
         _lib = C.CDLL(_LIB_PATH)
     return _lib
+
+
+def use_openmp(on: bool) -> None:
+    """Route the oracle's calls to the OpenMP build (the CPU baseline: the
+    deposit, deconvolution, binning and normalisation loops threaded as the
+    reference threads them, OMP_NUM_THREADS threads) or to the sequential
+    parity build."""
+    global _use_omp
+    _use_omp = bool(on)
 
 
 def _real(precision: int):

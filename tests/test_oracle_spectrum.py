@@ -344,3 +344,27 @@ def test_bispectrum_equals_fourier_triangle_sums(window):
         P = ng * np.sum(np.abs(Df[m]) ** 2) / (ng * m.sum())
         np.testing.assert_allclose(out["Pk"][0, a], P, rtol=1e-9)
         np.testing.assert_allclose(out["Nk"][a], m.sum(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_openmp_build_equals_the_parity_build(prec):
+    """The CPU baseline's OpenMP build (threaded deposit with atomic adds,
+    deconvolution and binning) gives the sequential build's grids and
+    spectra up to the order of the float sums."""
+    R = np.float32 if prec == 4 else np.float64
+    rng = np.random.default_rng(12)
+    nd, L = 24, 96.0
+    pos = (rng.random((30000, 3)) * L).astype(R)
+    ref_g = O.grid_compute(pos, nd, L, window=2, interlacing=True, precision=prec)
+    ref_p = O.power_compute(ref_g, 1, nd, L, window=2, interlacing=True, Nk=10, precision=prec)
+    O.use_openmp(True)
+    try:
+        assert O.lib().ora_omp_threads(4) == 4
+        g = O.grid_compute(pos, nd, L, window=2, interlacing=True, precision=prec)
+        p = O.power_compute(ref_g, 1, nd, L, window=2, interlacing=True, Nk=10, precision=prec)
+    finally:
+        O.use_openmp(False)
+    tol = 2e-6 if prec == 4 else 1e-13
+    np.testing.assert_allclose(g, ref_g, rtol=0, atol=tol * np.abs(ref_g).max())
+    np.testing.assert_array_equal(p["Nk"], ref_p["Nk"])
+    np.testing.assert_allclose(p["Pk64"], ref_p["Pk64"], rtol=1e-12)
