@@ -591,6 +591,41 @@ int exh_grid_compute(exh_ctx* c, const void* pos, const void* vel, const void* m
   EXH_API_END
 }
 
+int exh_grid_compute_noalloc(void* stream, const void* pos, const void* vel, const void* mass,
+                             const int* type, long long np, int ntype, int nd, double L,
+                             double Om0, double z, int direction, int window, double R,
+                             double R_times, int interlacing, int folds, int precision,
+                             void* grid) {
+  EXH_API_BEGIN
+  EXH_REQUIRE(precision == 4 || precision == 8, EXH_ERR_ARG, "precision must be 4 or 8");
+  EXH_REQUIRE(grid != nullptr && (pos != nullptr || np == 0), EXH_ERR_ARG, "null array");
+  EXH_REQUIRE(np >= 0 && nd > 0 && L > 0, EXH_ERR_ARG, "bad sizes");
+  EXH_REQUIRE(ntype >= 1 && ntype <= 64, EXH_ERR_ARG,
+              "the allocation-free deposit takes 1 to 64 tracer types (got %d)", ntype);
+  EXH_REQUIRE(ntype * (interlacing ? 2 : 1) <= NOALLOC_GRID_WORDS, EXH_ERR_ARG,
+              "too many grids for the allocation-free deposit's scratch");
+  EXH_REQUIRE(ntype == 1 || type != nullptr, EXH_ERR_ARG, "ntype > 1 needs a type array");
+  EXH_REQUIRE(direction >= -1 && direction <= 2, EXH_ERR_ARG, "direction must be -1, 0, 1 or 2");
+  int dev = 0;
+  EXH_CHECK_HIP(hipGetDevice(&dev));
+  void* scr = noalloc_grid_scratch(dev);
+  // a context on the caller's stream that holds no buffers and may not
+  // allocate any (ctx_buf raises)
+  exh_ctx bare;
+  bare.device = dev;
+  bare.stream = (hipStream_t)stream;
+  bare.no_alloc = true;
+  if (precision == 4)
+    run_grid<float>(&bare, (const float*)pos, (const float*)vel, (const float*)mass, type, np,
+                    ntype, nd, L, Om0, z, direction, window, R, R_times, interlacing, folds,
+                    (float*)grid, scr);
+  else
+    run_grid<double>(&bare, (const double*)pos, (const double*)vel, (const double*)mass, type, np,
+                     ntype, nd, L, Om0, z, direction, window, R, R_times, interlacing, folds,
+                     (double*)grid, scr);
+  EXH_API_END
+}
+
 int exh_power_compute(exh_ctx* c, const void* grid, int ntype, int nd, double L, int window,
                       double R, int interlacing, int Nk, double k_min, double k_max, int l_max,
                       int direction, int precision, int kbin_precision, double* k_out,

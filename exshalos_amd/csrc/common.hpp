@@ -159,6 +159,9 @@ struct exh_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_side_fork = nullptr, ev_side_join = nullptr;
   bool own_stream = false;
+  // set for the calls that must not allocate: ctx_buf and ctx_pinned then
+  // raise instead of allocating
+  bool no_alloc = false;
   void* buf[exh::BUF_NSLOTS] = {};
   size_t cap[exh::BUF_NSLOTS] = {};
   // allocation number of each slot's current buffer (0: none yet)

@@ -49,10 +49,19 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n,
                  double floor = -std::numeric_limits<double>::infinity());
 
 // spectrum.hip
+// scratch_dev: device memory of NOALLOC_GRID_WORDS words for the deposit's
+// counters instead of the context's buffer (the allocation-free entry point)
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
-              int window, double R, double R_times, int interlacing, int folds, T* grid);
+              int window, double R, double R_times, int interlacing, int folds, T* grid,
+              void* scratch_dev = nullptr);
+// The deposit scratch of the allocation-free entry point on `device` (which
+// must be current): the address of a static device array, looked up once
+// per device.
+constexpr int NOALLOC_GRID_WORDS = 128;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void* noalloc_grid_scratch(int device);
 template <typename T>
 void run_power(exh_ctx* c, const T* grid, int ntype, int nd, double L, int window, double R,
                int interlacing, int Nk, double k_min, double k_max, int l_max, int direction,

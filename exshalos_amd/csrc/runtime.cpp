@@ -27,6 +27,8 @@ const char* last_error() { return g_err; }
 void* ctx_buf(exh_ctx* c, Slot s, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (c->cap[s] < bytes) {
+    EXH_REQUIRE(!c->no_alloc, EXH_ERR_INTERNAL,
+                "an allocation-free call needs a device buffer (slot %d)", (int)s);
     if (c->buf[s]) {
       EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
       EXH_CHECK_HIP(hipFree(c->buf[s]));
@@ -43,6 +45,8 @@ void* ctx_buf(exh_ctx* c, Slot s, size_t bytes) {
 
 void* ctx_pinned(exh_ctx* c, size_t bytes) {
   if (c->pinned_cap < bytes) {
+    EXH_REQUIRE(!c->no_alloc, EXH_ERR_INTERNAL,
+                "an allocation-free call needs pinned host memory");
     if (c->pinned) EXH_CHECK_HIP(hipHostFree(c->pinned));
     c->pinned = nullptr;
     size_t want = (bytes + 4095) & ~(size_t)4095;

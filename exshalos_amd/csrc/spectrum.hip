@@ -26,6 +26,7 @@
 #include <array>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 namespace exh {
 
@@ -731,10 +732,39 @@ static int deposit_window_margin() {
   return v;
 }
 
+// Scratch of the allocation-free deposit (exh_grid_compute_noalloc): the
+// per-type particle counts (64 words) or the per-grid deposited weights
+// (128 doubles) that the normalisation reads.  Each device holds its own
+// copy; host code takes its address once per device from
+// hipGetSymbolAddress and never uses the array's name.  Calls that
+// overlap on one device share it and race.
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_GRID_WORDS];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+void* noalloc_grid_scratch(int device) {
+  EXH_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES, EXH_ERR_ARG,
+              "device %d out of range for the allocation-free entry point", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    EXH_CHECK_HIP(hipGetDevice(&cur));
+    EXH_REQUIRE(cur == device, EXH_ERR_INTERNAL, "device %d is not current", device);
+    void* p = nullptr;
+    EXH_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    EXH_REQUIRE(p != nullptr, EXH_ERR_HIP, "no address for the deposit scratch on device %d",
+                device);
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
-              int window, double R, double R_times, int interlacing, int folds, T* grid) {
+              int window, double R, double R_times, int interlacing, int folds, T* grid,
+              void* scratch_dev) {
   EXH_REQUIRE(window >= EXH_WINDOW_NGP && window <= EXH_WINDOW_TSC, EXH_ERR_UNSUPPORTED,
               "window %d: NGP (1), CIC (2), sphere (3), exponential (4) or TSC (5)", window);
   EXH_REQUIRE(window < 3 || window > 4 || (R > 0 && (window == 3 || R_times > 0)), EXH_ERR_ARG,
@@ -748,7 +778,8 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   // reference: vel * (1+z)^3 / H(Om0, z) (gridmodule.c:167-169)
   T Hz = (T)(100.0 * std::sqrt((T)Om0 * std::pow(1.0 + (T)z, 3) + (1.0 - (T)Om0)));
   T rsd = (T)(std::pow(1.0 + (T)z, 3.0) / (double)Hz);
-  unsigned long long* counts = (unsigned long long*)ctx_buf(c, BUF_GCNT, 64 * 8);
+  unsigned long long* counts = scratch_dev ? (unsigned long long*)scratch_dev
+                                           : (unsigned long long*)ctx_buf(c, BUF_GCNT, 64 * 8);
   EXH_REQUIRE(ntype <= 64, EXH_ERR_ARG, "at most 64 tracer types");
   {
     TimedScope ts(c, "grid_zero");
@@ -766,7 +797,7 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   // in lattice order, as LPT particles are)
   if (window == 3 || window == 4) {
     // spherical / exponential windows: normalised by the deposited weight
-    double* wsum = (double*)ctx_buf(c, BUF_GCNT, 128 * 8);
+    double* wsum = scratch_dev ? (double*)scratch_dev : (double*)ctx_buf(c, BUF_GCNT, 128 * 8);
     hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(128), 0, c->stream, wsum, (size_t)128);
     {
       TimedScope ts(c, window == 3 ? "sphere_deposit" : "exp_deposit");
@@ -1969,10 +2000,10 @@ template int run_bispectrum<double>(exh_ctx*, const double*, int, int, double, i
 
 template void run_grid<float>(exh_ctx*, const float*, const float*, const float*, const int*,
                               long long, int, int, double, double, double, int, int, double,
-                              double, int, int, float*);
+                              double, int, int, float*, void*);
 template void run_grid<double>(exh_ctx*, const double*, const double*, const double*, const int*,
                                long long, int, int, double, double, double, int, int, double,
-                               double, int, int, double*);
+                               double, int, int, double*, void*);
 template void run_power<float>(exh_ctx*, const float*, int, int, double, int, double, int, int,
                                double, double, int, int, int, double*, double*, long long*, bool);
 template void run_power<double>(exh_ctx*, const double*, int, int, double, int, double, int, int,

@@ -94,6 +94,10 @@ _SIGS = {
     "exh_grid_compute": (C.c_int, [_P, _P, _P, _P, _P, C.c_longlong, C.c_int, C.c_int,
                                    C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _P]),
+    "exh_grid_compute_noalloc": (C.c_int, [_P, _P, _P, _P, _P, C.c_longlong, C.c_int, C.c_int,
+                                           C.c_double, C.c_double, C.c_double, C.c_int,
+                                           C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                           C.c_int, _P]),
     "exh_power_compute": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
                                     C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                     C.c_int, C.c_int, _P, _P, _P]),

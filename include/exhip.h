@@ -241,6 +241,22 @@ int exh_grid_compute(exh_ctx* ctx, const void* pos, const void* vel, const void*
                      double Om0, double z, int direction, int window, double R,
                      double R_times, int interlacing, int folds, int precision, void* grid);
 
+/* Mass assignment that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as exh_grid_compute,
+ * without a context: the work is queued on `stream` (a hipStream_t; NULL:
+ * the null stream of the current device) and not waited for.  The
+ * deposit's counters (particles per type, or deposited weight per grid)
+ * live in a static 1 KiB device array, one per GPU, kept until the process
+ * ends and cleared by every call; calls that overlap on one GPU share it
+ * and race.  Limits: ntype <= 64 (EXH_ERR_ARG beyond). */
+int exh_grid_compute_noalloc(void* stream, const void* pos, const void* vel, const void* mass,
+                             const int* type, long long np, int ntype, int nd, double L,
+                             double Om0, double z, int direction, int window, double R,
+                             double R_times, int interlacing, int folds, int precision,
+                             void* grid);
+
 /* Power spectra of all tracer pairs: replaces power_compute
  * (spectrum.c:111-197) / Compute_gridk + Power_Spectrum
  * (spectrum_h.c:83-167, powermodule.c:4-108).  grid: device
