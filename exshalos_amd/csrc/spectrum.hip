@@ -496,38 +496,78 @@ __global__ __launch_bounds__(TILE_THREADS) void k_deposit_tiles(
 // rare stencil that leaves the window goes straight to the grid.  The
 // window is flushed row by row, lanes along z, one global atomic per
 // touched cell.
+//
+// A tile is PX x PY x PZ particles of the lattice and the window holds
+// ACC (float: ds_add_f32, or double: ds_add_f64) sums.  The flush's global
+// atomics scale with the window's cells per grid cell, (W + 2h)^3 / W^3 for
+// cubic tiles, so larger tiles in float windows issue fewer of them.
 static constexpr int WIN_THREADS = 1024;
 static constexpr int WIN_HALO = 4;  // default margin h
-static constexpr int WIN_PPT = TILE_L * TILE_L * TILE_L / WIN_THREADS;
 
-template <typename T, int WIN>
+// Window sums: float or double (LDS float/double atomic adds), or unsigned
+// 32-bit fixed point with 22 fraction bits (ds_add_u32; integer sums are
+// exact, so the window does not depend on the order of the adds).  Fixed
+// point serves unweighted deposits only, whose weights lie in [0, 1]: a
+// cell holds 1023 particles' weight before the sum would wrap.  Every add
+// checks the value it replaced, so the add that wraps is seen; its tile
+// then deposits straight to the grid instead of through the window.
+template <typename ACC>
+struct WinAcc {
+  template <typename T>
+  __device__ static ACC to(T v) {
+    return (ACC)v;
+  }
+  __device__ static double from(ACC a) { return (double)a; }
+  // add v to *p; true if the sum wrapped
+  __device__ static bool add(ACC* p, ACC v) {
+    atomicAdd(p, v);
+    return false;
+  }
+};
+template <>
+struct WinAcc<unsigned> {
+  template <typename T>
+  __device__ static unsigned to(T v) {
+    return (unsigned)((float)v * 4194304.0f + 0.5f);
+  }
+  __device__ static double from(unsigned a) { return (double)a * (1.0 / 4194304.0); }
+  __device__ static bool add(unsigned* p, unsigned v) {
+    const unsigned old = atomicAdd(p, v);
+    return old + v < old;
+  }
+};
+
+template <typename T, int WIN, typename ACC, int PX, int PY, int PZ>
 __device__ __forceinline__ void deposit_window_tile(
-    int tile, const T* pos, const T* vel, const T* mass, int nL, int nd, int W, int wh, T L, T Lb,
-    int direction, T rsd, int nsub, T* grid, unsigned long long* counts) {
+    int tile, const T* pos, const T* vel, const T* mass, int nL, int nd, int Wx, int Wy, int Wz,
+    int wh, T L, T Lb, int direction, T rsd, int nsub, T* grid, unsigned long long* counts) {
   constexpr int NW = WIN == 1 ? 1 : (WIN == 2 ? 2 : 3);
   constexpr int NWAVE = WIN_THREADS / 64;
+  constexpr int PPT = PX * PY * PZ / WIN_THREADS;
+  static_assert(PPT * WIN_THREADS == PX * PY * PZ, "tile must be a multiple of the block");
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  double* box = (double*)lds_raw;
+  ACC* box = (ACC*)lds_raw;
   __shared__ float samp[3][NWAVE];
   __shared__ int wo_s[3];
   __shared__ unsigned long long nvalid;
   const size_t ng = (size_t)nd * nd * nd;
   const T Ld = L / (T)nd;
-  const int nt = nL / TILE_L;
-  const int E = W + 2 * wh, sx = E * E;
-  const int tz = tile % nt, ty = (tile / nt) % nt, tx = tile / (nt * nt);
+  const int nty = nL / PY, ntz = nL / PZ;
+  const int W[3] = {Wx, Wy, Wz};
+  const int Ex = Wx + 2 * wh, Ey = Wy + 2 * wh, Ez = Wz + 2 * wh, sx = Ey * Ez;
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
   // tile centre in grid cells: the middle of the grid tile of the same index
-  const int qc[3] = {tx * W + W / 2, ty * W + W / 2, tz * W + W / 2};
+  const int qc[3] = {tx * Wx + Wx / 2, ty * Wy + Wy / 2, tz * Wz + Wz / 2};
   if (threadIdx.x == 0) nvalid = 0;
-  T xs[WIN_PPT][3];
-  T ms[WIN_PPT];
-  bool ok[WIN_PPT];
+  T xs[PPT][3];
+  T ms[PPT];
+  bool ok[PPT];
   unsigned long long myvalid = 0;
 #pragma unroll
-  for (int p = 0; p < WIN_PPT; p++) {
+  for (int p = 0; p < PPT; p++) {
     const int lin = threadIdx.x + p * WIN_THREADS;
-    const int c = lin % TILE_L, b = (lin / TILE_L) % TILE_L, a = lin / (TILE_L * TILE_L);
-    const size_t j = ((size_t)(tx * TILE_L + a) * nL + (ty * TILE_L + b)) * nL + (tz * TILE_L + c);
+    const int c = lin % PZ, b = (lin / PZ) % PY, a = lin / (PZ * PY);
+    const size_t j = ((size_t)(tx * PX + a) * nL + (ty * PY + b)) * nL + (tz * PZ + c);
     T x[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
     if (direction >= 0) x[direction] = wrap_while<T>(x[direction] + vel[3 * j + direction] * rsd, L);
     ok[p] = isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
@@ -540,7 +580,7 @@ __device__ __forceinline__ void deposit_window_tile(
   // to the tile centre minus its lattice point's offset from the centre
   if ((threadIdx.x & 63) == 0) {
     const int lin = threadIdx.x;
-    const int lat[3] = {lin / (TILE_L * TILE_L), (lin / TILE_L) % TILE_L, lin % TILE_L};
+    const int lat[3] = {lin / (PZ * PY), (lin / PZ) % PY, lin % PZ};
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       float r = 1e30f;
@@ -548,7 +588,7 @@ __device__ __forceinline__ void deposit_window_tile(
         int cc = (int)floor(wrap_while<T>(xs[0][q], L) / Ld);
         cc = cc < nd ? cc : nd - 1;
         r = (float)rel_cell(cc, qc[q], nd) -
-            ((lat[q] + 0.5f) * (float)nd / (float)nL - 0.5f * (float)W);
+            ((lat[q] + 0.5f) * (float)nd / (float)nL - 0.5f * (float)W[q]);
       }
       samp[q][threadIdx.x >> 6] = r;
     }
@@ -565,18 +605,20 @@ __device__ __forceinline__ void deposit_window_tile(
       }
     }
     const float mean = n ? sum / (float)n : 0.f;
-    wo_s[threadIdx.x] = (int)floorf(mean + 0.5f) - W / 2 - wh;
+    wo_s[threadIdx.x] = (int)floorf(mean + 0.5f) - W[threadIdx.x] / 2 - wh;
   }
   __syncthreads();
   const int wo[3] = {wo_s[0], wo_s[1], wo_s[2]};
-  const int vol = E * sx;
+  const int E[3] = {Ex, Ey, Ez};
+  const int vol = Ex * sx;
   const T shift = (T)(-(double)Lb / 2.0);
   for (int s = 0; s < nsub; s++) {
     T* g = grid + (size_t)s * ng;
-    for (int v = threadIdx.x; v < vol; v += WIN_THREADS) box[v] = 0.0;
+    for (int v = threadIdx.x; v < vol; v += WIN_THREADS) box[v] = (ACC)0;
     __syncthreads();
+    bool wrapped = false;
 #pragma unroll
-    for (int p = 0; p < WIN_PPT; p++) {
+    for (int p = 0; p < PPT; p++) {
       if (!ok[p]) continue;
       int cell[3][3], off[3][3];
       T w[3][3];
@@ -585,11 +627,11 @@ __device__ __forceinline__ void deposit_window_tile(
       for (int q = 0; q < 3; q++) {
         const T pa = wrap_while<T>(s == 0 ? xs[p][q] : xs[p][q] + shift, L);
         stencil_q<T, WIN>(scale_q<T, WIN>(pa, Ld), nd, cell[q], w[q]);
-        const int st = q == 0 ? sx : (q == 1 ? E : 1);
+        const int st = q == 0 ? sx : (q == 1 ? Ez : 1);
 #pragma unroll
         for (int u = 0; u < NW; u++) {
           const int l = rel_cell(cell[q][u], qc[q], nd) - wo[q];
-          in = in && (unsigned)l < (unsigned)E;
+          in = in && (unsigned)l < (unsigned)E[q];
           off[q][u] = l * st;
         }
       }
@@ -601,7 +643,8 @@ __device__ __forceinline__ void deposit_window_tile(
             const T wab = ms[p] * w[0][u] * w[1][b];
             const int o = off[0][u] + off[1][b];
 #pragma unroll
-            for (int c = 0; c < NW; c++) atomicAdd(&box[o + off[2][c]], (double)(wab * w[2][c]));
+            for (int c = 0; c < NW; c++)
+              wrapped |= WinAcc<ACC>::add(&box[o + off[2][c]], WinAcc<ACC>::to(wab * w[2][c]));
           }
         }
       } else {
@@ -614,18 +657,48 @@ __device__ __forceinline__ void deposit_window_tile(
           }
       }
     }
-    __syncthreads();
+    if (__syncthreads_or(wrapped)) {
+      // a fixed-point window cell wrapped: the window is void; every
+      // in-window stencil of the tile goes to the grid directly instead
+      // (the out-of-window ones already did)
+#pragma unroll
+      for (int p = 0; p < PPT; p++) {
+        if (!ok[p]) continue;
+        int cell[3][3];
+        T w[3][3];
+        bool in = true;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const T pa = wrap_while<T>(s == 0 ? xs[p][q] : xs[p][q] + shift, L);
+          stencil_q<T, WIN>(scale_q<T, WIN>(pa, Ld), nd, cell[q], w[q]);
+          for (int u = 0; u < NW; u++) {
+            const int l = rel_cell(cell[q][u], qc[q], nd) - wo[q];
+            in = in && (unsigned)l < (unsigned)E[q];
+          }
+        }
+        if (!in) continue;
+        for (int u = 0; u < NW; u++)
+          for (int b = 0; b < NW; b++) {
+            const T wab = ms[p] * w[0][u] * w[1][b];
+            for (int c = 0; c < NW; c++)
+              atomic_add_t(g + ((size_t)cell[0][u] * nd + cell[1][b]) * nd + cell[2][c],
+                           wab * w[2][c]);
+          }
+      }
+      continue;
+    }
     // flush: (y, z) rows over the threads, z fastest in 32-lane groups
     const int lz0 = threadIdx.x & 31;
-    for (int ly = threadIdx.x >> 5; ly < E; ly += WIN_THREADS / 32) {
+    for (int ly = threadIdx.x >> 5; ly < Ey; ly += WIN_THREADS / 32) {
       const int cy = wrap_cell(qc[1] + wo[1] + ly, nd);
-      for (int lz = lz0; lz < E; lz += 32) {
+      for (int lz = lz0; lz < Ez; lz += 32) {
         const int cz = wrap_cell(qc[2] + wo[2] + lz, nd);
-        const double* bp = box + ly * E + lz;
+        const ACC* bp = box + ly * Ez + lz;
         int cx = wrap_cell(qc[0] + wo[0], nd);
-        for (int lx = 0; lx < E; lx++) {
-          const double val = bp[lx * sx];
-          if (val != 0.0) atomic_add_t(g + ((size_t)cx * nd + cy) * nd + cz, (T)val);
+        for (int lx = 0; lx < Ex; lx++) {
+          const ACC val = bp[lx * sx];
+          if (val != (ACC)0)
+            atomic_add_t(g + ((size_t)cx * nd + cy) * nd + cz, (T)WinAcc<ACC>::from(val));
           cx = cx + 1 == nd ? 0 : cx + 1;
         }
       }
@@ -641,13 +714,13 @@ __device__ __forceinline__ void deposit_window_tile(
 
 // One workgroup per tile, or (EXH_DEPOSIT_WG) fewer workgroups that walk the
 // tiles, leaving CUs to the kernels of other streams.
-template <typename T, int WIN>
+template <typename T, int WIN, typename ACC, int PX, int PY, int PZ>
 __global__ __launch_bounds__(WIN_THREADS) void k_deposit_window(
-    const T* pos, const T* vel, const T* mass, int nL, int nd, int W, int wh, T L, T Lb,
-    int direction, T rsd, int nsub, T* grid, unsigned long long* counts, int ntiles) {
+    const T* pos, const T* vel, const T* mass, int nL, int nd, int Wx, int Wy, int Wz, int wh,
+    T L, T Lb, int direction, T rsd, int nsub, T* grid, unsigned long long* counts, int ntiles) {
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    deposit_window_tile<T, WIN>(tile, pos, vel, mass, nL, nd, W, wh, L, Lb, direction, rsd, nsub,
-                                grid, counts);
+    deposit_window_tile<T, WIN, ACC, PX, PY, PZ>(tile, pos, vel, mass, nL, nd, Wx, Wy, Wz, wh, L,
+                                                 Lb, direction, rsd, nsub, grid, counts);
     __syncthreads();
   }
 }
@@ -760,6 +833,89 @@ void* noalloc_grid_scratch(int device) {
   return g_scratch_ptr[device];
 }
 
+// Tile shape of the window deposit (EXH_DEPOSIT_TILE: 3216 = 32 x 16 x 16
+// particles, the default; 32 = 32 x 32 x 16; 16 = 16^3) and its
+// accumulator (EXH_DEPOSIT_ACC: 1 fixed point, the default for float grids
+// without masses; 8 double; 4 float).  At nd = nL = 512 (CIC, interlaced),
+// alone: 3.18 ms for 32 x 16 x 16 in fixed point, 3.06 ms for 32 x 32 x 16,
+// 3.63 ms for 16^3, 3.89 ms for 16^3 in double, 12.6 ms for 16^3 in float
+// (ds_add_f32 runs far slower than ds_add_u32 or ds_add_f64 here).  Beside
+// the halo finder's sort the 32 x 32 x 16 workgroups (153 KB of LDS each)
+// hold CUs long enough to slow the sort by 2.6 ms; 32 x 16 x 16 does not.
+struct WinShape {
+  int px, py, pz;
+};
+static WinShape deposit_window_shape() {
+  static const WinShape v = [] {
+    const char* e = std::getenv("EXH_DEPOSIT_TILE");
+    const int t = e ? std::atoi(e) : 3216;
+    if (t == 32) return WinShape{32, 32, 16};
+    if (t == 3216) return WinShape{32, 16, 16};
+    return WinShape{16, 16, 16};
+  }();
+  return v;
+}
+static int deposit_window_acc() {
+  static const int v = [] {
+    const char* e = std::getenv("EXH_DEPOSIT_ACC");
+    const int a = e ? std::atoi(e) : 1;
+    return (a == 4 || a == 1) ? a : 8;
+  }();
+  return v;
+}
+// LDS of the window (the kernel's static shared memory takes < 1 KB of the
+// 160 KB)
+static constexpr int WIN_LDS_MAX = 159 * 1024;
+
+template <typename T>
+struct DepositWinArgs {
+  const T *pos, *vel, *mass;
+  int nL, nd, Wx, Wy, Wz, wh;
+  T L, Lb;
+  int direction;
+  T rsd;
+  int nsub;
+  T* grid;
+  unsigned long long* counts;
+  int ntiles;
+};
+
+template <typename T, int WN, typename ACC, int PX, int PY, int PZ>
+static void launch_window(exh_ctx* c, unsigned nwg, int lds, const DepositWinArgs<T>& a) {
+  auto k = k_deposit_window<T, WN, ACC, PX, PY, PZ>;
+  EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    lds));
+  hipLaunchKernelGGL(k, dim3(nwg), dim3(WIN_THREADS), lds, c->stream, a.pos, a.vel, a.mass, a.nL,
+                     a.nd, a.Wx, a.Wy, a.Wz, a.wh, a.L, a.Lb, a.direction, a.rsd, a.nsub, a.grid,
+                     a.counts, a.ntiles);
+}
+
+template <typename T, int WN>
+static void launch_window_shape(exh_ctx* c, int acc, const WinShape& ws, unsigned nwg, int lds,
+                                const DepositWinArgs<T>& a) {
+  if (ws.px == 32 && ws.py == 32) {
+    if (acc == 1) launch_window<T, WN, unsigned, 32, 32, 16>(c, nwg, lds, a);
+    else if (acc == 4) launch_window<T, WN, float, 32, 32, 16>(c, nwg, lds, a);
+    else launch_window<T, WN, double, 32, 32, 16>(c, nwg, lds, a);
+  } else if (ws.px == 32) {
+    if (acc == 1) launch_window<T, WN, unsigned, 32, 16, 16>(c, nwg, lds, a);
+    else if (acc == 4) launch_window<T, WN, float, 32, 16, 16>(c, nwg, lds, a);
+    else launch_window<T, WN, double, 32, 16, 16>(c, nwg, lds, a);
+  } else {
+    if (acc == 1) launch_window<T, WN, unsigned, 16, 16, 16>(c, nwg, lds, a);
+    else if (acc == 4) launch_window<T, WN, float, 16, 16, 16>(c, nwg, lds, a);
+    else launch_window<T, WN, double, 16, 16, 16>(c, nwg, lds, a);
+  }
+}
+
+template <typename T>
+static void launch_window_any(exh_ctx* c, int window, int acc, const WinShape& ws, unsigned nwg,
+                              int lds, const DepositWinArgs<T>& a) {
+  if (window == 1) launch_window_shape<T, 1>(c, acc, ws, nwg, lds, a);
+  if (window == 2) launch_window_shape<T, 2>(c, acc, ws, nwg, lds, a);
+  if (window == 5) launch_window_shape<T, 5>(c, acc, ws, nwg, lds, a);
+}
+
 template <typename T>
 void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* type,
               long long np, int ntype, int nd, double L, double Om0, double z, int direction,
@@ -827,26 +983,48 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
     TimedScope ts(c, window == 2 ? "cic_deposit" : (window == 1 ? "ngp_deposit" : "tsc_deposit"));
     const int W = tiles ? nd / nt : 0;
     const int wh = deposit_window_margin();
-    const int E = W + 2 * wh;
-    const int win_lds = E * E * E * (int)sizeof(double);
-    // nd >= E keeps every window cell within (-nd, 2nd), where the flush's
-    // single-step wrap is exact
-    if (tiles && nd >= E && win_lds <= deposit_lds_bytes() && deposit_window_enabled()) {
-      const unsigned nblk = (unsigned)nt * nt * nt;
+    // the window's tile shape and accumulator (EXH_DEPOSIT_TILE,
+    // EXH_DEPOSIT_ACC), where the lattice and the grid divide into it and
+    // the window fits the LDS; nd >= E keeps every window cell within
+    // (-nd, 2nd), where the flush's single-step wrap is exact
+    // (float windows for float grids only; the default shape when the
+    // chosen one does not fit)
+    WinShape ws = deposit_window_shape();
+    int acc = sizeof(T) == 8 ? 8 : deposit_window_acc();
+    if (acc == 1 && mass) acc = 8;  // fixed point holds weights in [0, 1] only
+    int Wq[3] = {0, 0, 0}, win_lds = 0;
+    auto fits = [&](const WinShape& w, int a) {
+      if (!(tiles && deposit_window_enabled() && nL % w.px == 0 && nL % w.py == 0 &&
+            nL % w.pz == 0))
+        return false;
+      const int pq[3] = {w.px, w.py, w.pz};
+      long long cells = 1;
+      for (int q = 0; q < 3; q++) {
+        const int ntq = nL / pq[q];
+        if (nd % ntq != 0) return false;
+        Wq[q] = nd / ntq;
+        if (nd < Wq[q] + 2 * wh) return false;
+        cells *= Wq[q] + 2 * wh;
+      }
+      win_lds = (int)std::min<long long>(cells * (a == 1 ? 4 : a), 1 << 30);
+      return win_lds <= WIN_LDS_MAX;
+    };
+    bool win = fits(ws, acc);
+    if (!win && (ws.px != 16 || ws.py != 16 || ws.pz != 16)) {
+      ws = WinShape{16, 16, 16};
+      win = fits(ws, acc);
+    }
+    if (!win && acc != 8) {
+      acc = 8;
+      win = fits(ws, acc);
+    }
+    if (win) {
+      const unsigned nblk = (unsigned)((nL / ws.px) * (nL / ws.py) * (nL / ws.pz));
       const unsigned nwg = deposit_workgroups() > 0 ? std::min(nblk, (unsigned)deposit_workgroups())
                                                     : nblk;
-#define EXH_DEPW(WN)                                                                             \
-  do {                                                                                           \
-    EXH_CHECK_HIP(hipFuncSetAttribute((const void*)k_deposit_window<T, WN>,                      \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, win_lds));     \
-    hipLaunchKernelGGL((k_deposit_window<T, WN>), dim3(nwg), dim3(WIN_THREADS), win_lds,         \
-                       c->stream, pos, vel, mass, nL, nd, W, wh, Lf, Lb, direction, rsd, nsub,   \
-                       grid, counts, (int)nblk);                                                 \
-  } while (0)
-      if (window == 1) EXH_DEPW(1);
-      if (window == 2) EXH_DEPW(2);
-      if (window == 5) EXH_DEPW(5);
-#undef EXH_DEPW
+      const DepositWinArgs<T> da{pos, vel, mass, nL, nd, Wq[0], Wq[1], Wq[2], wh, Lf, Lb,
+                                 direction, rsd, nsub, grid, counts, (int)nblk};
+      launch_window_any<T>(c, window, acc, ws, nwg, win_lds, da);
     } else if (tiles) {
       const int lds = deposit_lds_bytes();
       const int cap = lds / (int)sizeof(double);

@@ -551,3 +551,25 @@ def test_bispectrum(prec, ntype, il, window):
     assert _rel(out["Pk"], ref["Pk"]) < tol
     assert out["Bk"].shape == ref["Bk"].shape
     assert _rel(out["Bk"], ref["Bk"]) < tol
+
+
+@pytest.mark.parametrize("window", [2, 5])
+@pytest.mark.parametrize("nL,nd", [(32, 32), (64, 64)])
+def test_grid_compute_lattice_dense_clump(window, nL, nd):
+    """Lattice-ordered particles whose first tile collapses onto one point:
+    thousands of unit weights land in one window cell, more than the
+    window's 32-bit fixed-point sums hold (1023 particles), so that tile
+    deposits straight to the grid; the grid still equals the oracle's."""
+    set_precision(4)
+    L = 128.0
+    rng = np.random.default_rng(3 + nd + window)
+    q = (np.indices((nL, nL, nL)).reshape(3, -1).T + 0.5) * (L / nL)
+    pos = q + rng.normal(0, 0.5 * L / nd, q.shape)
+    lat = np.indices((nL, nL, nL)).reshape(3, -1).T
+    first = np.all(lat < 16, axis=1)  # the 16^3 corner of the lattice
+    pos[first] = (10.3, 11.7, 9.2) + rng.normal(0, 0.01, (first.sum(), 3))
+    pos = np.mod(pos, L).astype(np.float32)
+    ref = O.grid_compute(pos, nd, L, window=window, interlacing=True, precision=4)
+    g = SP.grid_compute(pos, None, None, 0, None, 1, nd, L, 0.31, 0.0, -1, window, 4.0, 5.0, 1,
+                        1, 0, 1)
+    assert _rel(g, ref) < 2e-5
