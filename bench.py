@@ -165,6 +165,42 @@ def run_cpu_baseline(k, P, nd=256, threads=None):
             "port_1core": {"value": round(nd**3 / tot1 / 1e6, 3), "seconds": round(tot1, 2)}}
 
 
+def run_batch(k, P, reps=3):
+    """The batching workload: 64 realisations, 16 each of nd = 64, 96, 128,
+    192 (Lc = 4, seeds 1..64), each field -> halos -> 2LPT -> CIC of
+    matter and halos -> auto and cross P(k) in 64 bins (FP32 fields, FP64
+    bins), modes resident in HBM.  Run concurrently with the autotuned
+    worker count and tiles, and on one context for comparison; best of
+    `reps` timed runs after a warm-up."""
+    import torch
+    from exshalos_amd import synth
+    from exshalos_amd.batch import BatchRunner, load_table
+
+    sizes = (64, 96, 128, 192)
+    boxes = [(sizes[i // 16], i + 1) for i in range(64)]
+    cells = sum(nd**3 for nd, _ in boxes)
+    noises = [torch.as_tensor(synth.white_noise(nd, s, np.float32), device="cuda")
+              for nd, s in boxes]
+    table = load_table()
+    out = {"boxes": len(boxes), "cells": cells, "unit": "Mcells/s",
+           "tuning": {"workers": table.get("workers"),
+                      "tiles": {n: v.get("deposit_tile")
+                                for n, v in table.get("sizes", {}).items()}}}
+    for name, workers in (("concurrent", None), ("one_context", 1)):
+        r = BatchRunner(k, P, Nk=64, workers=workers)
+        r.run(boxes, noises)
+        best = min(r.run(boxes, noises)[1] for _ in range(reps))
+        res, _ = r.run(boxes, noises)
+        r.close()
+        out[name] = {"workers": r.nworkers, "seconds": round(best, 4),
+                     "value": round(cells / best / 1e6, 1)}
+        print(f"[bench] batch {name}: {r.nworkers} workers, {best * 1e3:.1f} ms, "
+              f"{cells / best / 1e6:.1f} Mcells/s, halos {sum(x.nh for x in res)}", flush=True)
+    del noises
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +212,9 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also copy a buffer of this size once (PMC calibration)")
     ap.add_argument("--cpu-nd", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=1,
+                    help="also time the batching workload (64 boxes, 16 each of nd 64, 96, "
+                         "128, 192; 2-tracer P(k) in 64 bins), 0 to skip")
     ap.add_argument("--api", choices=("fused", "calls"), default="fused",
                     help="fused: exh_halos_box_power (LPT, grid and P(k) beside the halo "
                          "finder on a second stream); calls: halos_box_from_pk + grid_compute "
@@ -295,6 +334,10 @@ def main():
                         "note": "same kernel, three separate calls (no stream overlap), "
                                 "outside the timed region"}
 
+    batch = None
+    if args.batch:
+        batch = run_batch(k, P)
+
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] timing the CPU baseline at nd={args.cpu_nd}", flush=True)
@@ -330,6 +373,7 @@ def main():
         "roofline_isolated": roof_iso,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

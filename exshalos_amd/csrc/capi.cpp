@@ -170,6 +170,23 @@ int exh_ctx_timings(exh_ctx* c, char* names, int names_len, double* ms, int max)
   return n;
 }
 
+int exh_ctx_set_tunable(exh_ctx* c, int key, int value) {
+  EXH_API_BEGIN
+  check_ctx(c);
+  if (key == EXH_TUNE_DEPOSIT_TILE) {
+    EXH_REQUIRE(value == 0 || value == 16 || value == 3216 || value == 32, EXH_ERR_ARG,
+                "deposit tile must be 0, 16, 3216 or 32 (got %d)", value);
+    c->tune_deposit_tile = value;
+  } else if (key == EXH_TUNE_DEPOSIT_ACC) {
+    EXH_REQUIRE(value == 0 || value == 1 || value == 4 || value == 8, EXH_ERR_ARG,
+                "deposit accumulator must be 0, 1, 4 or 8 (got %d)", value);
+    c->tune_deposit_acc = value;
+  } else {
+    EXH_REQUIRE(false, EXH_ERR_ARG, "unknown tunable %d", key);
+  }
+  EXH_API_END
+}
+
 int exh_ctx_halo_stats(exh_ctx* c, long long* out5) {
   EXH_API_BEGIN
   EXH_REQUIRE(c && out5, EXH_ERR_ARG, "null argument");

@@ -162,6 +162,10 @@ struct exh_ctx {
   // set for the calls that must not allocate: ctx_buf and ctx_pinned then
   // raise instead of allocating
   bool no_alloc = false;
+  // per-context tunables (exh_ctx_set_tunable); 0 = the library default
+  // (or its environment override)
+  int tune_deposit_tile = 0;
+  int tune_deposit_acc = 0;
   void* buf[exh::BUF_NSLOTS] = {};
   size_t cap[exh::BUF_NSLOTS] = {};
   // allocation number of each slot's current buffer (0: none yet)

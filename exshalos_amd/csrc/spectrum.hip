@@ -845,13 +845,15 @@ void* noalloc_grid_scratch(int device) {
 struct WinShape {
   int px, py, pz;
 };
+static WinShape tile_shape(int t) {
+  if (t == 32) return WinShape{32, 32, 16};
+  if (t == 3216) return WinShape{32, 16, 16};
+  return WinShape{16, 16, 16};
+}
 static WinShape deposit_window_shape() {
   static const WinShape v = [] {
     const char* e = std::getenv("EXH_DEPOSIT_TILE");
-    const int t = e ? std::atoi(e) : 3216;
-    if (t == 32) return WinShape{32, 32, 16};
-    if (t == 3216) return WinShape{32, 16, 16};
-    return WinShape{16, 16, 16};
+    return tile_shape(e ? std::atoi(e) : 3216);
   }();
   return v;
 }
@@ -989,8 +991,8 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
     // (-nd, 2nd), where the flush's single-step wrap is exact
     // (float windows for float grids only; the default shape when the
     // chosen one does not fit)
-    WinShape ws = deposit_window_shape();
-    int acc = sizeof(T) == 8 ? 8 : deposit_window_acc();
+    WinShape ws = c->tune_deposit_tile ? tile_shape(c->tune_deposit_tile) : deposit_window_shape();
+    int acc = sizeof(T) == 8 ? 8 : (c->tune_deposit_acc ? c->tune_deposit_acc : deposit_window_acc());
     if (acc == 1 && mass) acc = 8;  // fixed point holds weights in [0, 1] only
     int Wq[3] = {0, 0, 0}, win_lds = 0;
     auto fits = [&](const WinShape& w, int a) {

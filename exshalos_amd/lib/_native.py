@@ -28,6 +28,7 @@ ERRORS = {
 }
 
 SRC_MODES, SRC_DELTA, SRC_DELTAK = 0, 1, 2
+TUNE_DEPOSIT_TILE, TUNE_DEPOSIT_ACC = 1, 2
 
 
 class Box(C.Structure):
@@ -66,6 +67,7 @@ _SIGS = {
     "exh_ctx_set_timing": (C.c_int, [_P, C.c_int]),
     "exh_ctx_timings": (C.c_int, [_P, C.c_char_p, C.c_int, _P, C.c_int]),
     "exh_ctx_halo_stats": (C.c_int, [_P, _P]),
+    "exh_ctx_set_tunable": (C.c_int, [_P, C.c_int, C.c_int]),
     "exh_box_init": (C.c_int, [C.POINTER(Box), C.c_int, C.c_int, C.c_int, C.c_double,
                                C.c_double, C.c_double, C.c_double, C.c_int]),
     "exh_density_grid": (C.c_int, [_P, C.POINTER(Box), C.POINTER(FieldSrc), C.c_double, _P, _P]),

@@ -114,6 +114,16 @@ int exh_ctx_synchronize(exh_ctx* ctx);
  * names separated by '\n' in `names`. */
 int exh_ctx_set_timing(exh_ctx* ctx, int enable);
 int exh_ctx_timings(exh_ctx* ctx, char* names, int names_len, double* ms, int max);
+/* Per-context tunables (no reference counterpart): launch shapes the
+ * autotuner (exshalos_amd/autotune.py) picks per grid size; 0 restores the
+ * library's default.  EXH_TUNE_DEPOSIT_TILE: particles per window-deposit
+ * workgroup, 16 (16^3), 3216 (32 x 16 x 16) or 32 (32 x 32 x 16);
+ * EXH_TUNE_DEPOSIT_ACC: the window's sums, 1 (32-bit fixed point, float
+ * grids without masses), 4 (float) or 8 (double).  Results agree to the
+ * deposit's float rounding whatever the setting. */
+#define EXH_TUNE_DEPOSIT_TILE 1
+#define EXH_TUNE_DEPOSIT_ACC 2
+int exh_ctx_set_tunable(exh_ctx* ctx, int key, int value);
 /* Halo-finder statistics of the last run (5 values): peaks, candidates,
  * growth rounds, device-wide sort levels, sort levels that needed pointer
  * doubling. */
