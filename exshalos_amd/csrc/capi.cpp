@@ -52,6 +52,8 @@ static void check_src(const exh_field_src* s) {
 static void check_halo_opts(const exh_halo_opts* o) {
   EXH_REQUIRE(o != nullptr, EXH_ERR_ARG, "null halo options");
   EXH_REQUIRE(o->k && o->P && o->Nk >= 3, EXH_ERR_ARG, "halo finder needs P(k) (Nk >= 3)");
+  EXH_REQUIRE(o->sigma_radii == 0 || (o->sigma_radii >= 4 && o->sigma_radii <= 100000),
+              EXH_ERR_ARG, "sigma_radii must be 0 or 4..100000 (got %d)", o->sigma_radii);
 }
 
 extern "C" {

@@ -725,15 +725,16 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     toff += (bytes + 255) & ~(size_t)255;
     return r;
   };
-  const size_t o_bt = ttake(nBt * sizeof(T)), o_mk = ttake(o->Nk * 8), o_int = ttake(o->Nk * 8),
-               o_mi = ttake(o->Nk * 8), o_minv = ttake(o->Nk * 8), o_sph = ttake(sph.size() * 4),
+  const int Nr = halo_table_radii(o);  // knots of the sigma(M) and Tinker tables
+  const size_t o_bt = ttake(nBt * sizeof(T)), o_mk = ttake(Nr * 8), o_int = ttake(Nr * 8),
+               o_mi = ttake(Nr * 8), o_minv = ttake(Nr * 8), o_sph = ttake(sph.size() * 4),
                o_mats = ttake(mats.size() * 4);
   std::vector<unsigned char> key;
   const double oscal[3] = {o->a, o->beta, o->alpha};
-  const int sizes[2] = {o->Nk, nBt};
+  const int sizes[3] = {o->Nk, nBt, o->sigma_radii};
   key_append(key, b, 1);
   key_append(key, oscal, 3);
-  key_append(key, sizes, 2);
+  key_append(key, sizes, 3);
   key_append(key, o->k, o->Nk);
   key_append(key, o->P, o->Nk);
   const size_t cap0 = c->cap[BUF_HALOTAB];
@@ -741,22 +742,22 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   if (c->cap[BUF_HALOTAB] != cap0) c->halotab_key.clear();  // reallocated
   if (key != c->halotab_key) {
     std::vector<T> sig(std::max(1, (int)std::floor(1e16 / (T)b->Mcell)) + 1), Bt(nBt);
-    std::vector<double> Mk(o->Nk), Int(o->Nk);
+    std::vector<double> Mk(Nr), Int(Nr);
     int Ncells = halo_tables(b, o, sig.data(), Bt.data(), nBt, Mk.data(), Int.data());
     EXH_REQUIRE(Ncells >= 1, Ncells < 0 ? EXH_ERR_RANGE : EXH_ERR_ARG,
                 "sigma(M) table: cell mass %g gives %d cells below M_max", (double)b->Mcell,
                 Ncells);
-    std::vector<double> mI(o->Nk), mInv(o->Nk);
-    spline_second_derivs(Mk.data(), Int.data(), o->Nk, mI.data());
-    spline_second_derivs(Int.data(), Mk.data(), o->Nk, mInv.data());
+    std::vector<double> mI(Nr), mInv(Nr);
+    spline_second_derivs(Mk.data(), Int.data(), Nr, mI.data());
+    spline_second_derivs(Int.data(), Mk.data(), Nr, mInv.data());
     auto up = [&](size_t o2, const void* src, size_t bytes) {
       EXH_CHECK_HIP(hipMemcpyAsync(tabs + o2, src, bytes, hipMemcpyHostToDevice, c->stream));
     };
     up(o_bt, Bt.data(), nBt * sizeof(T));
-    up(o_mk, Mk.data(), o->Nk * 8);
-    up(o_int, Int.data(), o->Nk * 8);
-    up(o_mi, mI.data(), o->Nk * 8);
-    up(o_minv, mInv.data(), o->Nk * 8);
+    up(o_mk, Mk.data(), Nr * 8);
+    up(o_int, Int.data(), Nr * 8);
+    up(o_mi, mI.data(), Nr * 8);
+    up(o_minv, mInv.data(), Nr * 8);
     up(o_sph, sph.data(), sph.size() * 4);
     up(o_mats, mats.data(), mats.size() * 4);
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));  // the host tables are temporaries
@@ -1018,7 +1019,7 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     hipLaunchKernelGGL(k_masses<T>, dim3(grid_for(nh, 256)), dim3(256), 0, c->stream, S.hcount, nh,
                        (const int*)(tabs + o_sph), (int)sph.size(), Ncells, (T)b->Mcell,
                        (const double*)(tabs + o_mk), (const double*)(tabs + o_int),
-                       (const double*)(tabs + o_mi), (const double*)(tabs + o_minv), o->Nk, u,
+                       (const double*)(tabs + o_mi), (const double*)(tabs + o_minv), Nr, u,
                        (T*)S.Mh, derr);
     EXH_LAUNCH_CHECK();
     EXH_CHECK_HIP(hipMemcpyAsync(hbuf, derr, 4, hipMemcpyDeviceToHost, c->stream));

@@ -16,6 +16,11 @@ double spline_eval_host(const double* x, const double* y, const double* m, int n
                         int* err);
 
 // Returns Ncells (>= 0), or -1 when sigma(M) was evaluated outside its knots.
+// Knots of the sigma(M) and Tinker tables: sigma_radii, or Nk for the
+// reference's table.
+inline int halo_table_radii(const exh_halo_opts* o) {
+  return o->sigma_radii > 0 ? o->sigma_radii : o->Nk;
+}
 int halo_tables(const exh_box* b, const exh_halo_opts* o, void* sig_grid, void* Bt, int nB,
                 double* Mknots, double* Int);
 

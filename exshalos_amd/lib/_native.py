@@ -53,6 +53,7 @@ class HaloOpts(C.Structure):
     _fields_ = [
         ("Nmin", C.c_int), ("a", C.c_double), ("beta", C.c_double), ("alpha", C.c_double),
         ("seed", C.c_ulong), ("k", C.c_void_p), ("P", C.c_void_p), ("Nk", C.c_int),
+        ("sigma_radii", C.c_int),
     ]
 
 

@@ -60,7 +60,9 @@ def _modes_src(kh, Ph, noise_t, fixed, phase, R_max=100000.0):
     return s
 
 
-def _halo_opts(kh, Ph, Nmin, a, beta, alpha, seed, prec):
+def _halo_opts(kh, Ph, Nmin, a, beta, alpha, seed, prec, sigma_radii=0):
+    """sigma_radii > 0: the FP64 sigma(R) table over that many radii (an
+    option of this library; 0 is the reference's table, one radius per k)."""
     o = N.HaloOpts()
     R = np_real(prec)
     o.Nmin = int(Nmin)
@@ -69,6 +71,7 @@ def _halo_opts(kh, Ph, Nmin, a, beta, alpha, seed, prec):
     o.k = kh.ctypes.data
     o.P = Ph.ctypes.data
     o.Nk = len(kh)
+    o.sigma_radii = int(sigma_radii)
     return o
 
 
@@ -134,7 +137,7 @@ def _flag(ctx, nd3, device_out):
 
 
 def find_halos(delta, k, P, Lc, Om0, redshift, dc, Nmin, a, beta, alpha, OUT_FLAG, OUT_PROF,
-               verbose, *, device_out=False):
+               verbose, *, device_out=False, sigma_radii=0):
     """Halo catalogue of a density grid (exshalos.c:509-610); Lagrangian
     positions Lc (x + 1/2), mass-draw seed 12345 (exshalos.c:552)."""
     if OUT_PROF:
@@ -147,7 +150,7 @@ def find_halos(delta, k, P, Lc, Om0, redshift, dc, Nmin, a, beta, alpha, OUT_FLA
     nd3 = tuple(int(v) for v in d.shape)
     b = N.box(nd3, Lc, Om0, redshift, dc, prec)
     kh, Ph = host_f64(k), host_f64(P)
-    o = _halo_opts(kh, Ph, Nmin, a, beta, alpha, 12345, prec)
+    o = _halo_opts(kh, Ph, Nmin, a, beta, alpha, 12345, prec, sigma_radii)
     nh = C.c_longlong()
     N.check(N.load().exh_find_halos(ctx, C.byref(b), N.ptr(d), C.byref(o), C.byref(nh)),
             "find_halos")
@@ -193,7 +196,8 @@ def lpt_compute(delta, Lc, Om0, redshift, k_smooth, DO_2LPT, OUT_VEL, Ink, OUT_P
 
 def halos_box_from_pk(k, P, R_max, Ndx, Ndy, Ndz, Lc, seed, k_smooth, Om0, redshift, dc, Nmin,
                       a, beta, alpha, fixed, phase, OUT_DEN, OUT_LPT, OUT_VEL, DO_2LPT, OUT_FLAG,
-                      OUT_PROF, verbose, nthreads, *, noise=None, device_out=False):
+                      OUT_PROF, verbose, nthreads, *, noise=None, device_out=False,
+                      sigma_radii=0):
     """Halos (and particles) from a linear P(k) (exshalos.c:710-940)."""
     import torch
 
@@ -206,7 +210,7 @@ def halos_box_from_pk(k, P, R_max, Ndx, Ndy, Ndz, Lc, seed, k_smooth, Om0, redsh
     kh, Ph = host_f64(k), host_f64(P)
     noise_t = _noise_for(nd3, seed, prec, noise)
     src = _modes_src(kh, Ph, noise_t, fixed, phase, np_real(prec)(R_max))
-    o = _halo_opts(kh, Ph, Nmin, a, beta, alpha, seed, prec)
+    o = _halo_opts(kh, Ph, Nmin, a, beta, alpha, seed, prec, sigma_radii)
     dt = torch.float32 if prec == 4 else torch.float64
     ng = nd3[0] * nd3[1] * nd3[2]
     dev = noise_t.device
@@ -236,7 +240,7 @@ def halos_box_from_pk(k, P, R_max, Ndx, Ndy, Ndz, Lc, seed, k_smooth, Om0, redsh
 
 def halos_box_from_grid(k, P, grid, S, V, Lc, k_smooth, Om0, redshift, dc, Nmin, a, beta, alpha,
                         OUT_LPT, OUT_VEL, DO_2LPT, OUT_FLAG, IN_disp, OUT_PROF, verbose, nthreads,
-                        *, device_out=False):
+                        *, device_out=False, sigma_radii=0):
     """Halos from a density grid (exshalos.c:943-1103); mass-draw seed 1234."""
     import torch
 
@@ -249,7 +253,7 @@ def halos_box_from_grid(k, P, grid, S, V, Lc, k_smooth, Om0, redshift, dc, Nmin,
     nd3 = tuple(int(v) for v in d.shape)
     b = N.box(nd3, Lc, Om0, redshift, dc, prec)
     kh, Ph = host_f64(k), host_f64(P)
-    o = _halo_opts(kh, Ph, Nmin, a, beta, alpha, 1234, prec)
+    o = _halo_opts(kh, Ph, Nmin, a, beta, alpha, 1234, prec, sigma_radii)
     dt = torch.float32 if prec == 4 else torch.float64
     ng = nd3[0] * nd3[1] * nd3[2]
     S_in = to_device(S, R) if IN_disp else None

@@ -3,8 +3,10 @@
 Same signatures and defaults as the reference; the arguments are cast by
 ``check_precision()`` as the reference's wrappers do (mock.py:113-142) and
 passed positionally to ``lib.exshalos``.  Extra keyword-only arguments:
-``noise`` (explicit white-noise modes) and ``device_out`` (torch device
-tensors instead of numpy arrays).
+``noise`` (explicit white-noise modes), ``device_out`` (torch device
+tensors instead of numpy arrays) and ``sigma_radii`` (> 0: the halo masses
+use an FP64 sigma(R) table over that many radii instead of the reference's
+one radius per input k in the grid's precision).
 """
 
 from __future__ import annotations
@@ -50,6 +52,7 @@ def Generate_Halos_Box_from_Pk(
     *,
     noise=None,
     device_out: bool = False,
+    sigma_radii: int = 0,
 ) -> Dict[str, np.ndarray]:
     """Halos in a box from a linear power spectrum (reference mock.py:11-184).
 
@@ -75,7 +78,7 @@ def Generate_Halos_Box_from_Pk(
         Om0, z, delta_c, np.int32(Nmin), a, beta, alpha, np.int32(fixed), phase,
         np.int32(OUT_DEN), np.int32(OUT_LPT), np.int32(OUT_VEL), np.int32(DO_2LPT),
         np.int32(OUT_FLAG), np.int32(OUT_PROF), np.int32(verbose), np.int32(nthreads),
-        noise=noise, device_out=device_out)
+        noise=noise, device_out=device_out, sigma_radii=sigma_radii)
 
 
 def Generate_Halos_Box_from_Grid(
@@ -102,6 +105,7 @@ def Generate_Halos_Box_from_Grid(
     nthreads: int = 1,
     *,
     device_out: bool = False,
+    sigma_radii: int = 0,
 ) -> Dict[str, np.ndarray]:
     """Halos from a density grid (reference mock.py:188-353)."""
     In_disp = S is not None
@@ -124,4 +128,4 @@ def Generate_Halos_Box_from_Grid(
         k, P, grid, S, V, Lc, k_smooth, Om0, z, delta_c, np.int32(Nmin), a, beta, alpha,
         np.int32(OUT_LPT), np.int32(OUT_VEL), np.int32(DO_2LPT), np.int32(OUT_FLAG),
         np.int32(In_disp), np.int32(OUT_PROF), np.int32(verbose), np.int32(nthreads),
-        device_out=device_out)
+        device_out=device_out, sigma_radii=sigma_radii)

@@ -61,6 +61,7 @@ def Find_Halos_from_Grid(
     verbose: bool = False,
     *,
     device_out: bool = False,
+    sigma_radii: int = 0,
 ) -> Dict[str, np.ndarray]:
     """Lagrangian halo catalogue of a density grid (reference utils.py:113-209).
 
@@ -74,7 +75,8 @@ def Find_Halos_from_Grid(
     P = np.asarray(P).astype(R)
     dc = R(np.nan) if delta_c is None else R(delta_c)
     return find_halos(grid, k, P, R(Lc), Om0, R(z), dc, np.int32(Nmin), R(a), R(beta), R(alpha),
-                      np.int32(OUT_FLAG), np.int32(False), verbose, device_out=device_out)
+                      np.int32(OUT_FLAG), np.int32(False), verbose, device_out=device_out,
+                      sigma_radii=sigma_radii)
 
 
 def Displace_LPT(

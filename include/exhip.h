@@ -93,6 +93,9 @@ typedef struct {
   const double* k;     /* host: P(k) used for sigma(M) (find_halos.c:4-44) */
   const double* P;
   int Nk;
+  int sigma_radii;     /* 0: the reference's sigma(R) table, Nk radii in the grid's float
+                          type (find_halos.c:16-30, 493); > 0: that many radii (at least 4),
+                          sigma^2 and the Tinker integral in double */
 } exh_halo_opts;
 
 /* ---------------------------------------------------------------- basics */
@@ -146,7 +149,8 @@ int exh_density_grid(exh_ctx* ctx, const exh_box* box, const exh_field_src* mode
 /* ----------------------------------------------------------------- halos */
 /* Halo tables of find_halos.c:4-93,200-212 computed on the host:
  * sig_grid[Ncells] (T), barrier[nB] (T; entries >= Ncells repeat the last),
- * Mknots[Nk], Int[Nk] (double).  Returns Ncells through *Ncells. */
+ * Mknots[Nr], Int[Nr] (double) with Nr = opts->sigma_radii, or opts->Nk
+ * when that is 0.  Returns Ncells through *Ncells. */
 int exh_halo_tables(const exh_box* box, const exh_halo_opts* opts, void* sig_grid,
                     void* barrier, int nB, double* Mknots, double* Int, int* Ncells);
 

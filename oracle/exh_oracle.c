@@ -341,6 +341,24 @@ static REAL tinker_fh(REAL sigma) { /* find_halos.c:47-71, model 1 */
   return r;
 }
 
+/* The same three in double, for the FP64 sigma(R) table option. */
+static double tophat_W_d(double k, double R) {
+  double s, c, x = k * R;
+  sincos(x, &s, &c);
+  return 3.0 / (x * x) * (s / x - c);
+}
+static double calc_sigma2_d(const double *k, const double *P, int Nk, double R) {
+  double acc = 0.0;
+  for (int i = 0; i < Nk - 1; i++) {
+    const double a = k[i] * tophat_W_d(k[i], R), b = k[i + 1] * tophat_W_d(k[i + 1], R);
+    acc += (k[i + 1] - k[i]) / 2.0 * (P[i] * (a * a) + P[i + 1] * (b * b));
+  }
+  return acc / (2.0 * M_PI * M_PI);
+}
+static double tinker_fh_d(double sigma) {
+  return 0.466 * (pow(sigma / 0.99, -2.06) + pow(sigma, -0.48)) * exp(-1.310 / (sigma * sigma));
+}
+
 static REAL barrier_of(REAL S, int do_eb, REAL dc, REAL a, REAL beta, REAL alpha) {
   REAL r; /* find_halos.c:200-212 */
   if (do_eb == 0)
@@ -357,20 +375,41 @@ static REAL barrier_of(REAL S, int do_eb, REAL dc, REAL a, REAL beta, REAL alpha
  *   find_halos.c:316).
  * Mspl_x/Int (Nr each): knots of the M -> integrated mass function spline.
  * Returns Ncells. */
+/* sigma_radii > 0: the FP64 table option, that many radii with sigma^2
+ * (a trapezoid over the input k) and the Tinker integral in double;
+ * Mknots and Int then hold sigma_radii knots. */
 int FN(ora_halo_tables)(const FN(ora_params) * p, const REAL *K, const REAL *P, int Nk,
                         int do_eb, REAL a, REAL beta, REAL alpha, REAL *Sig_grid,
-                        REAL *Bt, int nB, double *Mknots, double *Int) {
-  const int Nr = Nk;
+                        REAL *Bt, int nB, double *Mknots, double *Int, int sigma_radii) {
+  const int Nr = sigma_radii > 0 ? sigma_radii : Nk;
   const int Ncells = (int)floor(EXH_M_MAX / p->Mcell);
-  REAL *R = (REAL *)malloc(sizeof(REAL) * Nr);
   double *Sig = (double *)malloc(sizeof(double) * Nr);
-  REAL Rmin = (REAL)pow(p->Mcell * 0.9 * 3.0 / (4.0 * M_PI * p->rhomz), 1.0 / 3.0);
-  REAL Rmax = (REAL)pow(EXH_M_MAX * 3.0 / (4.0 * M_PI * p->rhomz), 1.0 / 3.0);
-  for (int i = 0; i < Nr; i++) {
-    R[i] = pow(10, log10(Rmin) + i * (log10(Rmax) - log10(Rmin)) / (Nr - 1));
-    Mknots[i] = (double)4.0 / 3.0 * M_PI * ((REAL)p->rhomz * pow(R[i], 3));
+  double *Rd = (double *)malloc(sizeof(double) * Nr);
+  if (sigma_radii > 0) {
+    double *Kd = (double *)malloc(sizeof(double) * Nk), *Pd = (double *)malloc(sizeof(double) * Nk);
+    for (int i = 0; i < Nk; i++) {
+      Kd[i] = K[i];
+      Pd[i] = P[i];
+    }
+    const double Rmin = pow(p->Mcell * 0.9 * 3.0 / (4.0 * M_PI * p->rhomz), 1.0 / 3.0);
+    const double Rmax = pow(EXH_M_MAX * 3.0 / (4.0 * M_PI * p->rhomz), 1.0 / 3.0);
+    for (int i = 0; i < Nr; i++) {
+      Rd[i] = pow(10, log10(Rmin) + i * (log10(Rmax) - log10(Rmin)) / (Nr - 1));
+      Mknots[i] = 4.0 / 3.0 * M_PI * ((double)p->rhomz * pow(Rd[i], 3.0));
+      Sig[i] = sqrt(calc_sigma2_d(Kd, Pd, Nk, Rd[i]));
+    }
+    free(Kd);
+    free(Pd);
+  } else {
+    REAL Rmin = (REAL)pow(p->Mcell * 0.9 * 3.0 / (4.0 * M_PI * p->rhomz), 1.0 / 3.0);
+    REAL Rmax = (REAL)pow(EXH_M_MAX * 3.0 / (4.0 * M_PI * p->rhomz), 1.0 / 3.0);
+    for (int i = 0; i < Nr; i++) {
+      REAL Ri = pow(10, log10(Rmin) + i * (log10(Rmax) - log10(Rmin)) / (Nr - 1));
+      Rd[i] = Ri;
+      Mknots[i] = (double)4.0 / 3.0 * M_PI * ((REAL)p->rhomz * pow(Ri, 3));
+      Sig[i] = (double)sqrt(calc_sigma2(K, P, Nk, Ri));
+    }
   }
-  for (int i = 0; i < Nr; i++) Sig[i] = (double)sqrt(calc_sigma2(K, P, Nk, R[i]));
   ora_spline sp;
   spline_init(&sp, Mknots, Sig, Nr);
   int err = 0;
@@ -381,16 +420,21 @@ int FN(ora_halo_tables)(const FN(ora_params) * p, const REAL *K, const REAL *P, 
 
   Int[0] = 0.0;
   for (int i = 1; i < Nr; i++) {
-    Int[i] = Int[i - 1] - (double)(log(Sig[i]) - log(Sig[i - 1])) / 2.0 *
-                              (tinker_fh(Sig[i]) / pow(R[i], 3.0) +
-                               tinker_fh(Sig[i - 1]) / pow(R[i - 1], 3.0));
+    if (sigma_radii > 0)
+      Int[i] = Int[i - 1] - (log(Sig[i]) - log(Sig[i - 1])) / 2.0 *
+                                (tinker_fh_d(Sig[i]) / pow(Rd[i], 3.0) +
+                                 tinker_fh_d(Sig[i - 1]) / pow(Rd[i - 1], 3.0));
+    else
+      Int[i] = Int[i - 1] - (double)(log(Sig[i]) - log(Sig[i - 1])) / 2.0 *
+                                (tinker_fh(Sig[i]) / pow((REAL)Rd[i], 3.0) +
+                                 tinker_fh(Sig[i - 1]) / pow((REAL)Rd[i - 1], 3.0));
     if (Int[i] == Int[i - 1]) Int[i] = Int[i - 1] * 1.000001;
   }
   for (int i = 0; i < nB; i++) {
     int c = i < Ncells ? i : Ncells - 1;
     Bt[i] = barrier_of(Sig_grid[c], do_eb, p->dc, a, beta, alpha);
   }
-  free(R);
+  free(Rd);
   free(Sig);
   return err ? -Ncells : Ncells;
 }

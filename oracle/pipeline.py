@@ -268,8 +268,11 @@ def cells_in_spheres(smax=10000):
     return out[:m].copy()
 
 
-def halo_tables(p, k, P, a=1.0, beta=0.0, alpha=0.0, precision=8, nB=None):
-    """sigma^2 per cell count, barrier table, and the mass-function knots."""
+def halo_tables(p, k, P, a=1.0, beta=0.0, alpha=0.0, precision=8, nB=None, sigma_radii=0):
+    """sigma^2 per cell count, barrier table, and the mass-function knots.
+
+    sigma_radii > 0 selects the FP64 table: that many radii, sigma^2 and the
+    Tinker integral in double (the knots then number sigma_radii)."""
     R = _real(precision)
     k = np.ascontiguousarray(k, dtype=R)
     P = np.ascontiguousarray(P, dtype=R)
@@ -280,12 +283,13 @@ def halo_tables(p, k, P, a=1.0, beta=0.0, alpha=0.0, precision=8, nB=None):
         nB = max(Ncells, 1) + 8192
     sig = np.zeros(max(Ncells, 1), dtype=R)
     Bt = np.zeros(nB, dtype=R)
-    Mk = np.zeros(len(k), dtype=np.float64)
-    Int = np.zeros(len(k), dtype=np.float64)
+    Nr = sigma_radii if sigma_radii > 0 else len(k)
+    Mk = np.zeros(Nr, dtype=np.float64)
+    Int = np.zeros(Nr, dtype=np.float64)
     RC = _rc(precision)
     rc = getattr(lib(), "ora_halo_tables" + _sfx(precision))(
         C.byref(p), _ptr(k), _ptr(P), C.c_int(len(k)), C.c_int(do_eb), RC(a), RC(beta),
-        RC(alpha), _ptr(sig), _ptr(Bt), C.c_int(nB), _ptr(Mk), _ptr(Int))
+        RC(alpha), _ptr(sig), _ptr(Bt), C.c_int(nB), _ptr(Mk), _ptr(Int), C.c_int(sigma_radii))
     if rc < 0:
         raise ValueError("sigma(M) spline evaluated outside its knots")
     return dict(Ncells=rc, Sig_grid=sig, Bt=Bt, Mknots=Mk, Int=Int, do_eb=do_eb)
@@ -321,7 +325,7 @@ def taus_uniforms(seed, n):
 
 
 def find_halos(delta, k, P, Lc, Om0=0.31, z=0.0, dc=-1.0, Nmin=1, a=1.0, beta=0.0,
-               alpha=0.0, seed=12345, precision=8, p=None):
+               alpha=0.0, seed=12345, precision=8, p=None, sigma_radii=0):
     """Find_Halos: peaks, Lomuto sort, growth, masses.
 
     Returns a dict with the halo centres ``x`` [nh,3] (cell indices),
@@ -333,7 +337,7 @@ def find_halos(delta, k, P, Lc, Om0=0.31, z=0.0, dc=-1.0, Nmin=1, a=1.0, beta=0.
     nd = delta.shape
     if p is None:
         p = params(nd, Lc, Om0, z, dc, precision)
-    tabs = halo_tables(p, k, P, a, beta, alpha, precision)
+    tabs = halo_tables(p, k, P, a, beta, alpha, precision, sigma_radii=sigma_radii)
     px, pd = find_peaks(delta, precision)
     skey, perm = lomuto_sort(pd, precision)
     spx = np.ascontiguousarray(px[perm])
@@ -355,7 +359,7 @@ def find_halos(delta, k, P, Lc, Om0=0.31, z=0.0, dc=-1.0, Nmin=1, a=1.0, beta=0.
     u = taus_uniforms(seed, nh)
     Mh = np.zeros(max(nh, 1), dtype=R)
     fn2 = getattr(lib(), "ora_halo_masses" + _sfx(precision))
-    err = fn2(C.byref(p), _ptr(tabs["Mknots"]), _ptr(tabs["Int"]), C.c_int(len(k)),
+    err = fn2(C.byref(p), _ptr(tabs["Mknots"]), _ptr(tabs["Int"]), C.c_int(len(tabs["Int"])),
               _ptr(sph), C.c_int(len(sph)), C.c_int(tabs["Ncells"]), _ptr(hc),
               C.c_longlong(nh), _ptr(u), _ptr(Mh))
     return dict(x=hx, count=hc, r2=hr, Mh=Mh[:nh], flag=flag.reshape(nd), peak_x=spx,
@@ -445,7 +449,7 @@ def halo_positions_given(p, flag, S, V, halos, precision=8):
 
 def halos_box_from_grid(delta, k, P, Lc, S=None, V=None, Om0=0.31, z=0.0, k_smooth=1e4,
                         dc=-1.0, Nmin=1, a=1.0, beta=0.0, alpha=0.0, do_2lpt=False,
-                        precision=8):
+                        precision=8, sigma_radii=0):
     """Generate_Halos_Box_from_Grid (box.c:71-150) with the mass-draw seed
     1234 (exshalos.c:987).  With S given (IN_disp) the halos take the means
     of S (and V) over their cells; otherwise the LPT of the grid
@@ -456,7 +460,8 @@ def halos_box_from_grid(delta, k, P, Lc, S=None, V=None, Om0=0.31, z=0.0, k_smoo
     delta = np.ascontiguousarray(delta, dtype=R)
     nd = delta.shape
     p = params(nd, Lc, Om0, z, dc, precision)
-    hal = find_halos(delta, k, P, Lc, Om0, z, dc, Nmin, a, beta, alpha, 1234, precision, p)
+    hal = find_halos(delta, k, P, Lc, Om0, z, dc, Nmin, a, beta, alpha, 1234, precision, p,
+                     sigma_radii)
     out = dict(halos=hal, params=p, Mh=hal["Mh"])
     if S is not None:
         out["posh"], out["velh"] = halo_positions_given(p, hal["flag"], S, V, hal, precision)
@@ -470,7 +475,7 @@ def halos_box_from_grid(delta, k, P, Lc, S=None, V=None, Om0=0.31, z=0.0, k_smoo
 
 def halos_box_from_pk(k, P, nd, Lc, noise, Om0=0.31, z=0.0, k_smooth=1e4, dc=-1.0,
                       Nmin=1, a=1.0, beta=0.0, alpha=0.0, seed=12345, do_2lpt=False,
-                      out_lpt=False, precision=8, R_max=1e5):
+                      out_lpt=False, precision=8, R_max=1e5, sigma_radii=0):
     """Generate_Halos_Box_from_Pk (box.c:4-68) with explicit noise modes.
     R_max < 1e5 truncates the field's P(k) only; the halo tables use the
     P(k) as given (box.c:18-21)."""
@@ -478,7 +483,8 @@ def halos_box_from_pk(k, P, nd, Lc, noise, Om0=0.31, z=0.0, k_smooth=1e4, dc=-1.
     p = params(nd, Lc, Om0, z, dc, precision)
     dk = field_modes(p, k, P, noise, precision, 1e5, R_max=R_max)
     delta = (c2r(dk, tuple(p.nd)).astype(R) * R(p.Normx)).astype(R)
-    hal = find_halos(delta, k, P, Lc, Om0, z, dc, Nmin, a, beta, alpha, seed, precision, p)
+    hal = find_halos(delta, k, P, Lc, Om0, z, dc, Nmin, a, beta, alpha, seed, precision, p,
+                     sigma_radii)
     S1, S2 = lpt(dk, p, k_smooth, do_2lpt, precision)
     posh, _ = halo_positions(p, hal["flag"], S1, S2, hal, precision)
     out = dict(posh=posh, Mh=hal["Mh"], halos=hal, delta=delta, deltak=dk, params=p)

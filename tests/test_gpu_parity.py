@@ -105,6 +105,28 @@ def test_find_halos_ties_and_barrier(bbks, prec):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+def test_find_halos_fp64_sigma_table(bbks, prec):
+    """sigma_radii = 64 (the FP64 sigma(R) table) under an ellipsoidal
+    barrier: catalogue, flags and masses bit-exact against the oracle's
+    table of the same option; switching the option off and on again
+    rebuilds the cached tables."""
+    k, P = bbks
+    set_precision(prec)
+    nd = 64
+    delta, _, _ = O.field_from_modes(k, P, nd, LC, _noise(nd, 9, prec), precision=prec)
+    eb = (0.80, 0.29, 0.53)
+    refs = {r: O.find_halos(delta, k, P, LC, 0.31, 0.0, -1.0, 1, *eb, seed=12345,
+                            precision=prec, sigma_radii=r) for r in (0, 64)}
+    for r in (64, 0, 64):
+        out = X.find_halos(delta, k, P, LC, 0.31, 0.0, -1.0, 1, *eb, 1, 0, 0, sigma_radii=r)
+        cat = X.halo_catalogue()
+        assert len(cat["count"]) == len(refs[r]["count"]) > 0
+        np.testing.assert_array_equal(cat["count"], refs[r]["count"])
+        np.testing.assert_array_equal(out["flag"].reshape(-1), refs[r]["flag"].reshape(-1))
+        np.testing.assert_array_equal(out["Mh"], refs[r]["Mh"])
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 def test_cached_tables_follow_pk(bbks, prec):
     """The P(k) spline and the halo tables are kept on the device between
     calls; changing P(k), the box and back must rebuild them each time."""

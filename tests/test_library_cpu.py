@@ -76,22 +76,25 @@ def test_box_init_matches_oracle(lib, prec, Lc):
 
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("eb", [(1.0, 0.0, 0.0), (0.8, 0.29, 0.53)])
-def test_halo_tables_bit_identical_to_oracle(lib, bbks, prec, eb):
+@pytest.mark.parametrize("radii", [0, 64])
+def test_halo_tables_bit_identical_to_oracle(lib, bbks, prec, eb, radii):
+    """radii 0: the reference's table (one radius per k, the grid's float
+    type); 64: the FP64 table over 64 radii."""
     k, P = bbks
     R = np.float32 if prec == 4 else np.float64
     b = N.box(64, 4.0, precision=prec)
     p = O.params(64, 4.0, precision=prec)
-    ref = O.halo_tables(p, k, P, *eb, precision=prec)
+    ref = O.halo_tables(p, k, P, *eb, precision=prec, sigma_radii=radii)
     Ncells = ref["Ncells"]
     nB = len(ref["Bt"])
     kk = np.ascontiguousarray(np.asarray(k, dtype=R).astype(np.float64))
     PP = np.ascontiguousarray(np.asarray(P, dtype=R).astype(np.float64))
     o = N.HaloOpts(Nmin=1, a=eb[0], beta=eb[1], alpha=eb[2], seed=1, k=kk.ctypes.data,
-                   P=PP.ctypes.data, Nk=len(kk))
+                   P=PP.ctypes.data, Nk=len(kk), sigma_radii=radii)
     sig = np.zeros(Ncells, dtype=R)
     Bt = np.zeros(nB, dtype=R)
-    Mk = np.zeros(len(k))
-    Int = np.zeros(len(k))
+    Mk = np.zeros(len(ref["Mknots"]))
+    Int = np.zeros(len(ref["Int"]))
     nc = C.c_int()
     N.check(lib.exh_halo_tables(C.byref(b), C.byref(o), N.ptr(sig), N.ptr(Bt), nB, N.ptr(Mk),
                                 N.ptr(Int), C.byref(nc)))
