@@ -4,15 +4,19 @@
 Workload (the precision form the metric is quoted on): nd = 512, Lc = 4
 Mpc/h (L = 2048 Mpc/h), Om0 = 0.31, synthetic BBKS P(k) on 500 log-spaced k
 normalised to sigma8 = 0.81, unit white-noise modes from seed 1.  One step
-= white-noise modes already in HBM -> density grid (FP32 rocFFT) ->
-excursion-set halos (peaks, exact Lomuto sort, growth rounds, masses) ->
+= white-noise modes already in HBM -> density grid (FP32, the library's
+own FFT passes) -> excursion-set halos (peaks, exact Lomuto sort, growth
+rounds, masses from the FP64 sigma(R) table over 64 top-hat radii) ->
 1LPT + 2LPT displacements of the 512^3 particles -> interlaced CIC onto two
 512^3 grids -> r2c -> P(k) in 256 bins with FP64 accumulators, copied to
 the host.  value = nd^3 / (seconds per step) / 1e6.
 
 Also reported: the same pipeline in FP64 (the "plain FP64 GPU path"), the
-relative P(k) difference FP32 vs FP64, per-kernel HIP-event times with
-their HBM roofline fractions, and the CPU oracle timed on a bounded sample.
+relative P(k) difference FP32 vs FP64 and the agreement of their halo
+catalogues, per-kernel HIP-event times with their HBM roofline fractions,
+the batching workload (64 boxes of uneven size), the differentiation
+workload (forward and adjoint at nd = 128), and the CPU oracle timed on a
+bounded sample.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2]
 """
@@ -31,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+SIGMA_RADII = 64  # the precision form's FP64 sigma(R) table
 METRIC = "Mcells/s field\u21922LPT\u2192halos\u2192P(k) at nd=512, FP32 vs FP64 GPU at \u22641e-4 P(k) err"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_nd512.json")
 
@@ -201,6 +206,71 @@ def run_batch(k, P, reps=3):
     return out
 
 
+def run_adjoint(k, P, steps=5):
+    """The differentiation workload: nd = 128, L = 512 Mpc/h, FP32 forward
+    (2LPT -> CIC -> P(k) in 64 bins, FP64 bin index and accumulators), chi^2
+    against 1.05 x the input spectrum, and its gradient with respect to the
+    white-noise modes through the hand-written adjoint (positions
+    recomputed).  Forward and backward timed separately, mean of `steps`
+    after one warm-up."""
+    import torch
+    from exshalos_amd import adjoint as ADJ
+    from exshalos_amd import synth
+
+    nd = 128
+    cfg = ADJ.ChainConfig(k, P, nd=nd, Lc=4.0, do_2lpt=True, window=2, Nk=64, precision=4)
+    m = torch.as_tensor(synth.white_noise(nd, 1, np.float32), device="cuda")
+    _, kk, cnt = ADJ.power_of_modes(m, cfg)
+    target = 1.05 * np.interp(kk, k, P)
+    tf = tb = 0.0
+    for i in range(steps + 1):
+        x = m.detach().clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Pk = ADJ.power_from_modes(x, cfg)
+        loss = ADJ.pk_chi2(Pk, target, cnt)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if i > 0:
+            tf += t1 - t0
+            tb += t2 - t1
+    tf, tb = tf / steps, tb / steps
+    out = {"nd": nd, "L": 512.0, "Nk": 64, "window": "CIC", "do_2lpt": True, "dtype": "f32",
+           "ms_forward": round(tf * 1e3, 3), "ms_backward": round(tb * 1e3, 3),
+           "value": round(nd**3 / (tf + tb) / 1e6, 1), "unit": "Mcells/s (forward + gradient)",
+           "grad_norm": float(torch.linalg.vector_norm(x.grad.double()).item())}
+    print(f"[bench] adjoint nd={nd}: forward {tf * 1e3:.2f} ms, backward {tb * 1e3:.2f} ms",
+          flush=True)
+    return out
+
+
+def catalogue_agreement(c32, c64):
+    """The FP32 precision form's catalogue (FP32 grid, FP64 sigma table
+    over 64 radii) against the plain FP64 path's (FP64 grid, the
+    reference's table): halos with the same centre cell (centres are
+    distinct within a catalogue), those that also have the same cell count,
+    and those that also hold the same place in the catalogue.  Halo i's
+    mass is drawn with the i-th uniform, so masses are compared only over
+    the last set."""
+    x32, n32, m32 = c32
+    x64, n64, m64 = c64
+    cell = lambda x: (x[:, 0].astype(np.int64) << 40) | (x[:, 1].astype(np.int64) << 20) \
+        | x[:, 2].astype(np.int64)
+    _, i32, i64 = np.intersect1d(cell(x32), cell(x64), assume_unique=True, return_indices=True)
+    same = n32[i32] == n64[i64]
+    pos = same & (i32 == i64)
+    a, b = m32[i32][pos].astype(np.float64), m64[i64][pos]
+    rel = np.abs(a / b - 1.0) if len(a) else np.zeros(1)
+    return {"nh_fp32": int(len(x32)), "nh_fp64": int(len(x64)),
+            "same_centre": int(len(i32)), "same_centre_and_count": int(same.sum()),
+            "same_place": int(pos.sum()),
+            "agree_frac_of_fp64": round(float(same.sum()) / max(len(x64), 1), 6),
+            "mass_rel_diff_median": float(np.median(rel)), "mass_rel_diff_max": float(rel.max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +285,9 @@ def main():
     ap.add_argument("--batch", type=int, default=1,
                     help="also time the batching workload (64 boxes, 16 each of nd 64, 96, "
                          "128, 192; 2-tracer P(k) in 64 bins), 0 to skip")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="also time the differentiation workload (nd 128, chi^2 of the CIC "
+                         "P(k) in 64 bins and its gradient in the modes), 0 to skip")
     ap.add_argument("--api", choices=("fused", "calls"), default="fused",
                     help="fused: exh_halos_box_power (LPT, grid and P(k) beside the halo "
                          "finder on a second stream); calls: halos_box_from_pk + grid_compute "
@@ -241,8 +314,10 @@ def main():
     if args.fp64_steps > 0:
         runs.append((8, args.fp64_steps, 1))
     for prec, steps, warm in runs:
+        # precision form: FP64 sigma(R) table over 64 top-hat radii; the
+        # plain FP64 path keeps the reference's table
         pipe = BoxPipeline(k, P, nd=nd, Lc=4.0, precision=prec, do_2lpt=True, Nk=256,
-                           kbin_precision=8)
+                           kbin_precision=8, sigma_radii=SIGMA_RADII if prec == 4 else 0)
         pipe.set_modes(noise64.astype(np.float32) if prec == 4 else noise64)
         fused = args.api == "fused"
         for _ in range(warm):
@@ -259,7 +334,7 @@ def main():
         tim = pipe.timings()
         pipe.timing(False)
         results[prec] = dict(ms=dt * 1e3, k=kk, Pk=Pk, cnt=cnt, nh=nh, timings=tim,
-                             stats=pipe.halo_stats())
+                             stats=pipe.halo_stats(), cat=pipe.catalogue())
         print(f"[bench] prec {prec}: {dt * 1e3:.1f} ms/step, nh = {nh}, "
               f"{nd**3 / dt / 1e6:.1f} Mcells/s", flush=True)
         if prec == 4 and fused:
@@ -292,6 +367,10 @@ def main():
     if r64 is not None:
         ok = (r64["cnt"] > 0) & (r64["Pk"] != 0)
         pk_err = float(np.max(np.abs(r32["Pk"][ok] - r64["Pk"][ok]) / np.abs(r64["Pk"][ok])))
+
+    cat_agree = None
+    if r64 is not None:
+        cat_agree = catalogue_agreement(r32["cat"], r64["cat"])
 
     # per-kernel HIP-event times of the FP32 timed region
     steps = args.steps
@@ -337,6 +416,9 @@ def main():
     batch = None
     if args.batch:
         batch = run_batch(k, P)
+    adjoint = None
+    if args.adjoint:
+        adjoint = run_adjoint(k, P)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -368,12 +450,14 @@ def main():
             "speedup_fp32_over_fp64": round(r64["ms"] / ms, 3)},
         "pk_rel_err_fp32_vs_fp64": pk_err,
         "halos": {"fp32": int(r32["nh"]), "fp64": None if r64 is None else int(r64["nh"]),
-                  **r32["stats"]},
+                  "sigma_radii_fp32": SIGMA_RADII, **r32["stats"]},
+        "catalogue_fp32_vs_fp64": cat_agree,
         "roofline": roof,
         "roofline_isolated": roof_iso,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "batch": batch,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 

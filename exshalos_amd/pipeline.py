@@ -6,6 +6,8 @@ positions kept, OUT_LPT), Compute_Density_Grid (gridmodule.c:146-375) and
 Compute_Power_Spectrum (powermodule.c:4-108) without leaving the device;
 only the binned P(k) (and the halo count) come back to the host.  This is
 the workload the benchmark times; all buffers are allocated once.
+``sigma_radii`` > 0 gives the halo masses the FP64 sigma(R) table over that
+many top-hat radii (0: the reference's table).
 """
 
 from __future__ import annotations
@@ -20,7 +22,7 @@ from .lib import _native as N
 class BoxPipeline:
     def __init__(self, k, P, nd=512, Lc=4.0, Om0=0.31, z=0.0, precision=4, do_2lpt=True,
                  k_smooth=1e4, Nmin=1, seed=1, window=2, interlacing=True, Nk=256,
-                 kbin_precision=8, device=None):
+                 kbin_precision=8, device=None, sigma_radii=0):
         import torch
 
         self.torch = torch
@@ -41,6 +43,7 @@ class BoxPipeline:
         self.opts.Nmin, self.opts.a, self.opts.beta, self.opts.alpha = int(Nmin), 1.0, 0.0, 0.0
         self.opts.seed = int(seed)
         self.opts.k, self.opts.P, self.opts.Nk = self.kh.ctypes.data, self.Ph.ctypes.data, len(self.kh)
+        self.opts.sigma_radii = int(sigma_radii)
         dt = torch.float32 if self.prec == 4 else torch.float64
         dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         ng = self.nd**3
@@ -99,6 +102,16 @@ class BoxPipeline:
                                           N.ptr(self.Pout), N.ptr(self.cnt)), "power_compute")
         self.nh = nh.value
         return self.kout.copy(), self.Pout.copy(), self.cnt.copy(), self.nh
+
+    def catalogue(self):
+        """Centres [nh][3], cell counts and masses of the last run (host)."""
+        nh = int(self.nh)
+        x = np.zeros((max(nh, 1), 3), dtype=np.int32)
+        cnt = np.zeros(max(nh, 1), dtype=np.int32)
+        Mh = np.zeros(max(nh, 1), dtype=np.float32 if self.prec == 4 else np.float64)
+        N.check(N.load().exh_halos_get(self.ctx, N.ptr(x), N.ptr(cnt), None, N.ptr(Mh), 1),
+                "halos_get")
+        return x[:nh], cnt[:nh], Mh[:nh]
 
     def timing(self, on: bool):
         N.check(N.load().exh_ctx_set_timing(self.ctx, int(on)), "set_timing")

@@ -50,7 +50,8 @@ def _library_grad(noise64, cfg, target):
     _, kk, cnt = ADJ.power_of_modes(m.detach(), cfg)
     loss = ADJ.pk_chi2(Pk, target, cnt)
     loss.backward()
-    return float(loss), m.grad.detach().double().cpu().numpy(), Pk.detach().cpu().numpy(), cnt
+    return float(loss.detach()), m.grad.detach().double().cpu().numpy(), \
+        Pk.detach().cpu().numpy(), cnt
 
 
 def _reference_grad(noise64, kP, cfg, target, do2):
@@ -61,7 +62,7 @@ def _reference_grad(noise64, kP, cfg, target, do2):
     Pk = AR.pk_of_modes(m, c, do2)
     loss = AR.chi2(Pk, target, c.count)
     loss.backward()
-    return float(loss), m.grad.detach().cpu().numpy(), Pk.detach().cpu().numpy(), c.count
+    return float(loss.detach()), m.grad.detach().cpu().numpy(), Pk.detach().cpu().numpy(), c.count
 
 
 @pytest.mark.parametrize("prec", [8, 4])

@@ -408,6 +408,41 @@ def test_fp64_sigma_table_effect_on_the_catalogue(bbks):
     np.testing.assert_allclose(runs[2000]["Mh"], a["Mh"], rtol=1e-5)
 
 
+def test_fp64_sigma_table_mass_error_by_radii(bbks):
+    """The mass drawn for each sphere count (u = 0, 1/2, 0.999) from the FP64
+    table over 64, 128 and 256 radii against the reference's FP64 table (one
+    radius per k: 500), for the nd = 512, Lc = 4 box (Ncells = 1816):
+    64 radii leave the natural spline of M(Int) so coarse where Int(M)
+    flattens that a draw near 600 cells is off by more than 50%; 128 radii
+    bring every draw within 1e-2 and 256 within 1e-3.  The draw follows
+    find_halos.c:453-476 with scipy's natural splines, the same functions
+    as the oracle's (pinned by the mass-draw test below)."""
+    from scipy.interpolate import CubicSpline
+
+    k, P = bbks
+    p = O.params(512, LC)
+    sph = O.cells_in_spheres(20000)
+    counts = [int(c) for c in sph[:-1] if c * p.Mcell <= 1e16]
+
+    def draws(nr):
+        t = O.halo_tables(p, k, P, sigma_radii=nr)
+        I = CubicSpline(t["Mknots"], t["Int"], bc_type="natural")
+        inv = CubicSpline(t["Int"], t["Mknots"], bc_type="natural")
+        out = []
+        for i, c in enumerate(counts):
+            a, b = I(c * p.Mcell), I(sph[i + 1] * p.Mcell)
+            out.append([inv(a + (b - a) * u) for u in (0.0, 0.5, 0.999)])
+        return np.array(out)
+
+    ref = draws(0)
+    err = {nr: np.abs(draws(nr) / ref - 1).max(axis=1) for nr in (64, 128, 256)}
+    cs = np.array(counts)
+    assert err[64].max() > 0.5 and 300 < cs[err[64].argmax()] < 1000
+    assert err[64][cs < 50].max() < 1e-3
+    assert err[128].max() < 1e-2
+    assert err[256].max() < 1e-3
+
+
 # ------------------------------------------------------------- mass draw
 @pytest.mark.parametrize("prec", [4, 8])
 def test_mass_draw_uses_the_halo_index_uniform(bbks, prec):
