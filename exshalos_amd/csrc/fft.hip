@@ -1,20 +1,24 @@
 // Strided complex FFT passes along the two outer axes of a half-complex grid.
 //
 // A 3D real transform of [n0][n1][n2] runs as three passes: a batched 1D real
-// transform along the contiguous axis (rocFFT), and one complex pass along
-// each strided axis (this file).  A pass sees the grid as
+// transform along the contiguous axis (the row kernels at the end of this
+// file), and one complex pass along each strided axis.  A pass sees the grid
+// as
 //
 //     lines x [N points, `stride` elements apart] x [ncol contiguous columns]
 //
 // (axis 1: lines = n0, stride = ncol = n2/2+1; axis 0: lines = 1,
 // stride = ncol = n1 (n2/2+1)) and transforms every column in place.
+// Lengths are 2^a or 3 * 2^a (a leading radix-3 stage).
 //
 // One workgroup owns BZ adjacent columns of one line: 128 bytes per point,
 // so every global access is a full 128-byte row segment.  The transform is a
 // Stockham autosort FFT (Govindaraju et al., "High performance discrete
-// Fourier transforms on graphics processors", SC 2008) with radix-8 stages
-// and one leading radix-2 or radix-4 stage where log2 N is not a multiple of
-// three.  The first stage reads its operands straight from global memory and
+// Fourier transforms on graphics processors", SC 2008) with radix-8 stages,
+// one leading radix-2 or radix-4 stage where log2 of N's power-of-two part is
+// not a multiple of three, and a radix-3 stage before them where 3 divides N
+// (the mixed-radix Stockham indexing is the same for every radix).  The
+// first stage reads its operands straight from global memory and
 // the last stage writes straight back; only the intermediate stages go
 // through LDS, whose rows are padded by one element against bank conflicts.
 // Twiddles W_N^k are computed on the host in double and rounded once.
@@ -48,14 +52,24 @@ struct FftCfg<double> {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
+// power-of-two factor of a length 2^a or 3 * 2^a
+constexpr int pow2_part(int n) { return n % 3 == 0 ? n / 3 : n; }
+
+// Stages: one radix-3 stage where 3 divides N, then radix-8 stages with one
+// leading radix-2 or radix-4 stage where log2 of the power-of-two part is
+// not a multiple of three.
 template <int N>
 constexpr int fft_nstages() {
-  return (ilog2(N) % 3 ? 1 : 0) + ilog2(N) / 3;
+  return (N % 3 == 0 ? 1 : 0) + (ilog2(pow2_part(N)) % 3 ? 1 : 0) + ilog2(pow2_part(N)) / 3;
 }
 
 template <int N>
 constexpr int fft_radix(int s) {
-  return (ilog2(N) % 3 && s == 0) ? (1 << (ilog2(N) % 3)) : 8;
+  if (N % 3 == 0) {
+    if (s == 0) return 3;
+    s -= 1;
+  }
+  return (ilog2(pow2_part(N)) % 3 && s == 0) ? (1 << (ilog2(pow2_part(N)) % 3)) : 8;
 }
 
 template <typename C2>
@@ -131,8 +145,26 @@ __device__ __forceinline__ void dft(C2* v) {
     v[2] = csub(t0, t2);
     v[1] = cadd(t1, t3);
     v[3] = csub(t1, t3);
+  } else if constexpr (R == 3) {
+    // X1 = a - (b + c)/2 -+ i (sqrt 3 / 2)(b - c), X2 the conjugate rotation
+    using T = decltype(v[0].x);
+    const T s3 = (T)0.86602540378443864676;
+    const C2 a = v[0], t1 = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    C2 h, u;
+    h.x = a.x - (T)0.5 * t1.x;
+    h.y = a.y - (T)0.5 * t1.y;
+    if (INV) {  // +i s3 d
+      u.x = -s3 * d.y;
+      u.y = s3 * d.x;
+    } else {  // -i s3 d
+      u.x = s3 * d.y;
+      u.y = -s3 * d.x;
+    }
+    v[0] = cadd(a, t1);
+    v[1] = cadd(h, u);
+    v[2] = csub(h, u);
   } else {
-    static_assert(R == 8, "radix 2, 4 or 8");
+    static_assert(R == 8, "radix 2, 3, 4 or 8");
     C2 e[4] = {v[0], v[2], v[4], v[6]};
     C2 o[4] = {v[1], v[3], v[5], v[7]};
     dft<4, INV>(e);
@@ -347,6 +379,11 @@ void axis_dispatch(exh_ctx* c, int N, const void* src, void* dst, long long stri
     EXH_AXIS_CASE(256)
     EXH_AXIS_CASE(512)
     EXH_AXIS_CASE(1024)
+    EXH_AXIS_CASE(48)
+    EXH_AXIS_CASE(96)
+    EXH_AXIS_CASE(192)
+    EXH_AXIS_CASE(384)
+    EXH_AXIS_CASE(768)
 #undef EXH_AXIS_CASE
     default: EXH_REQUIRE(false, EXH_ERR_ARG, "no strided FFT pass of length %d", N);
   }
@@ -436,9 +473,12 @@ __device__ __forceinline__ void row_stages(const C2* gin, C2* gout, bool ok, C2*
                                                                       j0, twstep);
 }
 
+constexpr int pow2_floor(int n) { return n < 2 ? 1 : 2 * pow2_floor(n / 2); }
+
 template <int M>
 struct RowCfg {
-  static constexpr int G = M / 8 < 64 ? M / 8 : 64;  // threads per row
+  // threads per row: a power of two, so that the rows tile the workgroup
+  static constexpr int G = pow2_floor(M / 8 < 64 ? M / 8 : 64);
   static constexpr int RB = FFT_THREADS / G;          // rows per workgroup
   static constexpr int LDR = M + M / 8;               // padded LDS row
 };
@@ -537,6 +577,11 @@ void rows_dispatch(exh_ctx* c, int N, const void* in, void* out, long long nrows
     EXH_ROWS_CASE(128)
     EXH_ROWS_CASE(256)
     EXH_ROWS_CASE(512)
+    EXH_ROWS_CASE(24)
+    EXH_ROWS_CASE(48)
+    EXH_ROWS_CASE(96)
+    EXH_ROWS_CASE(192)
+    EXH_ROWS_CASE(384)
 #undef EXH_ROWS_CASE
     default: EXH_REQUIRE(false, EXH_ERR_ARG, "no real row transform of length %d", N);
   }
@@ -544,7 +589,13 @@ void rows_dispatch(exh_ctx* c, int N, const void* in, void* out, long long nrows
 
 }  // namespace
 
-bool fft_axis_supported(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+// 2^a, or 3 * 2^a with a >= 4
+static bool fft_length_ok(int n, int min_pow2) {
+  const int p = n % 3 == 0 ? n / 3 : n;
+  return n <= 1024 && p >= (n % 3 == 0 ? 16 : min_pow2) && (p & (p - 1)) == 0;
+}
+
+bool fft_axis_supported(int n) { return fft_length_ok(n, 16); }
 
 void fft_axis_pass(exh_ctx* c, int prec, bool inverse, int N, void* data, long long stride,
                    long long ncol, long long line_stride, long long nlines) {
@@ -573,7 +624,7 @@ void fft_axis0_inverse_poisson(exh_ctx* c, int prec, const void* src, void* dst,
                                 poisson_op<double>(p));
 }
 
-bool fft_rows_supported(int n) { return n >= 32 && n <= 1024 && (n & (n - 1)) == 0; }
+bool fft_rows_supported(int n) { return fft_length_ok(n, 32); }
 
 void fft_rows_pass(exh_ctx* c, int prec, bool inverse, int N, const void* in, void* out,
                    long long nrows) {

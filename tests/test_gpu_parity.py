@@ -164,6 +164,34 @@ def test_lpt(bbks, prec, do2):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("nd", [48, 96, 192])
+def test_fft_lengths_three_times_power_of_two(bbks, prec, nd):
+    """Sides 3 * 2^a run the FFT passes with a leading radix-3 stage (c2r of
+    the field, the Poisson-fused c2r and the r2c of the 2LPT source, the r2c
+    of the power spectrum): field, 2LPT displacements and P(k) against the
+    oracle (numpy FFTs) at the bars of the power-of-two sides."""
+    k, P = bbks
+    set_precision(prec)
+    noise = _noise(nd, 4, prec)
+    g = X.density_grid_compute(k, P, 1e5, nd, nd, nd, LC, 1, 4, 0, 0.0, 1e5, 0, 1, noise=noise)
+    delta, dk, p = O.field_from_modes(k, P, nd, LC, noise, precision=prec)
+    assert _rel(g["grid"], delta) < (2e-6 if prec == 4 else 1e-12)
+    assert _rel(g["gridk"], dk) < (1e-6 if prec == 4 else 1e-14)
+    if nd <= 96:
+        S1, S2 = O.lpt(dk, p, 1e4, True, prec)
+        out = X.lpt_compute(dk, LC, 0.31, 0.0, 1e4, 1, 0, 1, 0, 0)
+        assert _rel(out["pos"], S1.astype(np.float64) + S2) < (5e-5 if prec == 4 else 1e-10)
+    R = np.float32 if prec == 4 else np.float64
+    grid = np.random.default_rng(nd).normal(size=(1, 2, nd, nd, nd)).astype(R)
+    L = nd * LC
+    ref = O.power_compute(grid, 1, nd, L, window=2, interlacing=True, Nk=16, precision=prec)
+    out = SP.power_compute(grid, 1, nd, L, 2, 4.0, 1, 16, 2 * np.pi / L, np.pi / L * nd, 0, -1,
+                           0, 1)
+    np.testing.assert_array_equal(out["Nk"], ref["Nk"])
+    assert _rel(out["Pk"], ref["Pk64"]) < (1e-5 if prec == 4 else 1e-10)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("window", [1, 2, 5])
 @pytest.mark.parametrize("il", [0, 1])
 @pytest.mark.parametrize("nd", [32, 24, 20])
