@@ -313,7 +313,10 @@ __device__ inline int stencil_q(Q q, int nd, int* cell, T* w) {
   } else if constexpr (WIN == 2) {
     int xt = (int)floor(q);
     if (xt == nd) xt -= 1;
-    T dx = (T)((double)q - ((double)xt + 0.5));
+    // dx = q - (xt + 1/2) and 1 - dx in T: both differences are exact in
+    // double and round once, so T's correctly rounded subtraction gives the
+    // reference's double-then-round values bit for bit
+    T dx = q - ((T)xt + (T)0.5);
     int o;
     if (dx >= (T)0) {
       o = xt + 1 == nd ? 0 : xt + 1;
@@ -322,7 +325,7 @@ __device__ inline int stencil_q(Q q, int nd, int* cell, T* w) {
       dx = -dx;
     }
     cell[0] = xt;
-    w[0] = (T)(1.0 - (double)dx);
+    w[0] = (T)1 - dx;
     cell[1] = o;
     w[1] = dx;
     return 2;
@@ -612,6 +615,10 @@ __device__ __forceinline__ void deposit_window_tile(
   const int E[3] = {Ex, Ey, Ez};
   const int vol = Ex * sx;
   const T shift = (T)(-(double)Lb / 2.0);
+  // CIC scales by the reciprocal of the cell size (its weights are
+  // continuous in the position, so the rounding of x / Ld against x * (1 /
+  // Ld) moves them by an ulp); NGP and TSC divide as the reference does
+  const T invLd = (T)1 / Ld;
   for (int s = 0; s < nsub; s++) {
     T* g = grid + (size_t)s * ng;
     for (int v = threadIdx.x; v < vol; v += WIN_THREADS) box[v] = (ACC)0;
@@ -626,7 +633,10 @@ __device__ __forceinline__ void deposit_window_tile(
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         const T pa = wrap_while<T>(s == 0 ? xs[p][q] : xs[p][q] + shift, L);
-        stencil_q<T, WIN>(scale_q<T, WIN>(pa, Ld), nd, cell[q], w[q]);
+        if constexpr (WIN == 2)
+          stencil_q<T, WIN>(pa * invLd, nd, cell[q], w[q]);
+        else
+          stencil_q<T, WIN>(scale_q<T, WIN>(pa, Ld), nd, cell[q], w[q]);
         const int st = q == 0 ? sx : (q == 1 ? Ez : 1);
 #pragma unroll
         for (int u = 0; u < NW; u++) {
@@ -670,7 +680,11 @@ __device__ __forceinline__ void deposit_window_tile(
 #pragma unroll
         for (int q = 0; q < 3; q++) {
           const T pa = wrap_while<T>(s == 0 ? xs[p][q] : xs[p][q] + shift, L);
-          stencil_q<T, WIN>(scale_q<T, WIN>(pa, Ld), nd, cell[q], w[q]);
+          // the stencil of the window pass, so that `in` matches it
+          if constexpr (WIN == 2)
+            stencil_q<T, WIN>(pa * invLd, nd, cell[q], w[q]);
+          else
+            stencil_q<T, WIN>(scale_q<T, WIN>(pa, Ld), nd, cell[q], w[q]);
           for (int u = 0; u < NW; u++) {
             const int l = rel_cell(cell[q][u], qc[q], nd) - wo[q];
             in = in && (unsigned)l < (unsigned)E[q];
