@@ -191,24 +191,41 @@ struct OpNone {
 };
 
 // phi_k = c_k * pre / (-k^2) * scale on the pass along axis 0, whose columns
-// are the (j, k) pairs of a plane: the expression of the standalone Poisson
-// kernel, so the values are bit-identical.
+// are the (j, k) pairs of a plane.  k^2 = kx^2 + (ky^2 + kz^2) in double,
+// the column's part bound once; the factor pre / (-k^2) * scale is formed
+// once per point and multiplies both parts.  The standalone Poisson kernel
+// (lpt_fact) rounds |k| to T and divides each part, so the two agree to a
+// few ulp of T; its |k| <= k_smooth test is repeated exactly for the points
+// whose k^2 lies within 1e-6 of k_smooth^2, so both zero the same modes.
 template <typename T>
 struct OpPoisson {
   int n0, n1, n2, nz2;
   T kl0, kl1, kl2, k_smooth, pre, scale;
+  double ks2;  // k_smooth^2
   int j, k;
+  double kyz2;
   __device__ void bind(long long col) {
     j = (int)(col / nz2);
     k = (int)(col - (long long)j * nz2);
+    const T ky = (2 * j < n1) ? (T)j * kl1 : (T)(j - n1) * kl1;
+    T kz = (T)k * kl2;
+    if (k == n2 / 2) kz = (T)(-(T)n2 / 2.0 * (double)kl2);
+    kyz2 = (double)ky * ky + (double)kz * kz;
   }
   template <typename C2>
   __device__ C2 operator()(C2 v, int i) const {
-    T fact;
+    const T kx = (2 * i < n0) ? (T)i * kl0 : (T)(i - n0) * kl0;
+    const double s = (double)kx * kx + kyz2;
+    bool in = s > 0.0 && s <= ks2;
+    if (fabs(s - ks2) <= 1e-6 * ks2) {
+      T fact;
+      in = lpt_fact<T>(n0, n1, n2, kl0, kl1, kl2, k_smooth, i, j, k, fact);
+    }
     C2 r;
-    if (lpt_fact<T>(n0, n1, n2, kl0, kl1, kl2, k_smooth, i, j, k, fact)) {
-      r.x = v.x * pre / fact * scale;
-      r.y = v.y * pre / fact * scale;
+    if (in) {
+      const T m = pre / (T)(-s) * scale;
+      r.x = v.x * m;
+      r.y = v.y * m;
     } else {
       r.x = 0;
       r.y = 0;
@@ -402,6 +419,7 @@ OpPoisson<T> poisson_op(const PoissonParams& p) {
   o.k_smooth = (T)p.k_smooth;
   o.pre = (T)p.pre;
   o.scale = (T)p.scale;
+  o.ks2 = (double)o.k_smooth * (double)o.k_smooth;
   return o;
 }
 
