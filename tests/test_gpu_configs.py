@@ -311,3 +311,24 @@ def test_nd512_precision_form(bbks):
     _say(f"nd=512: P(k) error vs the FP64 oracle: FP32 {e32:.2e}, FP64 {e64:.2e}")
     assert e32 < 1e-4
     assert e64 < 1e-5
+
+
+@pytest.mark.timeout(900)
+def test_nd512_fp64_halos_bit_exact(bbks):
+    """The plain FP64 path at nd = 512 (seed-1 modes, 2LPT): halos, flags and
+    masses bit-exact against the FP64 oracle's finder on the GPU's FP64 grid,
+    and the catalogue invariants."""
+    k, P = bbks
+    nd = 512
+    set_precision(8)
+    _say("nd=512 FP64: GPU pipeline")
+    out = M.Generate_Halos_Box_from_Pk(k, P, nd=nd, Lc=LC, Om0=0.31, z=0.0, k_smooth=1e4,
+                                       seed=1, OUT_DEN=True, DO_2LPT=True, OUT_FLAG=True,
+                                       noise=_noise(nd, 1, 8))
+    cat = X.halo_catalogue()
+    _say("nd=512 FP64: oracle halo finder on the GPU's grid")
+    ref = O.find_halos(out["grid"], k, P, LC, seed=1, precision=8)
+    _halos_bit_exact(out, ref)
+    _catalogue_invariants(out["grid"], out["flag"], cat, float(ref["params"].dc))
+    _say(f"nd=512 FP64: {len(ref['count'])} halos bit-exact")
+
