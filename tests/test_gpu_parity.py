@@ -403,6 +403,44 @@ def test_halos_box_power_matches_three_calls(bbks, prec, do2):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+def test_halos_box_power_against_the_oracle(bbks, prec):
+    """The benchmarked call itself (exh_halos_box_power, 2LPT, interlaced
+    CIC, FP64 bin index) against the oracle at nd = 64: the catalogue and
+    masses bit-exact on the GPU's density grid (the same field kernels as
+    the call's), particle positions against the oracle's 2LPT, and P(k)
+    against the FP64 oracle's pipeline with exact mode counts."""
+    from exshalos_amd.pipeline import BoxPipeline
+
+    k, P = bbks
+    set_precision(prec)
+    nd, Nk = 64, 24
+    L = nd * LC
+    noise = _noise(nd, 9, prec)
+    pipe = BoxPipeline(k, P, nd=nd, Lc=LC, precision=prec, do_2lpt=True, Nk=Nk)
+    pipe.set_modes(noise)
+    kk, Pk, cnt, nh = pipe.run(fused=True)
+    x, count, Mh = pipe.catalogue()
+    pos = pipe.pos.cpu().numpy().astype(np.float64)
+    g = X.density_grid_compute(k, P, 1e5, nd, nd, nd, LC, 0, 1, 0, 0.0, 1e5, 0, 1, noise=noise)
+    ref = O.find_halos(g["grid"], k, P, LC, seed=1, precision=prec)
+    assert nh == len(ref["count"]) > 0
+    np.testing.assert_array_equal(x, ref["x"])
+    np.testing.assert_array_equal(count, ref["count"])
+    np.testing.assert_array_equal(Mh, ref["Mh"])
+    p8 = O.params(nd, LC, precision=8)
+    dk = O.field_modes(p8, k, P, noise.astype(np.float64), 8)
+    S1, S2 = O.lpt(dk, p8, 1e4, True, 8)
+    opos = O.positions((S1 + S2), p8, 8)
+    d = np.abs(pos - opos)
+    assert np.minimum(d, L - d).max() < (2e-3 if prec == 4 else 1e-8)
+    gr = O.grid_compute(opos, nd, L, window=2, interlacing=True, precision=8)
+    pr = O.power_compute(gr, 1, nd, L, window=2, interlacing=True, Nk=Nk, precision=8)
+    np.testing.assert_array_equal(cnt, pr["Nk"])
+    assert _rel(Pk, pr["Pk64"][0, 0]) < (1e-4 if prec == 4 else 1e-5)
+    assert _rel(kk, pr["k64"]) < (1e-6 if prec == 4 else 1e-12)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 def test_sort_extreme_pivots_and_ties(bbks, prec):
     """Isolated one-cell spikes on every other cell, heights nearly sorted in
     lexicographic order and quantised: Lomuto's last-element pivots are then
