@@ -249,6 +249,30 @@ def test_grid_compute_lattice_tiles(prec, window, il, nL, nd):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("folds", [2, 3])
+@pytest.mark.parametrize("window", [2, 5])
+def test_grid_compute_folds(prec, folds, window):
+    """folds > 1 (gridmodule.c:152): positions wrapped into the folded box
+    L / folds, grid cells of L / (folds nd).  Lattice-ordered particles (the
+    tiled path: a tile's particles then spread over folds times its cells,
+    so most stencils leave the window and deposit directly) and a random
+    set (the direct scatter), interlaced, against the oracle."""
+    set_precision(prec)
+    R = np.float32 if prec == 4 else np.float64
+    L, nL, nd = 128.0, 32, 32
+    rng = np.random.default_rng(41 + folds + window)
+    q = (np.indices((nL, nL, nL)).reshape(3, -1).T + 0.5) * (L / nL)
+    lattice = np.mod(q + rng.normal(0, 1.0, q.shape), L).astype(R)
+    scattered = (rng.random((20000, 3)) * L).astype(R)
+    for pos in (lattice, scattered):
+        ref = O.grid_compute(pos, nd, L, window=window, interlacing=True, folds=folds,
+                             precision=prec)
+        g = SP.grid_compute(pos, None, None, 0, None, 1, nd, L, 0.31, 0.0, -1, window, 4.0, 5.0,
+                            1, folds, 0, 1)
+        assert _rel(g, ref) < (2e-5 if prec == 4 else 1e-12)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("il", [0, 1])
 def test_grid_compute_lattice_bulk_flows(prec, il):
     """Lattice-ordered particles (nL = nd = 64, 4^3 tiles of 16 cells) moved
