@@ -847,15 +847,17 @@ void* noalloc_grid_scratch(int device) {
   return g_scratch_ptr[device];
 }
 
-// Tile shape of the window deposit (EXH_DEPOSIT_TILE: 3216 = 32 x 16 x 16
-// particles, the default; 32 = 32 x 32 x 16; 16 = 16^3) and its
+// Tile shape of the window deposit (EXH_DEPOSIT_TILE: 16 = 16^3 particles,
+// the default; 3216 = 32 x 16 x 16; 32 = 32 x 32 x 16) and its
 // accumulator (EXH_DEPOSIT_ACC: 1 fixed point, the default for float grids
 // without masses; 8 double; 4 float).  At nd = nL = 512 (CIC, interlaced),
 // alone: 3.18 ms for 32 x 16 x 16 in fixed point, 3.06 ms for 32 x 32 x 16,
 // 3.63 ms for 16^3, 3.89 ms for 16^3 in double, 12.6 ms for 16^3 in float
 // (ds_add_f32 runs far slower than ds_add_u32 or ds_add_f64 here).  Beside
 // the halo finder's sort the 32 x 32 x 16 workgroups (153 KB of LDS each)
-// hold CUs long enough to slow the sort by 2.6 ms; 32 x 16 x 16 does not.
+// hold CUs long enough to slow the sort by 2.6 ms; the 16^3 workgroups
+// (55 KB, two per CU) leave it most room: the fused nd = 512 step runs
+// 26.85-26.99 ms with them against 27.21-27.27 ms with 32 x 16 x 16.
 struct WinShape {
   int px, py, pz;
 };
@@ -867,7 +869,7 @@ static WinShape tile_shape(int t) {
 static WinShape deposit_window_shape() {
   static const WinShape v = [] {
     const char* e = std::getenv("EXH_DEPOSIT_TILE");
-    return tile_shape(e ? std::atoi(e) : 3216);
+    return tile_shape(e ? std::atoi(e) : 16);
   }();
   return v;
 }
