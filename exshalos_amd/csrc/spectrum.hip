@@ -1227,7 +1227,30 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
       // combined modes of every tracer
       double dr[8], di[8];
       const int nt = ONE ? 1 : a.ntype;
-      if (valid) {
+      if constexpr (ONE && sizeof(T) == 4) {
+        // one float tracer: the combine and deconvolution in float (the
+        // grid's precision), |delta|^2 summed in double below
+        if (valid) {
+          const double2 e1d = a.ex[i], e2d = a.ex[j], e3d = a.ez[k];
+          const float2 e1 = make_float2((float)e1d.x, (float)e1d.y);
+          const float2 e2 = make_float2((float)e2d.x, (float)e2d.y);
+          const float2 e3 = make_float2((float)e3d.x, (float)e3d.y);
+          const float wk = a.rwin ? (float)(a.Normk / (double)window_r<T>(a, fi, fj,
+                                                                           (2 * k < nd) ? k : k - nd))
+                                  : (float)a.wx[i] * (float)a.wx[j] * (float)a.wz[k] * (float)a.Normk;
+          const float pr = e1.x * e2.x - e1.y * e2.y, pi = e1.x * e2.y + e1.y * e2.x;
+          const float er = pr * e3.x - pi * e3.y, ei = pr * e3.y + pi * e3.x;
+          const C2 f1 = gk[id];
+          float re = (float)f1.x, im = (float)f1.y;
+          if (a.nsub == 2) {
+            const C2 f2 = gk[nh + id];
+            re = (re + ((float)f2.x * er - (float)f2.y * ei)) * 0.5f;
+            im = (im + ((float)f2.x * ei + (float)f2.y * er)) * 0.5f;
+          }
+          dr[0] = (double)(re * wk);
+          di[0] = (double)(im * wk);
+        }
+      } else if (valid) {
         const double2 e1 = a.ex[i], e2 = a.ex[j], e3 = a.ez[k];
         const double wk = a.rwin ? a.Normk / (double)window_r<T>(a, fi, fj, (2 * k < nd) ? k : k - nd)
                                  : a.wx[i] * a.wx[j] * a.wz[k] * a.Normk;
