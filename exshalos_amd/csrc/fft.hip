@@ -419,7 +419,8 @@ OpPoisson<T> poisson_op(const PoissonParams& p) {
   o.k_smooth = (T)p.k_smooth;
   o.pre = (T)p.pre;
   o.scale = (T)p.scale;
-  o.ks2 = (double)o.k_smooth * (double)o.k_smooth;
+  // |k| <= k_smooth as k^2 <= k_smooth^2 (a negative cut keeps no mode)
+  o.ks2 = o.k_smooth < (T)0 ? -1.0 : (double)o.k_smooth * (double)o.k_smooth;
   return o;
 }
 
