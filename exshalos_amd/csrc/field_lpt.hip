@@ -317,7 +317,9 @@ __global__ __launch_bounds__(256) void k_poisson_grid(int n0, int n1, int n2, T 
 
 // ---------------------------------------------------------------------------
 // Stencil tiles.  A workgroup owns a ST_TJ x ST_TK tile of (j, k) and walks
-// i over a slab of ST_TI planes.  Plane i, with halos of 3 cells in j and
+// i over a slab of g.ti planes (ST_TI, halved down to 2 while the grid has
+// fewer than 512 workgroups, so small boxes still fill the GPU; each slab
+// reads 6 planes beyond its own).  Plane i, with halos of 3 cells in j and
 // k, is staged in LDS for the y and z differences; the x difference uses a
 // register window of the thread's points over i-3..i+3, so each value is
 // read from HBM about once.  256 threads: 64 along k, 4 along j, each
@@ -335,13 +337,13 @@ __device__ inline double d6w(double m3, double m2, double m1, double p1, double 
 }
 
 struct TileGeom {
-  int n0, n1, n2, ntk, ntj, nti;
+  int n0, n1, n2, ntk, ntj, nti, ti;
 };
 
 __device__ inline void tile_origin(const TileGeom& g, int& i0, int& j0, int& k0) {
   const int b = blockIdx.x;
   const int bk = b % g.ntk, bj = (b / g.ntk) % g.ntj, bi = b / (g.ntk * g.ntj);
-  i0 = bi * ST_TI;
+  i0 = bi * g.ti;
   j0 = bj * ST_TJ;
   k0 = bk * ST_TK;
 }
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, do
       win[q][d] = phi[((size_t)ii * g.n1 + j) * g.n2 + kc];
     }
   }
-  const int i1 = min(i0 + ST_TI, g.n0);
+  const int i1 = min(i0 + g.ti, g.n0);
   for (int i = i0; i < i1; i++) {
     const int ip = wrapi(i + 3, g.n0);
     for (int q = 0; q < ST_Q; q++) {
@@ -503,7 +505,7 @@ __global__ __launch_bounds__(256) void k_source_tiles(TileGeom g, const T* S, do
       win[q][d] = S[(((size_t)ii * g.n1 + j) * g.n2 + kc) * 3];
     }
   }
-  const int i1 = min(i0 + ST_TI, g.n0);
+  const int i1 = min(i0 + g.ti, g.n0);
   for (int i = i0; i < i1; i++) {
     const int ip = wrapi(i + 3, g.n0);
     for (int q = 0; q < ST_Q; q++) {
@@ -547,7 +549,12 @@ static TileGeom tile_geom(int n0, int n1, int n2) {
   g.n2 = n2;
   g.ntk = (n2 + ST_TK - 1) / ST_TK;
   g.ntj = (n1 + ST_TJ - 1) / ST_TJ;
-  g.nti = (n0 + ST_TI - 1) / ST_TI;
+  g.ti = ST_TI;
+  g.nti = (n0 + g.ti - 1) / g.ti;
+  while (g.ti > 2 && (long long)g.ntk * g.ntj * g.nti < 512) {
+    g.ti /= 2;
+    g.nti = (n0 + g.ti - 1) / g.ti;
+  }
   return g;
 }
 
