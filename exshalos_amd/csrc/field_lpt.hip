@@ -348,40 +348,43 @@ __device__ inline void tile_origin(const TileGeom& g, int& i0, int& j0, int& k0)
   k0 = bk * ST_TK;
 }
 
-// Stage plane i of component `comp` (of ncomp) into pl[ST_PJ][ST_PK] with
-// periodic halos; cells outside a partial tile are staged too (harmless).
-template <typename T>
-__device__ inline void stage_plane(const TileGeom& g, const T* f, int ncomp, int comp, int i,
-                                   int j0, int k0, T* pl) {
-  for (int v = threadIdx.x; v < ST_PJ * ST_PK; v += 256) {
-    const int pj = v / ST_PK, pk = v % ST_PK;
-    int j = j0 + pj - ST_H, k = k0 + pk - ST_H;
-    j = j < 0 ? j + g.n1 : (j >= g.n1 ? j - g.n1 : j);
-    k = k < 0 ? k + g.n2 : (k >= g.n2 ? k - g.n2 : k);
-    j = min(j, g.n1 - 1);  // partial tiles smaller than the halo
-    k = min(k, g.n2 - 1);
-    pl[v] = f[(((size_t)i * g.n1 + j) * g.n2 + k) * ncomp + comp];
+// Staging of one plane (ST_PJ x ST_PK cells with periodic halos, NC
+// interleaved components per cell) into LDS in the field's own order:
+// thread t stages the plane's values t, t + 256, ..., so the LDS index is
+// the loop index and the offset of each value within a plane of the field
+// is the same for every plane of the slab.  The offsets are computed once
+// per workgroup (cells outside a partial tile are staged too, harmlessly);
+// a plane then costs one load and one LDS store per value.
+template <int NC>
+struct PlaneStage {
+  static constexpr int TOT = ST_PJ * ST_PK * NC;
+  static constexpr int NE = (TOT + 255) / 256;
+  int off[NE];
+  __device__ void init(const TileGeom& g, int j0, int k0) {
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      const int v = threadIdx.x + 256 * e;
+      const int cellv = v / NC, comp = v - NC * cellv;
+      const int pj = cellv / ST_PK, pk = cellv - ST_PK * pj;
+      int j = j0 + pj - ST_H, k = k0 + pk - ST_H;
+      j = j < 0 ? j + g.n1 : (j >= g.n1 ? j - g.n1 : j);
+      k = k < 0 ? k + g.n2 : (k >= g.n2 ? k - g.n2 : k);
+      j = min(j, g.n1 - 1);  // partial tiles smaller than the halo
+      k = min(k, g.n2 - 1);
+      off[e] = v < TOT ? (j * g.n2 + k) * NC + comp : 0;
+    }
   }
-}
-
-// All three components of an [ng][3] field for plane i at once: consecutive
-// threads read consecutive floats of a row's (k, component) run, and each
-// value goes to its component's LDS plane.
-template <typename T>
-__device__ inline void stage_plane3(const TileGeom& g, const T* f, int i, int j0, int k0,
-                                    T (*pl)[ST_PJ * ST_PK]) {
-  constexpr int ROW = 3 * ST_PK;
-  for (int v = threadIdx.x; v < ST_PJ * ROW; v += 256) {
-    const int pj = v / ROW, rem = v - pj * ROW;
-    const int pk = rem / 3, comp = rem - 3 * pk;
-    int j = j0 + pj - ST_H, k = k0 + pk - ST_H;
-    j = j < 0 ? j + g.n1 : (j >= g.n1 ? j - g.n1 : j);
-    k = k < 0 ? k + g.n2 : (k >= g.n2 ? k - g.n2 : k);
-    j = min(j, g.n1 - 1);  // partial tiles smaller than the halo
-    k = min(k, g.n2 - 1);
-    pl[comp][pj * ST_PK + pk] = f[(((size_t)i * g.n1 + j) * g.n2 + k) * 3 + comp];
+  // plane i of f ([ng][NC]) into pl[TOT]
+  template <typename T>
+  __device__ void stage(const TileGeom& g, const T* f, int i, T* pl) const {
+    const T* base = f + (size_t)i * g.n1 * g.n2 * NC;
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      const int v = threadIdx.x + 256 * e;
+      if (e < NE - 1 || v < TOT) pl[v] = base[off[e]];
+    }
   }
-}
+};
 
 __device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -417,23 +420,22 @@ __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, do
   const int k = k0 + tx;
   const bool kin = k < g.n2;
   const int kc = kin ? k : g.n2 - 1;
+  PlaneStage<1> ps;
+  ps.init(g, j0, k0);
+  const size_t plane = (size_t)g.n1 * g.n2;
+  int wo[ST_Q];  // the thread's cells within a plane
+  for (int q = 0; q < ST_Q; q++) wo[q] = min(j0 + ty + 4 * q, g.n1 - 1) * g.n2 + kc;
   T win[ST_Q][7];
-  for (int q = 0; q < ST_Q; q++) {
-    const int j = min(j0 + ty + 4 * q, g.n1 - 1);
-    for (int d = 0; d < 6; d++) {
-      const int ii = wrapi(i0 - 3 + d, g.n0);
-      win[q][d] = phi[((size_t)ii * g.n1 + j) * g.n2 + kc];
-    }
+  for (int d = 0; d < 6; d++) {
+    const T* pp = phi + (size_t)wrapi(i0 - 3 + d, g.n0) * plane;
+    for (int q = 0; q < ST_Q; q++) win[q][d] = pp[wo[q]];
   }
   const int i1 = min(i0 + g.ti, g.n0);
   for (int i = i0; i < i1; i++) {
-    const int ip = wrapi(i + 3, g.n0);
-    for (int q = 0; q < ST_Q; q++) {
-      const int j = min(j0 + ty + 4 * q, g.n1 - 1);
-      win[q][6] = phi[((size_t)ip * g.n1 + j) * g.n2 + kc];
-    }
+    const T* pp = phi + (size_t)wrapi(i + 3, g.n0) * plane;
+    for (int q = 0; q < ST_Q; q++) win[q][6] = pp[wo[q]];
     __syncthreads();  // previous plane's readers are done
-    stage_plane<T>(g, phi, 1, 0, i, j0, k0, pl);
+    ps.stage(g, phi, i, pl);
     __syncthreads();
     for (int q = 0; q < ST_Q; q++) {
       const int jl = ty + 4 * q, j = j0 + jl;
@@ -490,46 +492,49 @@ __global__ __launch_bounds__(256) void k_grad_tiles(TileGeom g, const T* phi, do
 template <typename T>
 __global__ __launch_bounds__(256) void k_source_tiles(TileGeom g, const T* S, double inv60,
                                                       T* src) {
-  __shared__ T pl[3][ST_PJ * ST_PK];
+  __shared__ T pl[3 * ST_PJ * ST_PK];  // [pj][pk][component]
   int i0, j0, k0;
   tile_origin(g, i0, j0, k0);
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int k = k0 + tx;
   const bool kin = k < g.n2;
   const int kc = kin ? k : g.n2 - 1;
+  PlaneStage<3> ps;
+  ps.init(g, j0, k0);
+  const size_t plane = (size_t)g.n1 * g.n2 * 3;
+  int wo[ST_Q];  // Sx of the thread's cells within a plane
+  for (int q = 0; q < ST_Q; q++) wo[q] = (min(j0 + ty + 4 * q, g.n1 - 1) * g.n2 + kc) * 3;
   T win[ST_Q][7];  // Sx over i-3..i+3
-  for (int q = 0; q < ST_Q; q++) {
-    const int j = min(j0 + ty + 4 * q, g.n1 - 1);
-    for (int d = 0; d < 6; d++) {
-      const int ii = wrapi(i0 - 3 + d, g.n0);
-      win[q][d] = S[(((size_t)ii * g.n1 + j) * g.n2 + kc) * 3];
-    }
+  for (int d = 0; d < 6; d++) {
+    const T* pp = S + (size_t)wrapi(i0 - 3 + d, g.n0) * plane;
+    for (int q = 0; q < ST_Q; q++) win[q][d] = pp[wo[q]];
   }
+  constexpr int DY = 3 * ST_PK, DZ = 3;
   const int i1 = min(i0 + g.ti, g.n0);
   for (int i = i0; i < i1; i++) {
-    const int ip = wrapi(i + 3, g.n0);
-    for (int q = 0; q < ST_Q; q++) {
-      const int j = min(j0 + ty + 4 * q, g.n1 - 1);
-      win[q][6] = S[(((size_t)ip * g.n1 + j) * g.n2 + kc) * 3];
-    }
+    const T* pp = S + (size_t)wrapi(i + 3, g.n0) * plane;
+    for (int q = 0; q < ST_Q; q++) win[q][6] = pp[wo[q]];
     __syncthreads();
-    stage_plane3<T>(g, S, i, j0, k0, pl);
+    ps.stage(g, S, i, pl);
     __syncthreads();
     for (int q = 0; q < ST_Q; q++) {
       const int jl = ty + 4 * q, j = j0 + jl;
       const T* w = win[q];
-      const int o = (jl + ST_H) * ST_PK + tx + ST_H;
-      const T* x = pl[0] + o;
-      const T* y = pl[1] + o;
-      const T* z = pl[2] + o;
+      const int o = ((jl + ST_H) * ST_PK + tx + ST_H) * 3;
+      const T* x = pl + o;
+      const T* y = pl + o + 1;
+      const T* z = pl + o + 2;
       const T sxx = (T)(d6w(w[0], w[1], w[2], w[4], w[5], w[6]) * inv60);
-      const T sxy = (T)(d6w(x[-3 * ST_PK], x[-2 * ST_PK], x[-ST_PK], x[ST_PK], x[2 * ST_PK],
-                            x[3 * ST_PK]) * inv60);
-      const T sxz = (T)(d6w(x[-3], x[-2], x[-1], x[1], x[2], x[3]) * inv60);
-      const T syy = (T)(d6w(y[-3 * ST_PK], y[-2 * ST_PK], y[-ST_PK], y[ST_PK], y[2 * ST_PK],
-                            y[3 * ST_PK]) * inv60);
-      const T syz = (T)(d6w(y[-3], y[-2], y[-1], y[1], y[2], y[3]) * inv60);
-      const T szz = (T)(d6w(z[-3], z[-2], z[-1], z[1], z[2], z[3]) * inv60);
+      const T sxy =
+          (T)(d6w(x[-3 * DY], x[-2 * DY], x[-DY], x[DY], x[2 * DY], x[3 * DY]) * inv60);
+      const T sxz =
+          (T)(d6w(x[-3 * DZ], x[-2 * DZ], x[-DZ], x[DZ], x[2 * DZ], x[3 * DZ]) * inv60);
+      const T syy =
+          (T)(d6w(y[-3 * DY], y[-2 * DY], y[-DY], y[DY], y[2 * DY], y[3 * DY]) * inv60);
+      const T syz =
+          (T)(d6w(y[-3 * DZ], y[-2 * DZ], y[-DZ], y[DZ], y[2 * DZ], y[3 * DZ]) * inv60);
+      const T szz =
+          (T)(d6w(z[-3 * DZ], z[-2 * DZ], z[-DZ], z[DZ], z[2 * DZ], z[3 * DZ]) * inv60);
       if (kin && j < g.n1) {
         const T t3 = sxx * syy + sxx * szz + syy * szz;
         const double v = (double)t3 - (double)sxy * (double)sxy - (double)sxz * (double)sxz -
