@@ -65,7 +65,7 @@ struct LevelState {
   unsigned long long rmax;        // most steps an unfinished chain still needs
   unsigned long long nsmall;      // small segments queued so far (all levels)
   unsigned long long need_any;    // some segment needs pointer doubling
-  unsigned long long async_open;  // chains left open by k_double_async
+  unsigned long long async_open;  // chains left open by k_double_walk
   unsigned long long err;         // doubling did not converge (all levels)
   unsigned long long pad;
 };
@@ -87,6 +87,7 @@ struct LevelBufs {
   int dst_id;  // 1 or 2: buffer id of dst for the small list
   long long small_max;
   SegInfo* info;
+  int* tseg;  // segment of each tile of the level (written by k_tile_count)
   int* seg_need;
   LevelState* st;
   // keys below `floor` need no order: a left child of a pivot below it
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(256) void k_tile_count(const K* keys, LevelBufs b, 
     st->async_open = 0;
   }
   const int s = find_seg(b.toff, b.nseg, t);
+  if (threadIdx.x == 0) b.tseg[t] = s;
   const long long lo = b.lo[s], hi = b.hi[s];
   const K pivot = keys[hi];
   const long long start = lo + (t - b.toff[s]) * TILE;
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int*
                                                       int* arot, int* s0) {
   __shared__ int shw[4];
   const long long t = blockIdx.x;
-  const int sg = find_seg(b.toff, b.nseg, t);
+  const int sg = b.tseg[t];
   const SegInfo g = seg_info(b, tile_pre, sg);
   const K pivot = keysA[g.hi];
   const long long m = (g.hi - g.lo) - g.c;
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(256) void k_tile_chase(const K* keysA, const int* v
                                                     LevelBufs b, const int* arot, int* s0,
                                                     int* jmp, int cap) {
   const long long t = blockIdx.x;
-  const int sg = find_seg(b.toff, b.nseg, t);
+  const int sg = b.tseg[t];
   const SegInfo g = b.info[sg];
   const K pivot = keysA[g.hi];
   const long long start = g.lo + (t - g.first_tile) * TILE;
@@ -388,9 +390,16 @@ static int chase_cap() {
 // agent-scope atomics so that advances made on other XCDs become visible.
 // Entries still open after `guard` sweeps set async_open; the host then
 // finishes with the synchronous rounds of k_double_round.
-__global__ __launch_bounds__(256) void k_double_async(LevelBufs b, int* jmp, int guard) {
+// The same kernel then walks the tile's own open elements to their slots,
+// as k_tile_chase2 (strict = 0) does: a walk that reaches the stream's end
+// has its exact slot whether or not the jumping of other tiles has
+// converged, since every pointer leads forward along its own chain.
+template <typename K>
+__global__ __launch_bounds__(256) void k_double_walk(const K* keysA, const int* valsA, K* keysB,
+                                                     int* valsB, K* keysF, int* valsF, LevelBufs b,
+                                                     int* jmp, const int* s0, int guard) {
   const long long t = blockIdx.x;
-  const int sg = find_seg(b.toff, b.nseg, t);
+  const int sg = b.tseg[t];
   if (!b.seg_need[sg]) return;
   const SegInfo g = b.info[sg];
   const long long q0 = (t - g.first_tile) * TILE;
@@ -421,7 +430,31 @@ __global__ __launch_bounds__(256) void k_double_async(LevelBufs b, int* jmp, int
       }
     }
   }
-  if (any) atomicExch(&b.st->async_open, 1ull);
+  bool open = any;
+  const K pivot = keysA[g.hi];
+  const bool single_right = (g.hi - g.lo) - g.c == 1;
+  const long long start = g.lo + q0;
+  for (int q = 0; q < PER_THREAD; q++) {
+    const long long j = start + (long long)q * 256 + threadIdx.x;
+    if (j >= g.hi) continue;
+    if (keysA[j] <= pivot) continue;
+    long long s = s0[j];
+    if (s >= g.r) continue;  // placed by the capped chase
+    for (int gd = 0; s < g.r && gd < 64; gd++)
+      s = __hip_atomic_load(&X[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s < g.r) {  // write nothing: the synchronous rounds finish it
+      open = true;
+      continue;
+    }
+    const long long dst = g.lo + g.c + 1 + (s - g.r);
+    keysB[dst] = keysA[j];
+    valsB[dst] = valsA[j];
+    if (single_right) {
+      keysF[dst] = keysA[j];
+      valsF[dst] = valsA[j];
+    }
+  }
+  if (open) atomicExch(&b.st->async_open, 1ull);
 }
 
 // Synchronous doubling round k (in place, same invariant as above); a
@@ -429,7 +462,7 @@ __global__ __launch_bounds__(256) void k_double_async(LevelBufs b, int* jmp, int
 __global__ __launch_bounds__(256) void k_double_round(LevelBufs b, int* jmp, int* round_open, int k) {
   if (k > 0 && !round_open[k - 1]) return;
   const long long t = blockIdx.x;
-  const int sg = find_seg(b.toff, b.nseg, t);
+  const int sg = b.tseg[t];
   if (!b.seg_need[sg]) return;
   const SegInfo g = b.info[sg];
   const long long q0 = (t - g.first_tile) * TILE;
@@ -465,7 +498,7 @@ __global__ void k_tile_chase2(const K* keysA, const int* valsA, K* keysB, int* v
                               int* valsF, LevelBufs b, const int* jmp, const int* s0,
                               int strict) {
   const long long t = blockIdx.x;
-  const int sg = find_seg(b.toff, b.nseg, t);
+  const int sg = b.tseg[t];
   if (!b.seg_need[sg]) return;
   const SegInfo g = b.info[sg];
   const K pivot = keysA[g.hi];
@@ -878,7 +911,8 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
                o_shi = take(maxsmall * 8), o_sbuf = take(maxsmall * 4),
                o_st = take(sizeof(LevelState)), o_round = take(64 * 4),
                o_arot = take((size_t)n * 4), o_s0 = take((size_t)n * 4),
-               o_need = take(maxseg * 4 + 64), o_jmp = take((size_t)n * 4 + 64);
+               o_need = take(maxseg * 4 + 64), o_jmp = take((size_t)n * 4 + 64),
+               o_tseg = take((maxtiles + 1) * 4);
   char* aux = (char*)ctx_buf(c, BUF_SORT_AUX, bytes);
   // work buffers: 1 = B, 2 = C; the output (0) is `keys`/`vals`
   K* kB = (K*)ctx_buf(c, BUF_SORT_K, (size_t)n * sizeof(K));
@@ -901,6 +935,7 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
   b.small_max = SMALL;
   b.info = (SegInfo*)(aux + o_info);
   b.seg_need = (int*)(aux + o_need);
+  b.tseg = (int*)(aux + o_tseg);
   b.st = st;
   b.floor = floor;
   int cur = 0;
@@ -988,9 +1023,8 @@ void lomuto_sort(exh_ctx* c, K* keys, int* vals, long long n, double floor) {
     // long chains (segments the chase flagged; the others return at once):
     // asynchronous pointer jumping, then the walk to the slots, before the
     // level's one host read
-    hipLaunchKernelGGL(k_double_async, dim3(nt), dim3(256), 0, c->stream, b, jmp, 64);
-    hipLaunchKernelGGL(k_tile_chase2<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd, keys,
-                       vals, b, jmp, s0, 0);
+    hipLaunchKernelGGL(k_double_walk<K>, dim3(nt), dim3(256), 0, c->stream, ks, vs, kd, vd, keys,
+                       vals, b, jmp, s0, 64);
     EXH_LAUNCH_CHECK();
     EXH_CHECK_HIP(hipMemcpyAsync(hs, st, sizeof(LevelState), hipMemcpyDeviceToHost, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
