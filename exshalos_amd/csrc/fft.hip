@@ -26,6 +26,7 @@
 // Direction and scaling follow rocFFT: forward uses exp(-2 pi i k n / N),
 // inverse exp(+2 pi i k n / N), neither is normalised.
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 #include "kspace.hpp"
@@ -238,12 +239,12 @@ struct OpPoisson {
 // column reads points j + r N/R, multiplies by W_{NS R}^{(j mod NS) r} and
 // writes points (j / NS) NS R + (j mod NS) + r NS.  The first stage reads
 // `gs` (through the load operator), the last writes `gd`.
-template <typename C2, int N, int R, int NS, int BZ, int LD, bool INV, bool FROM_GLOBAL,
+template <typename C2, int N, int R, int NS, int BZ, int LD, int TH, bool INV, bool FROM_GLOBAL,
           bool TO_GLOBAL, typename Op>
 __device__ __forceinline__ void fft_stage(const C2* gs, C2* gd, long long stride, bool ok,
                                           C2* __restrict__ lds, const C2* __restrict__ tw,
                                           int c, int grp, const Op& op) {
-  constexpr int G = AXIS_THREADS / BZ;  // threads per column
+  constexpr int G = TH / BZ;  // threads per column
   constexpr int NBF = N / R;            // butterflies per column
   constexpr int PER = (NBF + G - 1) / G;
   C2 v[PER][R];
@@ -293,16 +294,16 @@ __device__ __forceinline__ void fft_stage(const C2* gs, C2* gd, long long stride
   if constexpr (!TO_GLOBAL) __syncthreads();
 }
 
-template <typename C2, int N, int BZ, int LD, bool INV, int S, int NS, typename Op>
+template <typename C2, int N, int BZ, int LD, int TH, bool INV, int S, int NS, typename Op>
 __device__ __forceinline__ void fft_stages(const C2* gs, C2* gd, long long stride, bool ok,
                                            C2* __restrict__ lds, const C2* __restrict__ tw,
                                            int c, int grp, const Op& op) {
   constexpr int NST = fft_nstages<N>();
   constexpr int R = fft_radix<N>(S);
-  fft_stage<C2, N, R, NS, BZ, LD, INV, S == 0, S == NST - 1>(gs, gd, stride, ok, lds, tw, c, grp,
-                                                             op);
+  fft_stage<C2, N, R, NS, BZ, LD, TH, INV, S == 0, S == NST - 1>(gs, gd, stride, ok, lds, tw, c,
+                                                                 grp, op);
   if constexpr (S + 1 < NST)
-    fft_stages<C2, N, BZ, LD, INV, S + 1, NS * R>(gs, gd, stride, ok, lds, tw, c, grp, op);
+    fft_stages<C2, N, BZ, LD, TH, INV, S + 1, NS * R>(gs, gd, stride, ok, lds, tw, c, grp, op);
 }
 
 template <typename T, int N>
@@ -314,8 +315,8 @@ constexpr size_t fft_lds_bytes() {
 // Workgroup w runs on XCD w mod 8; each XCD gets a contiguous range of
 // (line, column block) pairs so that neighbouring blocks, which share
 // 128-byte lines when a row is not line-aligned, meet in one L2.
-template <typename T, int N, bool INV, typename Op>
-__global__ __launch_bounds__(AXIS_THREADS) void k_fft_axis(const typename cplx<T>::type* src,
+template <typename T, int N, bool INV, typename Op, int TH>
+__global__ __launch_bounds__(TH) void k_fft_axis(const typename cplx<T>::type* src,
                                                           typename cplx<T>::type* dst,
                                                           long long stride, long long ncol,
                                                           long long line_stride, int nbx,
@@ -335,13 +336,14 @@ __global__ __launch_bounds__(AXIS_THREADS) void k_fft_axis(const typename cplx<T
   const long long line = logical / nbx;
   const long long xb = logical - line * nbx;
   const int t = threadIdx.x, c = t % BZ, grp = t / BZ;
-  for (int i = t; i < N; i += AXIS_THREADS) tws[i] = tw[i];
+  for (int i = t; i < N; i += TH) tws[i] = tw[i];
   __syncthreads();
   const long long col = xb * BZ + c;
   const bool ok = col < ncol;
   const long long off = line * line_stride + (ok ? col : 0);
   op.bind(ok ? col : 0);
-  fft_stages<C2, N, BZ, LD, INV, 0, 1>(src + off, dst + off, stride, ok, lds, tws, c, grp, op);
+  fft_stages<C2, N, BZ, LD, TH, INV, 0, 1>(src + off, dst + off, stride, ok, lds, tws, c, grp,
+                                           op);
 }
 
 // W_N^k = exp(-2 pi i k / N), k < N, rounded from double; one table per
@@ -374,10 +376,14 @@ void launch_axis(exh_ctx* c, const void* src, void* dst, long long stride, long 
   const long long nblocks = (long long)nbx * nlines;
   EXH_REQUIRE(nblocks < (1LL << 31), EXH_ERR_ARG, "FFT pass too large");
   const size_t lds = fft_lds_bytes<T, N>();
-  auto kern = k_fft_axis<T, N, INV, Op>;
+  // the pass that applies the Poisson factor (double arithmetic per point)
+  // runs with twice the waves, which hide that work: 1LPT/2LPT inverse
+  // transforms at nd = 512 0.99 -> 0.86 ms; the plain passes gain nothing
+  constexpr int TH = std::is_same<Op, OpPoisson<T>>::value ? 2 * AXIS_THREADS : AXIS_THREADS;
+  auto kern = k_fft_axis<T, N, INV, Op, TH>;
   EXH_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-  hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AXIS_THREADS), lds, c->stream,
+  hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(TH), lds, c->stream,
                      (const typename cplx<T>::type*)src, (typename cplx<T>::type*)dst, stride,
                      ncol, line_stride, nbx, nblocks, twiddles<T>(c, N), op);
   EXH_LAUNCH_CHECK();
