@@ -183,6 +183,10 @@ int exh_ctx_set_tunable(exh_ctx* c, int key, int value) {
     EXH_REQUIRE(value == 0 || value == 1 || value == 4 || value == 8, EXH_ERR_ARG,
                 "deposit accumulator must be 0, 1, 4 or 8 (got %d)", value);
     c->tune_deposit_acc = value;
+  } else if (key == EXH_TUNE_HALO_CLAIMS) {
+    EXH_REQUIRE(value == 0 || value == 64, EXH_ERR_ARG, "halo claims must be 0 or 64 (got %d)",
+                value);
+    c->tune_halo_claims = value;
   } else {
     EXH_REQUIRE(false, EXH_ERR_ARG, "unknown tunable %d", key);
   }

@@ -166,6 +166,7 @@ struct exh_ctx {
   // (or its environment override)
   int tune_deposit_tile = 0;
   int tune_deposit_acc = 0;
+  int tune_halo_claims = 0;
   void* buf[exh::BUF_NSLOTS] = {};
   size_t cap[exh::BUF_NSLOTS] = {};
   // allocation number of each slot's current buffer (0: none yet)
@@ -175,6 +176,7 @@ struct exh_ctx {
   // round tag (epoch) used in them
   unsigned long long claim_gen = 0;
   unsigned claim_epoch = 0;
+  int claim_mode = 0;  // 32 or 64: the width of the claims in the array
   std::map<exh::FftKey, exh::FftPlan> plans;
   std::map<int, void*> fft_tw;  // twiddle tables of the strided FFT passes
   // per-kernel HIP-event timing (enabled by exh_ctx_set_timing)

@@ -347,7 +347,11 @@ __device__ inline int xcd_block(unsigned nb) {
 
 // The round's count is read on the device (*dn): the grid may be larger
 // (sized by an earlier round's count) and its extra blocks return.
-template <typename T>
+// Claims are 64-bit (round tag << 32 | ~rank) or, with fewer than 2^24
+// candidates, 32-bit (round tag << 24 | (2^24 - 1 - rank)): atomicMax keeps
+// the densest claim of the current round, and any claim of the round beats
+// those of earlier rounds.
+template <typename T, bool C32>
 __global__ void k_grow(GrowArgs<T> a, const long long* dn, unsigned long long tag_hi) {
   const int npend = (int)*dn;
   const unsigned nba = (unsigned)((npend + 255) / 256);
@@ -429,16 +433,21 @@ __global__ void k_grow(GrowArgs<T> a, const long long* dn, unsigned long long ta
   a.pcnt[t] = count;
   a.pr2[t] = grows_tmp;
   // claim every cell the final halo could occupy (r^2 < G)
-  const unsigned long long tag = tag_hi | (unsigned long long)(0xffffffffu - (unsigned)rank);
-  for (int o = 0; o < a.sh.start[G]; o++) {
-    size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
-    atomicMax(&a.claim[id], tag);
+  if constexpr (C32) {
+    unsigned* cl = reinterpret_cast<unsigned*>(a.claim);
+    const unsigned tag = (unsigned)tag_hi | (0xffffffu - (unsigned)rank);
+    for (int o = 0; o < a.sh.start[G]; o++)
+      atomicMax(&cl[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)], tag);
+  } else {
+    const unsigned long long tag = tag_hi | (unsigned long long)(0xffffffffu - (unsigned)rank);
+    for (int o = 0; o < a.sh.start[G]; o++)
+      atomicMax(&a.claim[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)], tag);
   }
 }
 
 // retry[t]: the rank again when candidate t must retry next round, else -1
 // (compacted in order, so the next round keeps this round's order).
-template <typename T>
+template <typename T, bool C32>
 __global__ void k_commit(GrowArgs<T> a, const long long* dn, unsigned long long tag_hi,
                          int* retry, int tmax) {
   const int npend = (int)*dn;
@@ -456,18 +465,34 @@ __global__ void k_commit(GrowArgs<T> a, const long long* dn, unsigned long long 
   int ci, cj, ck;
   decode_cell<T>(cell, a.n1, a.n2, ci, cj, ck);
   // safe iff no denser pending candidate claims a cell of the read ball
-  const unsigned lim = 0xffffffffu - (unsigned)rank;
   const int oend = a.sh.start[G + 1];
   bool blocked = false;
-  for (int ob = 0; ob < oend && !blocked; ob += 8) {
-    unsigned long long v[8];
-    const int nb = min(8, oend - ob);
+  if constexpr (C32) {
+    const unsigned* cl = reinterpret_cast<const unsigned*>(a.claim);
+    const unsigned lim = 0xffffffu - (unsigned)rank, th = (unsigned)tag_hi;
+    for (int ob = 0; ob < oend && !blocked; ob += 8) {
+      unsigned v[8];
+      const int nb = min(8, oend - ob);
 #pragma unroll
-    for (int e = 0; e < 8; e++)
-      v[e] = e < nb ? a.claim[cell_off(ci, cj, ck, a.sh.offs[ob + e], a.n0, a.n1, a.n2)] : 0ull;
+      for (int e = 0; e < 8; e++)
+        v[e] = e < nb ? cl[cell_off(ci, cj, ck, a.sh.offs[ob + e], a.n0, a.n1, a.n2)] : 0u;
 #pragma unroll
-    for (int e = 0; e < 8; e++)
-      blocked |= (v[e] & 0xffffffff00000000ull) == tag_hi && (unsigned)(v[e] & 0xffffffffu) > lim;
+      for (int e = 0; e < 8; e++)
+        blocked |= (v[e] & 0xff000000u) == th && (v[e] & 0xffffffu) > lim;
+    }
+  } else {
+    const unsigned lim = 0xffffffffu - (unsigned)rank;
+    for (int ob = 0; ob < oend && !blocked; ob += 8) {
+      unsigned long long v[8];
+      const int nb = min(8, oend - ob);
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        v[e] = e < nb ? a.claim[cell_off(ci, cj, ck, a.sh.offs[ob + e], a.n0, a.n1, a.n2)] : 0ull;
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        blocked |=
+            (v[e] & 0xffffffff00000000ull) == tag_hi && (unsigned)(v[e] & 0xffffffffu) > lim;
+    }
   }
   retry[t] = blocked ? rank : -1;
   if (blocked) return;
@@ -784,18 +809,6 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   int* flag = (int*)ctx_buf(c, BUF_FLAG, (size_t)ng * 4);
   T* eff = (T*)ctx_buf(c, BUF_EFF, (size_t)ng * sizeof(T));
   unsigned long long* claim = (unsigned long long*)ctx_buf(c, BUF_CLAIM, (size_t)ng * 8);
-  // Claims carry their round's tag (an epoch counted over the context's
-  // calls) in the high half, so the claims of earlier rounds and calls never
-  // match a later round: the array is cleared only when it is newly
-  // allocated or the epochs are about to run out.
-  if (c->claim_gen != c->buf_gen[BUF_CLAIM] || c->claim_epoch > 0xffffffffu - 2000000u) {
-    const long long ncap = (long long)(c->cap[BUF_CLAIM] / 8);
-    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncap, 256 * 8)), dim3(256), 0, c->stream, claim,
-                       ncap, 0ull);
-    EXH_LAUNCH_CHECK();
-    c->claim_gen = c->buf_gen[BUF_CLAIM];
-    c->claim_epoch = 0;
-  }
   {
     TimedScope ts(c, "peaks");
     hipLaunchKernelGGL(k_peak_count<T>, dim3((unsigned)nblk), dim3(256), 0, c->stream, delta, pg,
@@ -924,6 +937,26 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, (unsigned long long*)rt, 1,
                      (unsigned long long)nc);
   EXH_LAUNCH_CHECK();
+  // Claims carry their round's tag (an epoch counted over the context's
+  // calls) above the inverted rank, so the claims of earlier rounds and
+  // calls never match a later round: the array is cleared only when it is
+  // newly allocated, when the claim width changes, or before the epoch
+  // wraps (every 255 rounds with 32-bit claims).
+  const bool c32 = nc < (1LL << 24) && c->tune_halo_claims != 64;
+  const int cmode = c32 ? 32 : 64;
+  auto clear_claims = [&]() {
+    const long long ncap = (long long)(c->cap[BUF_CLAIM] / 8);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncap, 256 * 8)), dim3(256), 0, c->stream, claim,
+                       ncap, 0ull);
+    EXH_LAUNCH_CHECK();
+    c->claim_epoch = 0;
+  };
+  if (c->claim_gen != c->buf_gen[BUF_CLAIM] || c->claim_mode != cmode ||
+      c->claim_epoch > 0xffffffffu - 2000000u) {
+    clear_claims();
+    c->claim_gen = c->buf_gen[BUF_CLAIM];
+    c->claim_mode = cmode;
+  }
   long long ub = nc;
   int round = 0, launched = 0;
   {
@@ -935,15 +968,23 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
         const long long* dn = rt + launched % RT_RING;
         long long* dnext = rt + (launched + 1) % RT_RING;
         launched++;
-        const unsigned long long tag_hi = (unsigned long long)(++c->claim_epoch) << 32;
+        if (c32 && c->claim_epoch >= 255) clear_claims();
+        const unsigned long long ep = ++c->claim_epoch;
+        const unsigned long long tag_hi = c32 ? ep << 24 : ep << 32;
         ga.prank = pend;
         ga.pcell = qcell;
         ga.pden = qden;
-        hipLaunchKernelGGL(k_grow<T>, dim3(grid_for(ub, 256)), dim3(256), 0, c->stream, ga, dn,
-                           tag_hi);
-        EXH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_commit<T>, dim3(grid_for(ub, 256)), dim3(256), 0, c->stream, ga, dn,
-                           tag_hi, retry, tmax);
+        if (c32) {
+          hipLaunchKernelGGL((k_grow<T, true>), dim3(grid_for(ub, 256)), dim3(256), 0, c->stream,
+                             ga, dn, tag_hi);
+          hipLaunchKernelGGL((k_commit<T, true>), dim3(grid_for(ub, 256)), dim3(256), 0,
+                             c->stream, ga, dn, tag_hi, retry, tmax);
+        } else {
+          hipLaunchKernelGGL((k_grow<T, false>), dim3(grid_for(ub, 256)), dim3(256), 0, c->stream,
+                             ga, dn, tag_hi);
+          hipLaunchKernelGGL((k_commit<T, false>), dim3(grid_for(ub, 256)), dim3(256), 0,
+                             c->stream, ga, dn, tag_hi, retry, tmax);
+        }
         EXH_LAUNCH_CHECK();
         // candidates to retry, in this round's order
         const long long nbk = (ub + PK_TILE - 1) / PK_TILE;

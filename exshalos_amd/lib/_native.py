@@ -28,7 +28,7 @@ ERRORS = {
 }
 
 SRC_MODES, SRC_DELTA, SRC_DELTAK = 0, 1, 2
-TUNE_DEPOSIT_TILE, TUNE_DEPOSIT_ACC = 1, 2
+TUNE_DEPOSIT_TILE, TUNE_DEPOSIT_ACC, TUNE_HALO_CLAIMS = 1, 2, 3
 
 
 class Box(C.Structure):

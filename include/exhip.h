@@ -125,9 +125,12 @@ int exh_ctx_timings(exh_ctx* ctx, char* names, int names_len, double* ms, int ma
  * workgroup, 16 (16^3), 3216 (32 x 16 x 16) or 32 (32 x 32 x 16);
  * EXH_TUNE_DEPOSIT_ACC: the window's sums, 1 (32-bit fixed point, float
  * grids without masses), 4 (float) or 8 (double).  Results agree to the
- * deposit's float rounding whatever the setting. */
+ * deposit's float rounding whatever the setting.  EXH_TUNE_HALO_CLAIMS: the
+ * width of the halo growth rounds' per-cell claims, 0 (32 bits below 2^24
+ * candidates, else 64) or 64; the catalogue is the same either way. */
 #define EXH_TUNE_DEPOSIT_TILE 1
 #define EXH_TUNE_DEPOSIT_ACC 2
+#define EXH_TUNE_HALO_CLAIMS 3
 int exh_ctx_set_tunable(exh_ctx* ctx, int key, int value);
 /* Halo-finder statistics of the last run (5 values): peaks, candidates,
  * growth rounds, device-wide sort levels, sort levels that needed pointer

@@ -44,11 +44,28 @@ def test_density_grid(bbks, prec, nd):
     assert _rel(g["gridk"], dk) < (1e-6 if prec == 4 else 1e-14)
 
 
+@pytest.fixture
+def halo_claims():
+    """Sets the context's claim width (EXH_TUNE_HALO_CLAIMS) for one test and
+    restores the automatic choice afterwards."""
+    from exshalos_amd.lib import _native as N
+
+    def set_(v):
+        N.check(N.load().exh_ctx_set_tunable(N.context(), N.TUNE_HALO_CLAIMS, v),
+                "exh_ctx_set_tunable")
+
+    yield set_
+    set_(0)
+
+
+# 0: 32-bit claims (fewer than 2^24 candidates); 64: the wide claims
+@pytest.mark.parametrize("claims", [0, 64])
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("nd,seed", [(32, 1), (64, 1), (64, 7)])
-def test_find_halos_bit_exact(bbks, prec, nd, seed):
+def test_find_halos_bit_exact(bbks, prec, nd, seed, claims, halo_claims):
     k, P = bbks
     set_precision(prec)
+    halo_claims(claims)
     delta, _, _ = O.field_from_modes(k, P, nd, LC, _noise(nd, seed, prec), precision=prec)
     ref = O.find_halos(delta, k, P, LC, seed=12345, precision=prec)
     out = X.find_halos(delta, k, P, LC, 0.31, 0.0, -1.0, 1, 1.0, 0.0, 0.0, 1, 0, 0)
@@ -61,6 +78,35 @@ def test_find_halos_bit_exact(bbks, prec, nd, seed):
     R = np.float32 if prec == 4 else np.float64
     lag = (R(LC) * (ref["x"] + 0.5)).astype(R)
     np.testing.assert_array_equal(out["posh"], lag)
+
+
+def test_halo_claims_width_switch_and_epoch_wrap(bbks, halo_claims):
+    """The claim array keeps its contents between calls: switching the claim
+    width clears it, and 32-bit claims clear it again before their 8-bit
+    round tag wraps (every 255 rounds).  Calls on one box, with the width
+    switched on every fifth of the first twenty and then more than 300
+    rounds of 32-bit claims, must each give the oracle's catalogue."""
+    from exshalos_amd.lib import _native as N
+
+    k, P = bbks
+    set_precision(4)
+    delta, _, _ = O.field_from_modes(k, P, 32, LC, _noise(32, 3, 4), precision=4)
+    ref = O.find_halos(delta, k, P, LC, seed=12345, precision=4)
+    rounds32, call = 0, 0
+    # every round takes a tag, so more than 255 rounds of 32-bit claims
+    # after the last width switch wrap the tag at least once
+    while rounds32 <= 300 and call < 400:
+        wide = call % 5 == 4 and call < 20
+        halo_claims(64 if wide else 0)
+        out = X.find_halos(delta, k, P, LC, 0.31, 0.0, -1.0, 1, 1.0, 0.0, 0.0, 1, 0, 0)
+        cat = X.halo_catalogue()
+        np.testing.assert_array_equal(cat["x"], ref["x"])
+        np.testing.assert_array_equal(cat["count"], ref["count"])
+        np.testing.assert_array_equal(out["flag"].reshape(-1), ref["flag"].reshape(-1))
+        if call >= 20:
+            rounds32 += N.halo_stats(N.context())["rounds"]
+        call += 1
+    assert rounds32 > 300
 
 
 def test_find_halos_repeated_calls_mixed_sizes(bbks):
