@@ -377,9 +377,13 @@ void launch_axis(exh_ctx* c, const void* src, void* dst, long long stride, long 
   EXH_REQUIRE(nblocks < (1LL << 31), EXH_ERR_ARG, "FFT pass too large");
   const size_t lds = fft_lds_bytes<T, N>();
   // the pass that applies the Poisson factor (double arithmetic per point)
-  // runs with twice the waves, which hide that work: 1LPT/2LPT inverse
-  // transforms at nd = 512 0.99 -> 0.86 ms; the plain passes gain nothing
-  constexpr int TH = std::is_same<Op, OpPoisson<T>>::value ? 2 * AXIS_THREADS : AXIS_THREADS;
+  // runs with twice the waves where every thread still has radix-8
+  // butterflies (N / 8 >= threads per column), which hide that work: 1LPT
+  // and 2LPT inverse transforms at nd = 512 0.99 -> 0.86 ms; the plain
+  // passes gain nothing
+  constexpr bool WIDE =
+      std::is_same<Op, OpPoisson<T>>::value && N / 8 >= 2 * AXIS_THREADS / BZ;
+  constexpr int TH = WIDE ? 2 * AXIS_THREADS : AXIS_THREADS;
   auto kern = k_fft_axis<T, N, INV, Op, TH>;
   EXH_CHECK_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
