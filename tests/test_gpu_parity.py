@@ -193,10 +193,13 @@ def test_cached_tables_follow_pk(bbks, prec):
 
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("do2", [0, 1])
-def test_lpt(bbks, prec, do2):
+@pytest.mark.parametrize("nd", [32, 64])
+def test_lpt(bbks, prec, do2, nd):
+    """nd = 32 has partial (16 x 64) stencil tiles along k; nd = 64 only
+    full ones, where the FP32 2LPT source keeps its own cells in registers
+    and stages only the halos."""
     k, P = bbks
     set_precision(prec)
-    nd = 32
     noise = _noise(nd, 5, prec)
     delta, dk, p = O.field_from_modes(k, P, nd, LC, noise, precision=prec)
     S1, S2 = O.lpt(dk, p, 1e4, bool(do2), prec)
