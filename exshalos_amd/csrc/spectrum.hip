@@ -1107,8 +1107,42 @@ __device__ inline double legendre_d(int l, double x) {
   return p1;
 }
 
-__device__ inline double shfl_up_d(double v, int o) {
-  return __shfl_up(v, o, 64);
+// One DPP move of a double (its two halves) across the wave; lanes without
+// a source read 0.  Controls: row_shr:n = 0x110 + n (within rows of 16
+// lanes), row_bcast:15 = 0x142 (lane 15 of each row to the next row),
+// row_bcast:31 = 0x143 (lane 31 to rows 2 and 3).
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(
+      (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+
+// Inclusive segmented sum over a full wave: lane l gets x summed over lanes
+// sstart .. l (sstart <= l, the start of l's segment), in a fixed order:
+// shifts by 1, 2, 4 and 8 within rows of 16 lanes, then the last lane of
+// row 0 into row 1 and of row 2 into row 3, then lane 31 into rows 2 and 3,
+// each added only where the source lies in the lane's segment.  DPP moves
+// run on the vector ALU; the cross-lane shuffles they replace went through
+// the LDS crossbar.
+__device__ inline double seg_scan_d(double x, int lane, int sstart) {
+  const int rl = lane & 15, rs = lane & ~15;
+  double y;
+  y = dpp_d<0x111>(x);
+  if (rl >= 1 && lane - 1 >= sstart) x += y;
+  y = dpp_d<0x112>(x);
+  if (rl >= 2 && lane - 2 >= sstart) x += y;
+  y = dpp_d<0x114>(x);
+  if (rl >= 4 && lane - 4 >= sstart) x += y;
+  y = dpp_d<0x118>(x);
+  if (rl >= 8 && lane - 8 >= sstart) x += y;
+  y = dpp_d<0x142>(x);
+  if ((rs == 16 || rs == 48) && sstart < rs) x += y;
+  y = dpp_d<0x143>(x);
+  if (rs >= 32 && sstart < 32) x += y;
+  return x;
 }
 
 static constexpr int PW_MAXV = 64;  // max values per mode (pairs x multipoles + 1)
@@ -1284,10 +1318,7 @@ __global__ void k_power_bins(PowerArgs a, const typename cplx<T>::type* gk, doub
             x = (dr[l] * dr[m] + di[l] * di[m]) * legendre_d(l2, mu);
           }
         }
-        for (int o = 1; o < 64; o <<= 1) {
-          double y = shfl_up_d(x, o);
-          if (lane - o >= sstart) x += y;
-        }
+        x = seg_scan_d(x, lane, sstart);
         for (int p = 0; p < nparts; p++)
           if (tail && part == p) acc[(size_t)b * NV + v] += x;
       }
