@@ -659,13 +659,33 @@ __global__ void k_halo_posh(const int* hx, const int* hcount, const int* hr2, lo
     int ci = hx[3 * h], cj = hx[3 * h + 1], ck = hx[3 * h + 2];
     double s[3] = {0, 0, 0}, v[3] = {0, 0, 0};
     if (!lagrangian) {
-      int end = sh.start[hr2[h]];
-      for (int o = 0; o < end; o++) {
-        size_t id = cell_off(ci, cj, ck, sh.offs[o], n0, n1, n2);
-        for (int a = 0; a < 3; a++) {
-          s[a] += (double)S[3 * id + a];
-          if (V) v[a] += (double)V[3 * id + a];
-        }
+      // cells in batches of 8, each cell's three components in one load,
+      // all of a batch's loads in flight before the sums, which keep the
+      // cell order per component
+      using V3 = typename vec3<T>::type;
+      const int end = sh.start[hr2[h]];
+      for (int ob = 0; ob < end; ob += 8) {
+        const int nb = min(8, end - ob);
+        V3 sv[8], vv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+          if (e < nb) {
+            const size_t id = cell_off(ci, cj, ck, sh.offs[ob + e], n0, n1, n2);
+            sv[e] = *reinterpret_cast<const V3*>(S + 3 * id);
+            if (V) vv[e] = *reinterpret_cast<const V3*>(V + 3 * id);
+          }
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+          if (e < nb) {
+            s[0] += (double)sv[e].x;
+            s[1] += (double)sv[e].y;
+            s[2] += (double)sv[e].z;
+            if (V) {
+              v[0] += (double)vv[e].x;
+              v[1] += (double)vv[e].y;
+              v[2] += (double)vv[e].z;
+            }
+          }
       }
     }
     int q[3] = {ci, cj, ck};
