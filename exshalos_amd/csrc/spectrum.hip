@@ -746,6 +746,18 @@ __global__ void k_zero(T* g, size_t n) {
     g[i] = (T)0;
 }
 
+// The same for a 16-byte aligned array, with 16-byte stores (the values
+// past the last whole 16 bytes one by one).
+template <typename T>
+__global__ void k_zero16(T* g, size_t n) {
+  const size_t nv = n * sizeof(T) / 16;
+  const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  uint4* v = reinterpret_cast<uint4*>(g);
+  for (size_t i = t0; i < nv; i += st) v[i] = make_uint4(0u, 0u, 0u, 0u);
+  for (size_t i = nv * 16 / sizeof(T) + t0; i < n; i += st) g[i] = (T)0;
+}
+
 // grid (blockIdx.y) of ngrids: delta = rho / (M / ng) - 1
 template <typename T>
 __global__ void k_normalise(T* grid, size_t ng, int nsub, const unsigned long long* counts) {
@@ -957,8 +969,13 @@ void run_grid(exh_ctx* c, const T* pos, const T* vel, const T* mass, const int* 
   EXH_REQUIRE(ntype <= 64, EXH_ERR_ARG, "at most 64 tracer types");
   {
     TimedScope ts(c, "grid_zero");
-    hipLaunchKernelGGL(k_zero<T>, dim3(grid_for(ng * ntype * nsub, 256 * 8)), dim3(256), 0,
-                       c->stream, grid, ng * ntype * nsub);
+    const size_t nz = ng * ntype * nsub;
+    if (reinterpret_cast<uintptr_t>(grid) % 16 == 0)
+      hipLaunchKernelGGL(k_zero16<T>, dim3(grid_for(nz * sizeof(T) / 16 + 1, 256 * 4)),
+                         dim3(256), 0, c->stream, grid, nz);
+    else
+      hipLaunchKernelGGL(k_zero<T>, dim3(grid_for(nz, 256 * 8)), dim3(256), 0, c->stream, grid,
+                         nz);
     hipLaunchKernelGGL(k_zero<unsigned long long>, dim3(1), dim3(64), 0, c->stream, counts,
                        (size_t)64);
     EXH_LAUNCH_CHECK();
