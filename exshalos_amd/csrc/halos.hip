@@ -603,8 +603,8 @@ __device__ inline double spline_eval_d(const double* x, const double* y, const d
 // Mh = InvI(I(M_count) + (I(M_next) - I(M_count)) u)   (find_halos.c:455-476)
 template <typename T>
 __global__ void k_masses(const int* hcount, long long nh, const int* sphere, int nsph,
-                         int Ncells, T Mcell, const double* Mk, const double* Int,
-                         const double* mI, const double* mInv, int Nr, const double* u, T* Mh,
+                         int Ncells, const double* itab, const int* ibad, const double* Mk,
+                         const double* Int, const double* mInv, int Nr, const double* u, T* Mh,
                          int* err) {
   for (long long h = blockIdx.x * (long long)blockDim.x + threadIdx.x; h < nh;
        h += (long long)gridDim.x * blockDim.x) {
@@ -624,11 +624,27 @@ __global__ void k_masses(const int* hcount, long long nh, const int* sphere, int
       Mh[h] = (T)0;
       continue;
     }
-    T m0 = (T)cnt * Mcell, m1 = (T)sphere[lo + 1] * Mcell;
-    double I0 = spline_eval_d(Mk, Int, mI, Nr, (double)m0, &bad);
-    T v = (T)(I0 + (spline_eval_d(Mk, Int, mI, Nr, (double)m1, &bad) - I0) * u[h]);
+    // I of the masses of spheres lo and lo + 1, from the table
+    const double I0 = itab[lo];
+    T v = (T)(I0 + (itab[lo + 1] - I0) * u[h]);
+    bad = ibad[lo] | ibad[lo + 1];
     Mh[h] = (T)spline_eval_d(Int, Mk, mInv, Nr, (double)v, &bad);
     if (bad) atomicOr(err, 16);
+  }
+}
+
+// I(M) at the mass (T)sphere[q] * Mcell of every sphere size q < nt, as
+// k_masses would evaluate it for a halo of that size (a table, since most
+// halos share a few sizes); ibad[q] flags a mass outside the spline's range.
+template <typename T>
+__global__ void k_mass_table(const int* sphere, int nt, T Mcell, const double* Mk,
+                             const double* Int, const double* mI, int Nr, double* itab,
+                             int* ibad) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nt; q += gridDim.x * blockDim.x) {
+    int bad = 0;
+    const T m = (T)sphere[q] * Mcell;
+    itab[q] = spline_eval_d(Mk, Int, mI, Nr, (double)m, &bad);
+    ibad[q] = bad;
   }
 }
 
@@ -1058,6 +1074,10 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
   size_t o_hx = htake(hs * 12), o_hc = htake(hs * 4), o_hr = htake(hs * 4),
          o_mh = htake(hs * sizeof(T)), o_u = htake(hs * 8), o_ph = htake(hs * 3 * sizeof(T)),
          o_vh = htake(hs * 3 * sizeof(T));
+  // I(M) per sphere size for the masses: every index k_masses can read
+  const int nsph = (int)sph.size();
+  const int ntab = std::max(1, std::min(std::min(Ncells, nsph) + 1, nsph));
+  const size_t o_itab = htake((size_t)ntab * 8), o_ibad = htake((size_t)ntab * 4);
   char* hal = (char*)ctx_buf(c, BUF_HALO, hoff);
   S.hx = (int*)(hal + o_hx);
   S.hcount = (int*)(hal + o_hc);
@@ -1077,11 +1097,16 @@ long long run_find_halos(exh_ctx* c, const exh_box* b, const T* delta, const exh
     hipLaunchKernelGGL(k_taus, dim3(grid_for((nh + 63) / 64, 256)), dim3(256), 0, c->stream,
                        (const unsigned*)(tabs + o_mats), st[0], st[1], st[2], nh, u);
     EXH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_masses<T>, dim3(grid_for(nh, 256)), dim3(256), 0, c->stream, S.hcount, nh,
-                       (const int*)(tabs + o_sph), (int)sph.size(), Ncells, (T)b->Mcell,
+    double* itab = (double*)(hal + o_itab);
+    int* ibad = (int*)(hal + o_ibad);
+    hipLaunchKernelGGL(k_mass_table<T>, dim3(grid_for(ntab, 256)), dim3(256), 0, c->stream,
+                       (const int*)(tabs + o_sph), ntab, (T)b->Mcell,
                        (const double*)(tabs + o_mk), (const double*)(tabs + o_int),
-                       (const double*)(tabs + o_mi), (const double*)(tabs + o_minv), Nr, u,
-                       (T*)S.Mh, derr);
+                       (const double*)(tabs + o_mi), Nr, itab, ibad);
+    hipLaunchKernelGGL(k_masses<T>, dim3(grid_for(nh, 256)), dim3(256), 0, c->stream, S.hcount, nh,
+                       (const int*)(tabs + o_sph), nsph, Ncells, itab, ibad,
+                       (const double*)(tabs + o_mk), (const double*)(tabs + o_int),
+                       (const double*)(tabs + o_minv), Nr, u, (T*)S.Mh, derr);
     EXH_LAUNCH_CHECK();
     EXH_CHECK_HIP(hipMemcpyAsync(hbuf, derr, 4, hipMemcpyDeviceToHost, c->stream));
     EXH_CHECK_HIP(hipStreamSynchronize(c->stream));
