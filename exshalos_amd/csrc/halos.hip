@@ -230,12 +230,29 @@ __global__ void k_brick_scatter(const int* ccell, const T* cden, long long nc, i
   __syncthreads();
   const long long r0 = (long long)blockIdx.x * ORD_RB;
   const long long r1 = r0 + ORD_RB < nc ? r0 + ORD_RB : nc;
-  for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    const int cell = ccell[r];
-    const int pos = atomicAdd(&cur[brick_of(cell, n1, n2, bsh, nby)], 1);
-    prank[pos] = (int)r;
-    pcell[pos] = cell;
-    pden[pos] = cden[r];
+  // the block's cells and densities first, all loads in flight together
+  // (the stores below could alias them for the compiler); then the ranks
+  // in the same order as a strided loop would take them
+  constexpr int PER = ORD_RB / 256;
+  int cells[PER];
+  T dens[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const long long r = r0 + threadIdx.x + 256 * q;
+    if (r < r1) {
+      cells[q] = ccell[r];
+      dens[q] = cden[r];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const long long r = r0 + threadIdx.x + 256 * q;
+    if (r < r1) {
+      const int pos = atomicAdd(&cur[brick_of(cells[q], n1, n2, bsh, nby)], 1);
+      prank[pos] = (int)r;
+      pcell[pos] = cells[q];
+      pden[pos] = dens[q];
+    }
   }
 }
 
