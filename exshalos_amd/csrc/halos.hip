@@ -266,16 +266,30 @@ __global__ void k_retry_write(const int* retry, const long long* dn, const long 
   const long long n = *dn;
   if (blockIdx.x == 0 && threadIdx.x == 0) *dnext = boff[nbk];  // next round's count
   long long base = (long long)blockIdx.x * PK_TILE + (long long)threadIdx.x * PK_ITEMS;
+  // the thread's entries, cells and densities are all loaded before the
+  // first store (which could alias them for the compiler)
+  int rv[PK_ITEMS], cv[PK_ITEMS];
+  T dv[PK_ITEMS];
   long long cnt = 0;
+#pragma unroll
+  for (int q = 0; q < PK_ITEMS; q++) {
+    rv[q] = base + q < n ? retry[base + q] : -1;
+    cnt += rv[q] >= 0;
+  }
+#pragma unroll
   for (int q = 0; q < PK_ITEMS; q++)
-    if (base + q < n && retry[base + q] >= 0) cnt++;
+    if (rv[q] >= 0) {
+      cv[q] = cell_in[base + q];
+      dv[q] = den_in[base + q];
+    }
   long long tot;
   long long o = block_exclusive_256<long long>(cnt, sh, &tot) + boff[blockIdx.x];
+#pragma unroll
   for (int q = 0; q < PK_ITEMS; q++)
-    if (base + q < n && retry[base + q] >= 0) {
-      rank_out[o] = retry[base + q];
-      cell_out[o] = cell_in[base + q];
-      den_out[o] = den_in[base + q];
+    if (rv[q] >= 0) {
+      rank_out[o] = rv[q];
+      cell_out[o] = cv[q];
+      den_out[o] = dv[q];
       o++;
     }
 }
