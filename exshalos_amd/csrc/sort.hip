@@ -243,6 +243,7 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int*
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long long base = g.lo + (t - g.first_tile) * TILE + (long long)w * (TILE / 4);
   K kv[PER_THREAD];
+  int vv[PER_THREAD];
   unsigned long long bal[PER_THREAD];
   int wcnt = 0;
   for (int q = 0; q < PER_THREAD; q++) {
@@ -251,6 +252,12 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int*
     kv[q] = in ? keysA[j] : pivot;
     bal[q] = __ballot(in && kv[q] <= pivot);
     wcnt += __popcll(bal[q]);
+  }
+  // the "<=" elements' values, loaded before the first store (which could
+  // alias them for the compiler)
+  for (int q = 0; q < PER_THREAD; q++) {
+    const long long j = base + q * 64 + lane;
+    vv[q] = (bal[q] >> lane) & 1ull ? valsA[j] : 0;
   }
   if (lane == 0) shw[w] = wcnt;
   __syncthreads();
@@ -264,12 +271,11 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const K* keysA, const int*
     if (j < g.hi) {
       const long long rank = run + __popcll(bal[q] & below);
       if ((bal[q] >> lane) & 1ull) {
-        const int vv = valsA[j];
         keysB[g.lo + rank] = kv[q];
-        valsB[g.lo + rank] = vv;
+        valsB[g.lo + rank] = vv[q];
         if (single_left) {
           keysF[g.lo] = kv[q];
-          valsF[g.lo] = vv;
+          valsF[g.lo] = vv[q];
         }
         if (g.r > 0 && j > g.j0) arot[g.lo + rank - jlead] = (int)((j - g.lo) - rank);
       } else {
@@ -337,23 +343,30 @@ __global__ __launch_bounds__(256) void k_tile_chase(const K* keysA, const int* v
   // steps still needed by an unfinished chain: every later step advances
   // by at least A(s), since A never decreases along the stream
   unsigned long long rem = 0;
+  // what the stores below need, loaded before the first of them (which
+  // could alias it for the compiler): a finished element's value, an open
+  // chain's next rotation entry
+  int ld[PER_THREAD];
+  for (int q = 0; q < PER_THREAD; q++) {
+    const long long j = start + (long long)q * 256 + threadIdx.x;
+    ld[q] = !act[q] ? 0 : (s[q] < r ? A[s[q]] : valsA[j]);
+  }
   for (int q = 0; q < PER_THREAD; q++) {
     if (!act[q]) continue;
     const long long j = start + (long long)q * 256 + threadIdx.x;
     if (s[q] < r) {
       s0[j] = s[q];
-      const int a = max(A[s[q]], 1);
+      const int a = max(ld[q], 1);
       rem = max(rem, (unsigned long long)((r - s[q] + a - 1) / a));
       continue;
     }
     s0[j] = S0_DONE;
     const long long dst = g.lo + g.c + 1 + (s[q] - r);
-    const int vv = valsA[j];
     keysB[dst] = kv[q];
-    valsB[dst] = vv;
+    valsB[dst] = ld[q];
     if (single_right) {
       keysF[dst] = kv[q];
-      valsF[dst] = vv;
+      valsF[dst] = ld[q];
     }
   }
   __shared__ unsigned long long sh_rem[4];
