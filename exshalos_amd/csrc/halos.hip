@@ -134,6 +134,12 @@ __global__ __launch_bounds__(256) void k_peak_write(const T* d, PeakGeom g, cons
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned base = blockIdx.x * (unsigned)PK_TILE + threadIdx.x;
   const unsigned bits = pbits[blockIdx.x * 256u + threadIdx.x];
+  // the peaks' densities, loaded before the first store (which could alias
+  // them for the compiler)
+  T dv[PK_ITEMS];
+#pragma unroll
+  for (int q = 0; q < PK_ITEMS; q++)
+    dv[q] = (bits >> q) & 1u ? d[base + (unsigned)q * 256u] : (T)0;
 #pragma unroll
   for (int q = 0; q < PK_ITEMS; q++) {
     const unsigned c = base + (unsigned)q * 256u;
@@ -154,7 +160,7 @@ __global__ __launch_bounds__(256) void k_peak_write(const T* d, PeakGeom g, cons
     if ((bal[q] >> lane) & 1ull) {
       const long long at = ow + __popcll(bal[q] & below);
       const unsigned c = base + (unsigned)q * 256u;
-      key[at] = d[c];
+      key[at] = dv[q];
       cell[at] = (int)c;
     }
     o += wcnt[q][0] + wcnt[q][1] + wcnt[q][2] + wcnt[q][3];
@@ -567,15 +573,15 @@ __global__ void k_gather_halos(const unsigned char* kept, const long long* hidx,
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nc;
        r += (long long)gridDim.x * blockDim.x) {
     if (!kept[r]) continue;
-    long long h = hidx[r];
-    int cell = ccell[r];
+    const long long h = hidx[r];
+    const int cell = ccell[r], hc = cnt[r], hr = r2[r];  // all loads before the stores
     int i, j, k;
     decode_cell<T>(cell, n1, n2, i, j, k);
     hx[3 * h] = i;
     hx[3 * h + 1] = j;
     hx[3 * h + 2] = k;
-    hcount[h] = cnt[r];
-    hr2[h] = r2[r];
+    hcount[h] = hc;
+    hr2[h] = hr;
   }
 }
 
