@@ -514,6 +514,23 @@ k_grad_tiles(TileGeom g, const T* phi, double inv60,
     ps.load(phi + (size_t)i * plane, hv);
     const T* pp = phi + (size_t)wrapi(i + 3, g.n0) * plane;
     for (int q = 0; q < ST_Q; q++) win[q][6] = pp[wo[q]];
+    // the second-order pass adds to S (and V): for float fields on full
+    // tiles the plane's values of the thread's cells are loaded here, before
+    // any store of the plane (which could alias them for the compiler); the
+    // other variants keep their registers (and occupancy) and load them
+    // below
+    using V3 = typename vec3<T>::type;
+    constexpr bool PRE = MODE == 2 && sizeof(T) == 4 && FULL;
+    V3 s_old[PRE ? ST_Q : 1], v_old[PRE ? ST_Q : 1];
+    if (PRE)
+      for (int q = 0; q < ST_Q; q++) {
+        const int j = j0 + ty + 4 * q;
+        if (kin && j < g.n1) {
+          const size_t c = ((size_t)i * g.n1 + j) * g.n2 + k;
+          s_old[q] = *reinterpret_cast<const V3*>(S + 3 * c);
+          if (V) v_old[q] = *reinterpret_cast<const V3*>(V + 3 * c);
+        }
+      }
     __syncthreads();  // previous plane's readers are done
     ps.store(hv, pl);
     if (FULL)
@@ -529,7 +546,6 @@ k_grad_tiles(TileGeom g, const T* phi, double inv60,
       const double gz = d6w(rowj[-3], rowj[-2], rowj[-1], rowj[1], rowj[2], rowj[3]);
       if (kin && j < g.n1) {
         // the cell's three components move as one 3-vector access
-        using V3 = typename vec3<T>::type;
         const size_t c = ((size_t)i * g.n1 + j) * g.n2 + k;
         V3* Sp = reinterpret_cast<V3*>(S + 3 * c);
         V3* Vp = V ? reinterpret_cast<V3*>(V + 3 * c) : nullptr;
@@ -542,14 +558,14 @@ k_grad_tiles(TileGeom g, const T* phi, double inv60,
         } else {
           const T sx = (T)(coef * gx * inv60), sy = (T)(coef * gy * inv60),
                   sz = (T)(coef * gz * inv60);
-          V3 o = *Sp;
+          V3 o = PRE ? s_old[PRE ? q : 0] : *Sp;
           o.x += sx;
           o.y += sy;
           o.z += sz;
           *Sp = o;
           snew = o;
           if (Vp) {
-            V3 u = *Vp;
+            V3 u = PRE ? v_old[PRE ? q : 0] : *Vp;
             u.x += fv * sx;
             u.y += fv * sy;
             u.z += fv * sz;
