@@ -114,11 +114,9 @@ __device__ inline T mode_amp(const FieldArgs<T>& a, const SplineView& sp, int i,
 // Generated (non-copied) mode at (i, j, k): A(|k|) * noise, with the
 // reference's float/double placement (density_grid.c:152-198).
 template <typename T>
-__device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j,
-                                      int k, T& re, T& im) {
+__device__ inline void generated_mode_g(const FieldArgs<T>& a, const SplineView& sp, int i, int j,
+                                        int k, T g0, T g1, T& re, T& im) {
   const T A = mode_amp(a, sp, i, j, k);
-  size_t id = ((size_t)i * a.n1 + j) * (size_t)a.nz2 + k;
-  T g0 = a.noise[2 * id], g1 = a.noise[2 * id + 1];
   bool selfconj = (i == 0 || i == a.n0 / 2) && (j == 0 || j == a.n1 / 2) &&
                   (k == 0 || k == a.n2 / 2);
   if (selfconj) {
@@ -138,6 +136,14 @@ __device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& s
   }
 }
 
+// The same with the mode's noise read here.
+template <typename T>
+__device__ inline void generated_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j,
+                                      int k, T& re, T& im) {
+  const size_t id = ((size_t)i * a.n1 + j) * (size_t)a.nz2 + k;
+  generated_mode_g(a, sp, i, j, k, a.noise[2 * id], a.noise[2 * id + 1], re, im);
+}
+
 // True where the kz = 0, nd/2 planes copy the conjugate of the mirror mode
 // (-i, -j) instead of drawing one (density_grid.c:109-150).
 template <typename T>
@@ -147,13 +153,15 @@ __device__ inline bool copied_mode(const FieldArgs<T>& a, int i, int j, int k) {
          (j == 0 || j == a.n1 / 2) || (i > a.n0 / 2 && j < a.n1 / 2);
 }
 
-// Mode (i, j, k) of the field including the Hermitian copy rule of the
-// kz = 0 and kz = nd/2 planes (density_grid.c:109-150).
+// The Hermitian copy rule of the kz = 0 and kz = nd/2 planes
+// (density_grid.c:109-150): mode (i, j, k) is generated mode (pi, pj, k),
+// conjugated where cj.
 template <typename T>
-__device__ inline void field_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j, int k,
-                                  T& re, T& im) {
-  int pi = i, pj = j;
-  bool cj = false;
+__device__ inline void mode_source(const FieldArgs<T>& a, int i, int j, int k, int& pi, int& pj,
+                                   bool& cj) {
+  pi = i;
+  pj = j;
+  cj = false;
   if ((k == 0 || k == a.n2 / 2) && (i > a.n0 / 2 || j > a.n1 / 2)) {
     if (i > a.n0 / 2 && j > a.n1 / 2) {
       pi = a.n0 - i;
@@ -171,6 +179,15 @@ __device__ inline void field_mode(const FieldArgs<T>& a, const SplineView& sp, i
       cj = true;
     }
   }
+}
+
+// Mode (i, j, k) of the field including that copy rule.
+template <typename T>
+__device__ inline void field_mode(const FieldArgs<T>& a, const SplineView& sp, int i, int j, int k,
+                                  T& re, T& im) {
+  int pi, pj;
+  bool cj;
+  mode_source(a, i, j, k, pi, pj, cj);
   generated_mode(a, sp, pi, pj, k, re, im);
   if (cj) im = -im;
 }
@@ -202,14 +219,33 @@ __global__ __launch_bounds__(256) void k_field(FieldArgs<T> a, T scale, FastDiv 
   extern __shared__ double spl[];
   const SplineView sp = stage_spline<STAGE>(a, spl);
   const size_t nh = (size_t)a.n0 * a.n1 * a.nz2;
-  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh;
-       id += (size_t)gridDim.x * blockDim.x) {
-    int i, j, k;
-    mode_coords(id, nh, a.n1, a.nz2, dz, dy, i, j, k);
-    T re, im;
-    field_mode(a, sp, i, j, k, re, im);
-    if (raw) raw[id] = {re, im};
-    out[id] = {re * scale, im * scale};
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  // two modes per iteration, both modes' noise loaded before either's stores
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < nh; id += 2 * st) {
+    const size_t idb = id + st;
+    const bool two = idb < nh;
+    int i[2], j[2], k[2], pi[2], pj[2];
+    bool cj[2];
+    T g0[2], g1[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (u == 1 && !two) break;
+      mode_coords(u ? idb : id, nh, a.n1, a.nz2, dz, dy, i[u], j[u], k[u]);
+      mode_source(a, i[u], j[u], k[u], pi[u], pj[u], cj[u]);
+      const size_t src = ((size_t)pi[u] * a.n1 + pj[u]) * (size_t)a.nz2 + k[u];
+      g0[u] = a.noise[2 * src];
+      g1[u] = a.noise[2 * src + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (u == 1 && !two) break;
+      T re, im;
+      generated_mode_g(a, sp, pi[u], pj[u], k[u], g0[u], g1[u], re, im);
+      if (cj[u]) im = -im;
+      const size_t o = u ? idb : id;
+      if (raw) raw[o] = {re, im};
+      out[o] = {re * scale, im * scale};
+    }
   }
 }
 
