@@ -469,16 +469,28 @@ __global__ void k_grow(GrowArgs<T> a, const long long* dn, unsigned long long ta
   a.pG[t] = G;
   a.pcnt[t] = count;
   a.pr2[t] = grows_tmp;
-  // claim every cell the final halo could occupy (r^2 < G)
-  if constexpr (C32) {
-    unsigned* cl = reinterpret_cast<unsigned*>(a.claim);
-    const unsigned tag = (unsigned)tag_hi | (0xffffffu - (unsigned)rank);
-    for (int o = 0; o < a.sh.start[G]; o++)
-      atomicMax(&cl[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)], tag);
-  } else {
-    const unsigned long long tag = tag_hi | (unsigned long long)(0xffffffffu - (unsigned)rank);
-    for (int o = 0; o < a.sh.start[G]; o++)
-      atomicMax(&a.claim[cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2)], tag);
+  // claim every cell the final halo could occupy (r^2 < G); the offsets
+  // of 8 cells are loaded before their atomics (which could alias them for
+  // the compiler)
+  const int oend = a.sh.start[G];
+  for (int ob = 0; ob < oend; ob += 8) {
+    const int nb = min(8, oend - ob);
+    unsigned off[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      if (e < nb) off[e] = a.sh.offs[ob + e];
+    if constexpr (C32) {
+      unsigned* cl = reinterpret_cast<unsigned*>(a.claim);
+      const unsigned tag = (unsigned)tag_hi | (0xffffffu - (unsigned)rank);
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < nb) atomicMax(&cl[cell_off(ci, cj, ck, off[e], a.n0, a.n1, a.n2)], tag);
+    } else {
+      const unsigned long long tag = tag_hi | (unsigned long long)(0xffffffffu - (unsigned)rank);
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (e < nb) atomicMax(&a.claim[cell_off(ci, cj, ck, off[e], a.n0, a.n1, a.n2)], tag);
+    }
   }
 }
 
@@ -546,10 +558,20 @@ __global__ void k_commit(GrowArgs<T> a, const long long* dn, unsigned long long 
   }
   bool keep = count >= a.Nmin;
   int v = keep ? (rank << 1) : FLAG_REJECTED;
-  for (int o = 0; o < a.sh.start[R2]; o++) {
-    const size_t id = cell_off(ci, cj, ck, a.sh.offs[o], a.n0, a.n1, a.n2);
-    a.flag[id] = v;
-    a.eff[id] = a.negMtot;
+  const int fend = a.sh.start[R2];
+  for (int ob = 0; ob < fend; ob += 8) {  // offsets of 8 cells before their stores
+    const int nb = min(8, fend - ob);
+    unsigned off[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      if (e < nb) off[e] = a.sh.offs[ob + e];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      if (e < nb) {
+        const size_t id = cell_off(ci, cj, ck, off[e], a.n0, a.n1, a.n2);
+        a.flag[id] = v;
+        a.eff[id] = a.negMtot;
+      }
   }
   if (keep) a.flag[cell] = (rank << 1) | 1;
   a.kept[rank] = keep ? 1 : 0;
